@@ -1,0 +1,104 @@
+/*
+ * CPU oracle of the uDeviceX DPD per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * it is loaded by tests/, by the smoke check of __graft_entry__.py and by the
+ * cpu_baseline leg of bench.py, never by the product library (ctc_amd/).
+ *
+ * Every function restates the reference algorithm from its source; the
+ * reference file:line is cited next to each declaration.  Arithmetic is IEEE
+ * FP32 with the operation order spelled out (compile with -ffp-contract=off),
+ * so that the HIP kernels, which use the same order, can be compared bit for
+ * bit where the operations are correctly rounded.
+ */
+#ifndef UDX_ORACLE_H
+#define UDX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- KISS trunk generator: src/math/rnd/imp.cpp:9-34 ------------------- */
+typedef struct { uint32_t x, y, z, c; } orc_kiss;
+void     orc_kiss_ini(orc_kiss *r, int x, int y, int z, int c);
+uint32_t orc_kiss_get_int(orc_kiss *r);
+float    orc_kiss_get(orc_kiss *r);
+void     orc_kiss_fill(int x, int y, int z, int c, int n, float *out);
+
+/* ---- branch generator: src/math/rnd/dev.h:27-72 ------------------------- */
+float orc_fma_rz(float a, float b, float c);      /* exact emulation       */
+float orc_fma_rz_fenv(float a, float b, float c); /* fesetround + fmaf     */
+float orc_mean0var1ii(float seed, int u, int v);
+float orc_mean0var1uu(float seed, uint32_t u, uint32_t v);
+void  orc_rng_ii_batch(float seed, int n, const int *u, const int *v, float *out);
+void  orc_rng_uu_batch(float seed, int n, const uint32_t *u, const uint32_t *v, float *out);
+void  orc_fma_rz_batch(int n, const float *a, const float *b, const float *c, float *out, int fenv);
+
+/* ---- per-step seeds ----------------------------------------------------- */
+/* 27 trunks: index fid (0..25) are the halo fragment trunks of
+ * src/fluforces/imp/ini.h:22-58 on a single rank, index 26 is the bulk trunk
+ * (0,0,0,0) of src/fluforces/imp/ini.h:6.  One draw per trunk per step
+ * (src/fluforces/bulk/imp/main.h:34, src/fluforces/imp/main.h:27-31). */
+typedef struct { orc_kiss t[27]; } orc_seeds;
+void orc_seeds_ini(orc_seeds *s);
+void orc_seeds_next(orc_seeds *s, float out[27]);
+void orc_seeds_table(int nsteps, float *out /* nsteps*27 */);
+
+/* ---- fragments: src/frag/dev.h:19-81 ------------------------------------ */
+void orc_frag_i2d3(int fid, int d[3]);
+int  orc_frag_d2i(int x, int y, int z);
+int  orc_frag_anti(int fid);
+/* per-fragment exclusive prefix of cell counts in fragment-cell order
+ * (src/exch/flu/dev/map.h:1-98).  fss_off[27] gives each fragment's offset
+ * into fss; fragment fid holds ncell(fid)+1 entries. */
+int  orc_frag_table_size(const int L[3]);
+void orc_frag_tables(const int L[3], const int *counts, int *fss, int *fss_off);
+
+/* ---- gen_random: pre/units/gen_random.cpp:13-33 ------------------------- */
+/* drand48 from glibc's zero initial state; text round trip through "%f"
+ * (pre/units/common/dump.cpp:8-15) when roundtrip != 0. */
+void orc_gen_random(int X, int Y, int Z, int n, int roundtrip, float *pp);
+
+/* ---- cell list: src/clist/dev/main.h:17-101, src/clist/imp/main.h:75-84 -- */
+/* Stable counting sort by cell (ties keep input order).  Particles outside
+ * [-L/2, L/2) are dropped.  starts has ncells+1 entries.  Returns nout. */
+int orc_clist_build(const int L[3], int n, const float *pp_in, int *starts, int *counts,
+                    float *pp_out, int *order);
+int orc_cell_of(const int L[3], const float r[3], int c[3]);
+
+/* ---- DPD pair force: src/pair/dev.h:12-47,110-139 ------------------------ */
+typedef struct { float a, g, s, spow; } orc_dpd;
+float orc_dpd_sigma(float kBT, float g, float dt);  /* src/pair/imp/main.h:54-62 */
+/* returns 1 if r2 < 1 (a pair), 0 otherwise; f is written in both cases */
+int orc_pair_force(const orc_dpd *p, const float *a, const float *b, float rnd, float *f);
+
+/* ---- DPD forces (full-shell gather, defined order) ---------------------- */
+/* pp: cell-ordered AoS particles (r[3], v[3]); seeds[27] as above.
+ * periodic=1: neighbour cells wrap, wrapped pairs use the halo RNG of
+ * src/fluforces/halo/dev/main.h:7-12 with fragment positions;
+ * periodic=0: cells outside the box are skipped (bulk-only unit harness,
+ * src/u/bulk/main.cu). Forces are ADDED to ff.  Returns the number of ordered
+ * pairs (each unordered pair counts twice). */
+long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                    orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff);
+
+/* ---- integrator: src/scheme/move/dev/vv.h:1-10 + body force + wrap ------ */
+void orc_integrate(const int L[3], int n, float dt, float mass, const float accel[3],
+                   const float *ff, float *pp);
+
+/* ---- diagnostics: src/io/diag/part/imp/main.h:37-63 (FP64) -------------- */
+/* out: [0..2] momentum (sum v), [3] kBT = 0.5*sum|v|^2/(1.5 n), [4] max |v|^2 */
+void orc_diag(int n, const float *pp, double out[5]);
+
+/* ---- one full step: clist, forces, integrate --------------------------- */
+/* pp (n particles, any order) is replaced by the cell-ordered, integrated
+ * state; returns the number of unordered pairs of the step. */
+long orc_flu_step(const int L[3], int n, float *pp, float *scratch_pp, int *starts, int *counts,
+                  float *ff, orc_dpd p, const float seeds[27], float dt, float mass,
+                  const float accel[3], int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
