@@ -1,0 +1,490 @@
+// C ABI of libudx (declared in include/udx.h): handles, host-side KISS seed
+// draws, and the per-step pipeline of the solvent-only solver.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "udx_internal.h"
+
+namespace udx {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+// KISS trunk (src/math/rnd/imp.cpp:21-34)
+static uint32_t kiss_int(udx_rnd *r) {
+    uint64_t t, a = 698769069ULL;
+    r->x = 69069u * r->x + 12345u;
+    r->y ^= (r->y << 13);
+    r->y ^= (r->y >> 17);
+    r->y ^= (r->y << 5);
+    t = a * (uint64_t)r->z + (uint64_t)r->c;
+    r->c = (uint32_t)(t >> 32);
+    r->z = (uint32_t)t;
+    return r->x + r->y + r->z;
+}
+
+// the 27 per-step trunks: halo fragments 0..25 (src/fluforces/imp/ini.h:11-58,
+// one rank: seed_base = 0) and the bulk trunk (0,0,0,0) at index 26
+struct Trunks {
+    udx_rnd t[27];
+    void ini() {
+        for (int f = 0; f < 26; f++) {
+            int d[3] = {(f + 2) % 3 - 1, (f / 3 + 2) % 3 - 1, (f / 9 + 2) % 3 - 1};
+            int sum = d[0] + d[1] + d[2];
+            int plus = (sum > 0) || (sum == 0 && d[0] > 0) || (d[0] == 0 && d[1] > 0) || (d[1] == 0 && d[2] > 0);
+            int sign = 2 * plus - 1, v[3];
+            for (int c = 0; c < 3; c++) v[c] = 1 + sign * d[c];
+            int seed = v[0] + 3 * (v[1] + 3 * v[2]);
+            udx_rnd_ini(390 + seed, seed + 615, 12309, 23094, &t[f]);
+        }
+        udx_rnd_ini(0, 0, 0, 0, &t[26]);
+    }
+    void next(float out[27]) {
+        for (int f = 0; f < 27; f++) out[f] = udx_rnd_get(&t[f]);
+    }
+};
+
+static Box make_box(int Lx, int Ly, int Lz) {
+    Box b;
+    b.L[0] = Lx;
+    b.L[1] = Ly;
+    b.L[2] = Lz;
+    return b;
+}
+
+}  // namespace udx
+
+using namespace udx;
+
+// ------------------------------------------------------------------ handles
+struct udx_clist {
+    udx_ctx *ctx;
+    ClistBufs c;
+};
+
+struct udx_fluforces {
+    udx_ctx *ctx;
+    Box box;
+    int maxp, n, exact;
+    float4 *pos, *vel;
+    float *seeds_dev;
+    FragTables frag;
+    Trunks trunks;
+    unsigned long long *pairs_dev;
+};
+
+struct udx_flu {
+    udx_ctx *ctx;
+    udx_flu_conf conf;
+    Box box;
+    int n;
+    float4 *pos[2], *vel[2], *ff;
+    int cur;
+    ClistBufs cl;
+    FragTables frag;
+    Trunks trunks;
+    float *seeds_dev;   // seed_cap x 27
+    int seed_cap;
+    float *seeds_host;  // pinned staging
+    unsigned long long *pairs_dev;
+    int timing;
+    hipEvent_t ev[4];
+    float tms[4];
+};
+
+extern "C" {
+
+const char *udx_last_error(void) { return g_err; }
+
+void udx_rnd_ini(int x, int y, int z, int c, udx_rnd *r) {
+    r->x = (uint32_t)x;
+    r->y = (uint32_t)y;
+    r->z = (uint32_t)z;
+    r->c = (uint32_t)c;
+}
+
+float udx_rnd_get(udx_rnd *r) {
+    uint32_t i = kiss_int(r);
+    return (float)i / (float)UINT32_MAX;
+}
+
+void udx_pair_compute_dpd_sigma(float kBT, float dt, udx_pair_dpd *p) { p->s = sqrtf(2 * kBT * p->g / dt); }
+
+int udx_ctx_ini(void *stream, udx_ctx **ctx) {
+    UDX_CHECK(ctx, UDX_EINVAL, "udx_ctx_ini: null output");
+    int dev = 0;
+    UDX_HIP(hipGetDevice(&dev));
+    udx_ctx *c = new udx_ctx;
+    c->stream = (hipStream_t)stream;
+    c->device = dev;
+    *ctx = c;
+    return UDX_OK;
+}
+
+int udx_ctx_fin(udx_ctx *ctx) {
+    delete ctx;
+    return UDX_OK;
+}
+
+int udx_ctx_sync(udx_ctx *ctx) {
+    UDX_HIP(hipStreamSynchronize(ctx->stream));
+    return UDX_OK;
+}
+
+int udx_rnd_branch(udx_ctx *ctx, float seed, int n, const int *u, const int *v, int uu, float *out) {
+    UDX_CHECK(ctx && n >= 0, UDX_EINVAL, "udx_rnd_branch: bad arguments");
+    return rnd_branch(ctx->stream, seed, n, u, v, uu, out);
+}
+
+// ------------------------------------------------------------------ clist
+int udx_clist_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_clist **out) {
+    UDX_CHECK(ctx && out, UDX_EINVAL, "udx_clist_ini: null argument");
+    UDX_CHECK(Lx > 0 && Ly > 0 && Lz > 0 && maxp >= 0, UDX_EINVAL, "udx_clist_ini: bad box %d %d %d / maxp %d",
+              Lx, Ly, Lz, maxp);
+    udx_clist *c = new udx_clist;
+    c->ctx = ctx;
+    int s = clist_alloc(&c->c, make_box(Lx, Ly, Lz), maxp);
+    if (s) {
+        clist_free(&c->c);
+        delete c;
+        return s;
+    }
+    *out = c;
+    return UDX_OK;
+}
+
+int udx_clist_fin(udx_clist *c) {
+    if (c) clist_free(&c->c);
+    delete c;
+    return UDX_OK;
+}
+
+int udx_clist_build(udx_clist *c, int n, const float *pp, float *ppout, int *nout) {
+    UDX_CHECK(c, UDX_EINVAL, "udx_clist_build: null clist");
+    hipStream_t s = c->ctx->stream;
+    UDX_TRY(clist_sort_aos(s, &c->c, n, pp));
+    if (ppout) UDX_TRY(gather_aos(s, &c->c, n, pp, ppout));
+    if (nout) {
+        UDX_HIP(hipMemcpyAsync(nout, c->c.starts + c->c.box.ncells(), sizeof(int), hipMemcpyDeviceToHost, s));
+        UDX_HIP(hipStreamSynchronize(s));
+    }
+    return UDX_OK;
+}
+
+const int *udx_clist_starts(const udx_clist *c) { return c->c.starts; }
+const int *udx_clist_counts(const udx_clist *c) { return c->c.counts; }
+const int *udx_clist_order(const udx_clist *c) { return c->c.order; }
+
+// ------------------------------------------------------------------ fluforces
+int udx_fluforces_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_fluforces **out) {
+    UDX_CHECK(ctx && out, UDX_EINVAL, "udx_fluforces_ini: null argument");
+    UDX_CHECK(Lx > 0 && Ly > 0 && Lz > 0 && maxp >= 0, UDX_EINVAL, "udx_fluforces_ini: bad box or capacity");
+    udx_fluforces *f = new udx_fluforces();
+    f->ctx = ctx;
+    f->box = make_box(Lx, Ly, Lz);
+    f->maxp = maxp;
+    f->trunks.ini();
+    int s = UDX_OK;
+    if (!s) s = dev_alloc(&f->pos, maxp);
+    if (!s) s = dev_alloc(&f->vel, maxp);
+    if (!s) s = dev_alloc(&f->seeds_dev, 27);
+    if (!s) s = dev_alloc(&f->pairs_dev, 1);
+    if (!s) s = frag_alloc(&f->frag, f->box);
+    if (!s && hipMemset(f->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess) {
+        set_error("udx_fluforces_ini: memset failed");
+        s = UDX_EHIP;
+    }
+    if (s) {
+        udx_fluforces_fin(f);
+        return s;
+    }
+    *out = f;
+    return UDX_OK;
+}
+
+int udx_fluforces_fin(udx_fluforces *f) {
+    if (!f) return UDX_OK;
+    dev_free(f->pos);
+    dev_free(f->vel);
+    dev_free(f->seeds_dev);
+    dev_free(f->pairs_dev);
+    frag_free(&f->frag);
+    delete f;
+    return UDX_OK;
+}
+
+int udx_fluforces_set_mode(udx_fluforces *f, int exact) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_fluforces_set_mode: null handle");
+    f->exact = exact ? 1 : 0;
+    return UDX_OK;
+}
+
+int udx_fluforces_prepare(udx_fluforces *f, int n, const float *pp) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_fluforces_prepare: null handle");
+    UDX_CHECK(n >= 0 && n <= f->maxp, UDX_ECAP, "udx_fluforces_prepare: %d particles exceed capacity %d", n,
+              f->maxp);
+    f->n = n;
+    return aos_to_f4(f->ctx->stream, n, pp, f->pos, f->vel);
+}
+
+int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
+                              const float seeds[27], float *ff) {
+    UDX_CHECK(f && p && starts && seeds, UDX_EINVAL, "udx_fluforces_apply_seeds: null argument");
+    hipStream_t s = f->ctx->stream;
+    if (f->n == 0) return UDX_OK;
+    UDX_HIP(hipMemcpyAsync(f->seeds_dev, seeds, 27 * sizeof(float), hipMemcpyHostToDevice, s));
+    UDX_TRY(frag_build(s, &f->frag, f->box, starts));
+    DpdLaunch a;
+    a.pos = f->pos;
+    a.vel = f->vel;
+    a.starts = starts;
+    a.frag = &f->frag;
+    a.box = f->box;
+    a.nmax = f->n;
+    a.periodic = periodic;
+    a.par = *p;
+    a.seeds = f->seeds_dev;
+    a.ff = ff;
+    a.ff_stride = 3;
+    a.accumulate = 1;
+    a.pairs = f->pairs_dev;
+    UDX_TRY(dpd_forces_mode(s, a, f->exact));
+    // the seeds live on the host stack: wait for the copy before returning
+    UDX_HIP(hipStreamSynchronize(s));
+    return UDX_OK;
+}
+
+int udx_fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic, float *ff) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_fluforces_apply: null handle");
+    float seeds[27];
+    f->trunks.next(seeds);
+    return udx_fluforces_apply_seeds(f, p, starts, periodic, seeds, ff);
+}
+
+int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset) {
+    UDX_CHECK(f && npairs, UDX_EINVAL, "udx_fluforces_pairs: null argument");
+    unsigned long long v = 0;
+    UDX_HIP(hipMemcpyAsync(&v, f->pairs_dev, sizeof v, hipMemcpyDeviceToHost, f->ctx->stream));
+    UDX_HIP(hipStreamSynchronize(f->ctx->stream));
+    *npairs = v / 2;
+    if (reset) UDX_HIP(hipMemsetAsync(f->pairs_dev, 0, sizeof v, f->ctx->stream));
+    return UDX_OK;
+}
+
+// ------------------------------------------------------------------ scheme
+int udx_scheme_move_apply(udx_ctx *ctx, float dt, float mass, int n, const float *ff, float *pp) {
+    UDX_CHECK(ctx, UDX_EINVAL, "udx_scheme_move_apply: null context");
+    return move_aos(ctx->stream, n, dt, mass, ff, pp);
+}
+
+int udx_bforce_apply_cste(udx_ctx *ctx, float mass, const float accel[3], int n, float *ff) {
+    UDX_CHECK(ctx && accel, UDX_EINVAL, "udx_bforce_apply_cste: null argument");
+    return bforce_aos(ctx->stream, n, mass, accel, ff);
+}
+
+int udx_wrap_pp(udx_ctx *ctx, int Lx, int Ly, int Lz, int n, float *pp) {
+    UDX_CHECK(ctx, UDX_EINVAL, "udx_wrap_pp: null context");
+    return wrap_aos(ctx->stream, make_box(Lx, Ly, Lz), n, pp);
+}
+
+int udx_diag_part(udx_ctx *ctx, int n, const float *pp, double out[5]) {
+    UDX_CHECK(ctx && out, UDX_EINVAL, "udx_diag_part: null argument");
+    double *part = nullptr;
+    UDX_TRY(dev_alloc(&part, 5 * 1024 + 5));
+    int s = diag_aos(ctx->stream, n, pp, part, out);
+    dev_free(part);
+    return s;
+}
+
+// ------------------------------------------------------------------ flu
+int udx_flu_fin(udx_flu *f);
+
+int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
+    UDX_CHECK(ctx && conf && out, UDX_EINVAL, "udx_flu_ini: null argument");
+    const int *L = conf->L;
+    UDX_CHECK(L[0] >= 3 && L[1] >= 3 && L[2] >= 3, UDX_EINVAL,
+              "udx_flu_ini: a periodic box needs at least 3 cells per side (got %d %d %d)", L[0], L[1], L[2]);
+    UDX_CHECK(conf->maxp > 0 && conf->mass > 0, UDX_EINVAL, "udx_flu_ini: bad capacity or mass");
+    udx_flu *f = new udx_flu();
+    f->ctx = ctx;
+    f->conf = *conf;
+    f->box = make_box(L[0], L[1], L[2]);
+    f->trunks.ini();
+    f->seed_cap = 0;
+    int s = UDX_OK;
+    for (int b = 0; b < 2 && !s; b++) {
+        s = dev_alloc(&f->pos[b], conf->maxp);
+        if (!s) s = dev_alloc(&f->vel[b], conf->maxp);
+    }
+    if (!s) s = dev_alloc(&f->ff, conf->maxp);
+    if (!s) s = clist_alloc(&f->cl, f->box, conf->maxp);
+    if (!s) s = frag_alloc(&f->frag, f->box);
+    if (!s) s = dev_alloc(&f->pairs_dev, 1);
+    if (!s && hipMemset(f->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess) {
+        set_error("udx_flu_ini: memset failed");
+        s = UDX_EHIP;
+    }
+    for (int e = 0; e < 4 && !s; e++)
+        if (hipEventCreate(&f->ev[e]) != hipSuccess) {
+            set_error("udx_flu_ini: event create failed");
+            s = UDX_EHIP;
+        }
+    if (s) {
+        udx_flu_fin(f);
+        return s;
+    }
+    *out = f;
+    return UDX_OK;
+}
+
+int udx_flu_fin(udx_flu *f) {
+    if (!f) return UDX_OK;
+    for (int b = 0; b < 2; b++) {
+        dev_free(f->pos[b]);
+        dev_free(f->vel[b]);
+    }
+    dev_free(f->ff);
+    clist_free(&f->cl);
+    frag_free(&f->frag);
+    dev_free(f->seeds_dev);
+    dev_free(f->pairs_dev);
+    if (f->seeds_host) (void)hipHostFree(f->seeds_host);
+    for (int e = 0; e < 4; e++)
+        if (f->ev[e]) (void)hipEventDestroy(f->ev[e]);
+    delete f;
+    return UDX_OK;
+}
+
+int udx_flu_set_pp(udx_flu *f, int n, const float *pp) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_pp: null handle");
+    UDX_CHECK(n >= 0 && n <= f->conf.maxp, UDX_ECAP, "udx_flu_set_pp: %d particles exceed capacity %d", n,
+              f->conf.maxp);
+    f->n = n;
+    f->cur = 0;
+    return aos_to_f4(f->ctx->stream, n, pp, f->pos[0], f->vel[0]);
+}
+
+int udx_flu_get_pp(udx_flu *f, float *pp, int *n) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_get_pp: null handle");
+    if (n) *n = f->n;
+    return f4_to_aos(f->ctx->stream, f->n, nullptr, f->pos[f->cur], f->vel[f->cur], pp);
+}
+
+int udx_flu_get_ff(udx_flu *f, float *ff) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_get_ff: null handle");
+    return f4_to_force_aos(f->ctx->stream, f->n, nullptr, f->ff, ff);
+}
+
+static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
+    hipStream_t s = f->ctx->stream;
+    if (nsteps > f->seed_cap) {
+        UDX_HIP(hipStreamSynchronize(s));
+        dev_free(f->seeds_dev);
+        if (f->seeds_host) (void)hipHostFree(f->seeds_host);
+        f->seeds_host = nullptr;
+        f->seed_cap = 0;
+        UDX_TRY(dev_alloc(&f->seeds_dev, 27 * (size_t)nsteps));
+        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, 27 * sizeof(float) * (size_t)nsteps));
+        f->seed_cap = nsteps;
+    } else {
+        // the staging buffer may still feed an earlier copy
+        UDX_HIP(hipStreamSynchronize(s));
+    }
+    for (int k = 0; k < nsteps; k++) {
+        if (seeds_host)
+            memcpy(f->seeds_host + 27 * k, seeds_host + 27 * k, 27 * sizeof(float));
+        else
+            f->trunks.next(f->seeds_host + 27 * k);
+    }
+    UDX_HIP(hipMemcpyAsync(f->seeds_dev, f->seeds_host, 27 * sizeof(float) * (size_t)nsteps,
+                           hipMemcpyHostToDevice, s));
+    return UDX_OK;
+}
+
+// one step of src/sim/imp/run.h:8-68 for the solvent: cell list + reorder,
+// DPD forces, body force + kick-drift + periodic wrap
+static int flu_step(udx_flu *f, const float *seeds_dev) {
+    hipStream_t s = f->ctx->stream;
+    const int nc = f->box.ncells();
+    const int a = f->cur, b = 1 - f->cur;
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[0], s));
+    UDX_TRY(clist_sort(s, &f->cl, f->n, f->pos[a]));
+    UDX_TRY(gather_f4(s, &f->cl, f->n, f->pos[a], f->vel[a], f->pos[b], f->vel[b]));
+    UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts));
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[1], s));
+    DpdLaunch d;
+    d.pos = f->pos[b];
+    d.vel = f->vel[b];
+    d.starts = f->cl.starts;
+    d.frag = &f->frag;
+    d.box = f->box;
+    d.nmax = f->n;
+    d.periodic = 1;
+    d.par = f->conf.dpd;
+    d.seeds = seeds_dev;
+    d.ff = (float *)f->ff;
+    d.ff_stride = 4;
+    d.accumulate = 0;
+    d.pairs = f->pairs_dev;
+    UDX_TRY(dpd_forces_mode(s, d, f->conf.exact));
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
+    UDX_TRY(move_f4(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff, f->pos[b],
+                    f->vel[b]));
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
+    f->cur = b;
+    return UDX_OK;
+}
+
+int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
+    UDX_CHECK(f && nsteps >= 0, UDX_EINVAL, "udx_flu_run: bad arguments");
+    if (nsteps == 0) return UDX_OK;
+    UDX_TRY(flu_seed_table(f, nsteps, seeds_host));
+    float acc[3] = {0, 0, 0};
+    for (int k = 0; k < nsteps; k++) {
+        UDX_TRY(flu_step(f, f->seeds_dev + 27 * k));
+        if (f->timing) {
+            UDX_HIP(hipEventSynchronize(f->ev[3]));
+            float t;
+            UDX_HIP(hipEventElapsedTime(&t, f->ev[0], f->ev[1]));
+            acc[0] += t;
+            UDX_HIP(hipEventElapsedTime(&t, f->ev[1], f->ev[2]));
+            acc[1] += t;
+            UDX_HIP(hipEventElapsedTime(&t, f->ev[2], f->ev[3]));
+            acc[2] += t;
+        }
+    }
+    if (f->timing)
+        for (int c = 0; c < 3; c++) f->tms[c] = acc[c] / nsteps;
+    f->tms[3] = f->tms[0] + f->tms[1] + f->tms[2];
+    return UDX_OK;
+}
+
+int udx_flu_pairs(udx_flu *f, unsigned long long *npairs, int reset) {
+    UDX_CHECK(f && npairs, UDX_EINVAL, "udx_flu_pairs: null argument");
+    unsigned long long v = 0;
+    UDX_HIP(hipMemcpyAsync(&v, f->pairs_dev, sizeof v, hipMemcpyDeviceToHost, f->ctx->stream));
+    UDX_HIP(hipStreamSynchronize(f->ctx->stream));
+    *npairs = v / 2;
+    if (reset) UDX_HIP(hipMemsetAsync(f->pairs_dev, 0, sizeof v, f->ctx->stream));
+    return UDX_OK;
+}
+
+int udx_flu_timing(udx_flu *f, int enable, float out[4]) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_timing: null handle");
+    f->timing = enable ? 1 : 0;
+    if (out)
+        for (int c = 0; c < 4; c++) out[c] = f->tms[c];
+    return UDX_OK;
+}
+
+}  // extern "C"

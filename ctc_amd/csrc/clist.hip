@@ -1,0 +1,262 @@
+// Cell list for MI355X: counting sort of particles into unit cells.
+//
+// The reference assigns each particle a slot with atomicAdd (so its in-cell
+// order depends on thread timing, src/clist/dev/main.h:44-68) and builds the
+// start table with a three-kernel uchar scan (src/algo/scan/dev.h).  Here the
+// in-cell order is defined: ties keep the input order (a stable counting
+// sort).  Atomics still hand out provisional slots; a per-cell insertion
+// sort of the (few) slot owners makes the result independent of timing.
+// Start/count tables come from a wavefront scan (64-lane shuffles, one LDS
+// exchange per block).
+#include "udx_dev.h"
+#include "udx_internal.h"
+
+namespace udx {
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// block-wide inclusive scan of one value per thread (blockDim = 256)
+__device__ __forceinline__ int block_inclusive_scan(int v, int *lds_waves) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_inclusive_scan(v);
+    if (lane == 63) lds_waves[wave] = v;
+    __syncthreads();
+    int add = 0;
+    for (int w = 0; w < wave; w++) add += lds_waves[w];
+    __syncthreads();
+    return v + add;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *in, int n, int *part) {
+    __shared__ int lw[4];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) s += (base + k < n) ? in[base + k] : 0;
+    s = block_inclusive_scan(s, lw);
+    if (threadIdx.x == SCAN_THREADS - 1) part[blockIdx.x] = s;
+}
+
+// exclusive scan of the per-tile sums by one block; part_ex[nb] = total
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(const int *part, int nb, int *part_ex) {
+    __shared__ int lw[4];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += SCAN_THREADS) {
+        int idx = b0 + threadIdx.x;
+        int v = idx < nb ? part[idx] : 0;
+        int inc = block_inclusive_scan(v, lw);
+        int c = carry;
+        if (idx < nb) part_ex[idx] = c + inc - v;
+        __syncthreads();
+        if (threadIdx.x == SCAN_THREADS - 1) carry = c + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part_ex[nb] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int *in, int n, const int *part_ex,
+                                                             int nb, int *out) {
+    __shared__ int lw[4];
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int inc = block_inclusive_scan(s, lw);
+    int run = part_ex[blockIdx.x] + inc - s;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part_ex[nb];
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---- counting sort --------------------------------------------------------
+struct PosF4 {
+    const float4 *p;
+    __device__ float3 operator()(int i) const { float4 q = p[i]; return make_float3(q.x, q.y, q.z); }
+};
+struct PosAos {
+    const float *p;
+    __device__ float3 operator()(int i) const {
+        const float *q = p + 6 * (long)i;
+        return make_float3(q[0], q[1], q[2]);
+    }
+};
+
+template <typename Pos>
+__global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int Ly, int Lz, int *counts,
+                                                     int2 *cellslot) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float3 r = pos(i);
+    int cx, cy, cz;
+    bool ok = cell_coord(r.x, Lx, &cx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz);
+    int2 cs = make_int2(-1, 0);
+    if (ok) {
+        int cid = cx + Lx * (cy + Ly * cz);
+        cs = make_int2(cid, atomicAdd(counts + cid, 1));
+    }
+    cellslot[i] = cs;
+}
+
+__global__ __launch_bounds__(256) void k_clist_scatter(int n, const int2 *cellslot, const int *starts,
+                                                       int *order) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int2 cs = cellslot[i];
+    if (cs.x >= 0) order[starts[cs.x] + cs.y] = i;
+}
+
+// sort each cell's input indices ascending: the stable order
+__global__ __launch_bounds__(256) void k_clist_cellsort(int ncells, const int *starts, int *order) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncells) return;
+    int s = starts[c], e = starts[c + 1];
+    for (int k = s + 1; k < e; k++) {
+        int v = order[k], m = k - 1;
+        while (m >= s && order[m] > v) {
+            order[m + 1] = order[m];
+            m--;
+        }
+        order[m + 1] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather_f4(int nmax, const int *nkept, const int *order,
+                                                   const float4 *pin, const float4 *vin, float4 *pout,
+                                                   float4 *vout) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmax || k >= *nkept) return;
+    int i = order[k];
+    pout[k] = pin[i];
+    vout[k] = vin[i];
+}
+
+__global__ __launch_bounds__(256) void k_gather_aos(int nmax, const int *nkept, const int *order,
+                                                    const float2 *pin, float2 *pout) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmax || k >= *nkept) return;
+    long i = order[k];
+    pout[3 * (long)k + 0] = pin[3 * i + 0];
+    pout[3 * (long)k + 1] = pin[3 * i + 1];
+    pout[3 * (long)k + 2] = pin[3 * i + 2];
+}
+
+}  // namespace
+
+int scan_scratch_size(int n) {
+    int nb = ceil_div(n, SCAN_TILE);
+    return 2 * nb + 2;
+}
+
+int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch) {
+    int nb = ceil_div(n, SCAN_TILE);
+    if (nb == 0) nb = 1;
+    int *part = scratch, *part_ex = scratch + nb;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part);
+    UDX_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_THREADS), 0, s, part, nb, part_ex);
+    UDX_LAUNCHED();
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part_ex, nb, out);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+int clist_alloc(ClistBufs *c, const Box &b, int maxp) {
+    *c = ClistBufs{};
+    c->box = b;
+    c->maxp = maxp;
+    int nc = b.ncells();
+    UDX_TRY(dev_alloc(&c->counts, nc));
+    UDX_TRY(dev_alloc(&c->starts, nc + 1));
+    UDX_TRY(dev_alloc(&c->cellslot, maxp));
+    UDX_TRY(dev_alloc(&c->order, maxp));
+    UDX_TRY(dev_alloc(&c->scan_tmp, scan_scratch_size(nc)));
+    return UDX_OK;
+}
+
+void clist_free(ClistBufs *c) {
+    dev_free(c->counts);
+    dev_free(c->starts);
+    dev_free(c->cellslot);
+    dev_free(c->order);
+    dev_free(c->scan_tmp);
+}
+
+template <typename Pos>
+static int clist_sort_impl(hipStream_t s, ClistBufs *c, int n, Pos pos) {
+    UDX_CHECK(n >= 0 && n <= c->maxp, UDX_ECAP, "clist: %d particles exceed capacity %d", n, c->maxp);
+    const Box &b = c->box;
+    int nc = b.ncells();
+    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c->counts, nc, 0);
+    UDX_LAUNCHED();
+    if (n > 0) {
+        hipLaunchKernelGGL(k_clist_count<Pos>, dim3(ceil_div(n, 256)), dim3(256), 0, s, pos, n, b.L[0],
+                           b.L[1], b.L[2], c->counts, c->cellslot);
+        UDX_LAUNCHED();
+    }
+    UDX_TRY(scan_exclusive(s, c->counts, nc, c->starts, c->scan_tmp));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_clist_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c->cellslot,
+                           c->starts, c->order);
+        UDX_LAUNCHED();
+        hipLaunchKernelGGL(k_clist_cellsort, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, c->starts,
+                           c->order);
+        UDX_LAUNCHED();
+    }
+    return UDX_OK;
+}
+
+int clist_sort(hipStream_t s, ClistBufs *c, int n, const float4 *pos4) {
+    return clist_sort_impl(s, c, n, PosF4{pos4});
+}
+
+int clist_sort_aos(hipStream_t s, ClistBufs *c, int n, const float *pp) {
+    return clist_sort_impl(s, c, n, PosAos{pp});
+}
+
+int gather_f4(hipStream_t s, const ClistBufs *c, int n, const float4 *pin, const float4 *vin, float4 *pout,
+              float4 *vout) {
+    if (n <= 0) return UDX_OK;
+    const int *nkept = c->starts + c->box.ncells();
+    hipLaunchKernelGGL(k_gather_f4, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, nkept, c->order, pin, vin,
+                       pout, vout);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+int gather_aos(hipStream_t s, const ClistBufs *c, int n, const float *pin, float *pout) {
+    if (n <= 0) return UDX_OK;
+    const int *nkept = c->starts + c->box.ncells();
+    hipLaunchKernelGGL(k_gather_aos, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, nkept, c->order,
+                       (const float2 *)pin, (float2 *)pout);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+}  // namespace udx

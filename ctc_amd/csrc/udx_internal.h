@@ -1,0 +1,135 @@
+// Host-side internals of libudx: contexts, error reporting, device buffers and
+// the kernel launchers each module exports to the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/udx.h"
+
+namespace udx {
+
+void set_error(const char *fmt, ...);
+
+#define UDX_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            ::udx::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #call,               \
+                             hipGetErrorString(e_));                                   \
+            return UDX_EHIP;                                                           \
+        }                                                                              \
+    } while (0)
+
+#define UDX_CHECK(cond, code, ...)                                                     \
+    do {                                                                               \
+        if (!(cond)) {                                                                 \
+            ::udx::set_error(__VA_ARGS__);                                             \
+            return (code);                                                             \
+        }                                                                              \
+    } while (0)
+
+#define UDX_TRY(call)                                                                  \
+    do {                                                                               \
+        int s_ = (call);                                                               \
+        if (s_ != UDX_OK) return s_;                                                   \
+    } while (0)
+
+// launch-error check after a kernel launch
+#define UDX_LAUNCHED() UDX_HIP(hipGetLastError())
+
+template <typename T>
+int dev_alloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    UDX_HIP(hipMalloc((void **)p, n * sizeof(T)));
+    return UDX_OK;
+}
+
+template <typename T>
+void dev_free(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ grid
+struct Box {
+    int L[3];
+    int ncells() const { return L[0] * L[1] * L[2]; }
+};
+
+// ------------------------------------------------------------------ scan
+// exclusive scan of n ints into out[0..n] (out[n] = total); scratch holds
+// scan_scratch_size(n) ints
+int scan_scratch_size(int n);
+int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch);
+
+// ------------------------------------------------------------------ cell list
+struct ClistBufs {
+    Box box;
+    int maxp;
+    int *counts;     // ncells
+    int *starts;     // ncells + 1 (starts[ncells] = number kept)
+    int2 *cellslot;  // maxp: (cell, provisional slot)
+    int *order;      // maxp: input index of each output slot
+    int *scan_tmp;
+};
+int clist_alloc(ClistBufs *c, const Box &b, int maxp);
+void clist_free(ClistBufs *c);
+// pos4 (input, n particles): counting sort into cell order; order[] and
+// starts[] filled; the number kept is starts[ncells] on the device
+int clist_sort(hipStream_t s, ClistBufs *c, int n, const float4 *pos4);
+int clist_sort_aos(hipStream_t s, ClistBufs *c, int n, const float *pp);
+// out[k] = in[order[k]] for k < starts[ncells]; n bounds the launch
+int gather_f4(hipStream_t s, const ClistBufs *c, int n, const float4 *pin, const float4 *vin,
+              float4 *pout, float4 *vout);
+int gather_aos(hipStream_t s, const ClistBufs *c, int n, const float *pin, float *pout);
+
+// ------------------------------------------------------------------ forces
+struct FragTables {
+    int *fss;        // concatenated per-fragment exclusive scans
+    int off[27];     // offsets of each fragment's table
+    int size;
+};
+int frag_alloc(FragTables *t, const Box &b);
+void frag_free(FragTables *t);
+int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *starts);
+
+struct DpdLaunch {
+    const float4 *pos, *vel;
+    const int *starts;      // ncells + 1, starts[ncells] = n
+    const FragTables *frag;
+    Box box;
+    int nmax;               // launch bound (>= particles)
+    int periodic;
+    udx_pair_dpd par;
+    const float *seeds;     // device, 27 floats
+    float *ff;              // AoS Force (stride 3) or float4 (stride 4)
+    int ff_stride;
+    int accumulate;
+    unsigned long long *pairs;   // ordered-pair counter or null
+};
+int dpd_forces(hipStream_t s, const DpdLaunch &a);
+int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int exact);
+
+// ------------------------------------------------------------------ scheme
+int aos_to_f4(hipStream_t s, int n, const float *pp, float4 *pos, float4 *vel);
+int f4_to_aos(hipStream_t s, int n, const int *n_dev, const float4 *pos, const float4 *vel, float *pp);
+int f4_to_force_aos(hipStream_t s, int n, const int *n_dev, const float4 *ff, float *out);
+int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass,
+            const float accel[3], const float4 *ff, float4 *pos, float4 *vel);
+int move_aos(hipStream_t s, int n, float dt, float mass, const float *ff, float *pp);
+int bforce_aos(hipStream_t s, int n, float mass, const float accel[3], float *ff);
+int wrap_aos(hipStream_t s, const Box &b, int n, float *pp);
+int rnd_branch(hipStream_t s, float seed, int n, const int *u, const int *v, int uu, float *out);
+int diag_aos(hipStream_t s, int n, const float *pp, double *partials_dev, double out[5]);
+
+}  // namespace udx
+
+struct udx_ctx {
+    hipStream_t stream;
+    int device;
+};

@@ -1,0 +1,303 @@
+"""ctypes bindings of libudx.so and thin Python mirrors of the reference's
+module interfaces (clist, fluforces, scheme, diag, solvent solver).
+
+Device memory and streams come from torch (ROCm); the arithmetic is entirely
+in the HIP kernels of libudx.  Particle tensors are float32 (n, 6) rows
+(x, y, z, vx, vy, vz), forces float32 (n, 3), as in src/inc/type.h:6-22.
+"""
+import ctypes as C
+import os
+
+from . import LIB_PATH
+
+_lib = None
+
+
+class UdxError(RuntimeError):
+    pass
+
+
+class Rnd(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("z", C.c_uint32), ("c", C.c_uint32)]
+
+
+class PairDpd(C.Structure):
+    _fields_ = [("a", C.c_float), ("g", C.c_float), ("s", C.c_float), ("spow", C.c_float)]
+
+
+class FluConf(C.Structure):
+    _fields_ = [("L", C.c_int * 3), ("maxp", C.c_int), ("dpd", PairDpd), ("dt", C.c_float),
+                ("mass", C.c_float), ("accel", C.c_float * 3), ("exact", C.c_int)]
+
+
+_VP, _I, _F = C.c_void_p, C.c_int, C.c_float
+_SIGS = {
+    "udx_last_error": (C.c_char_p, []),
+    "udx_rnd_ini": (None, [_I, _I, _I, _I, C.POINTER(Rnd)]),
+    "udx_rnd_get": (_F, [C.POINTER(Rnd)]),
+    "udx_pair_compute_dpd_sigma": (None, [_F, _F, C.POINTER(PairDpd)]),
+    "udx_ctx_ini": (_I, [_VP, C.POINTER(_VP)]),
+    "udx_ctx_fin": (_I, [_VP]),
+    "udx_ctx_sync": (_I, [_VP]),
+    "udx_rnd_branch": (_I, [_VP, _F, _I, _VP, _VP, _I, _VP]),
+    "udx_clist_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "udx_clist_fin": (_I, [_VP]),
+    "udx_clist_build": (_I, [_VP, _I, _VP, _VP, C.POINTER(_I)]),
+    "udx_clist_starts": (_VP, [_VP]),
+    "udx_clist_counts": (_VP, [_VP]),
+    "udx_clist_order": (_VP, [_VP]),
+    "udx_fluforces_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "udx_fluforces_fin": (_I, [_VP]),
+    "udx_fluforces_prepare": (_I, [_VP, _I, _VP]),
+    "udx_fluforces_apply": (_I, [_VP, C.POINTER(PairDpd), _VP, _I, _VP]),
+    "udx_fluforces_apply_seeds": (_I, [_VP, C.POINTER(PairDpd), _VP, _I, C.POINTER(_F), _VP]),
+    "udx_fluforces_set_mode": (_I, [_VP, _I]),
+    "udx_fluforces_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
+    "udx_scheme_move_apply": (_I, [_VP, _F, _F, _I, _VP, _VP]),
+    "udx_bforce_apply_cste": (_I, [_VP, _F, C.POINTER(_F), _I, _VP]),
+    "udx_wrap_pp": (_I, [_VP, _I, _I, _I, _I, _VP]),
+    "udx_diag_part": (_I, [_VP, _I, _VP, C.POINTER(C.c_double)]),
+    "udx_flu_ini": (_I, [_VP, C.POINTER(FluConf), C.POINTER(_VP)]),
+    "udx_flu_fin": (_I, [_VP]),
+    "udx_flu_set_pp": (_I, [_VP, _I, _VP]),
+    "udx_flu_get_pp": (_I, [_VP, _VP, C.POINTER(_I)]),
+    "udx_flu_get_ff": (_I, [_VP, _VP]),
+    "udx_flu_run": (_I, [_VP, _I, C.POINTER(_F)]),
+    "udx_flu_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
+    "udx_flu_timing": (_I, [_VP, _I, C.POINTER(_F)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def lib():
+    """Load libudx.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UdxError("libudx.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ok(status):
+    if status != 0:
+        raise UdxError(lib().udx_last_error().decode())
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+# ---------------------------------------------------------------- host RNG
+class RndTrunk:
+    """KISS trunk (src/math/rnd/imp.cpp): rnd_ini / rnd_get."""
+
+    def __init__(self, x, y, z, c):
+        self.s = Rnd()
+        lib().udx_rnd_ini(x, y, z, c, C.byref(self.s))
+
+    def get(self):
+        return lib().udx_rnd_get(C.byref(self.s))
+
+
+def dpd_params(a, g, kBT, dt, spow=0.25):
+    """pair_set_dpd + pair_compute_dpd_sigma (src/pair/imp/main.h:54-62)."""
+    p = PairDpd(a, g, 0.0, spow)
+    lib().udx_pair_compute_dpd_sigma(kBT, dt, C.byref(p))
+    return p
+
+
+def rnd_branch(ctx, seed, u, v, out, uu=False):
+    """rnd::mean0var1ii/uu on the GPU for int32 device tensors u, v."""
+    _ok(lib().udx_rnd_branch(ctx.h, seed, u.shape[0], _ptr(u), _ptr(v), 1 if uu else 0, _ptr(out)))
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    def __init__(self, stream=None):
+        self.h = C.c_void_p()
+        _ok(lib().udx_ctx_ini(stream if stream is not None else _stream(), C.byref(self.h)))
+
+    def sync(self):
+        _ok(lib().udx_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().udx_ctx_fin(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- clist
+class Clist:
+    """clist_ini / clist_build / clists_get_ss / clists_get_cc (src/clist/imp.h)."""
+
+    def __init__(self, ctx, L, maxp):
+        self.ctx, self.L, self.maxp = ctx, tuple(L), maxp
+        self.h = C.c_void_p()
+        _ok(lib().udx_clist_ini(ctx.h, L[0], L[1], L[2], maxp, C.byref(self.h)))
+
+    @property
+    def ncells(self):
+        return self.L[0] * self.L[1] * self.L[2]
+
+    def build(self, pp, ppout):
+        n = C.c_int()
+        _ok(lib().udx_clist_build(self.h, pp.shape[0], _ptr(pp), _ptr(ppout), C.byref(n)))
+        return n.value
+
+    def _view(self, fn, count):
+        import torch
+        out = torch.empty(count, dtype=torch.int32, device="cuda")
+        _copy_dev(out, fn(self.h), count * 4)
+        return out
+
+    def starts(self):
+        return self._view(lib().udx_clist_starts, self.ncells + 1)
+
+    def counts(self):
+        return self._view(lib().udx_clist_counts, self.ncells)
+
+    def order(self, n):
+        return self._view(lib().udx_clist_order, n)
+
+    def starts_ptr(self):
+        return C.c_void_p(lib().udx_clist_starts(self.h))
+
+    def close(self):
+        if self.h:
+            lib().udx_clist_fin(self.h)
+            self.h = C.c_void_p()
+
+
+def _copy_dev(dst, src_ptr, nbytes):
+    """Device-to-device copy from a raw pointer owned by libudx."""
+    import torch
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipMemcpyAsync.restype = C.c_int
+    st = hip.hipMemcpyAsync(C.c_void_p(dst.data_ptr()), C.c_void_p(src_ptr), nbytes, 3, _stream())
+    if st != 0:
+        raise UdxError("hipMemcpyAsync failed: %d" % st)
+    torch.cuda.current_stream().synchronize()
+
+
+# ---------------------------------------------------------------- fluforces
+class FluForces:
+    """fluforces_bulk_* + fluforces_halo_* (src/fluforces/imp.h) on one
+    periodic domain."""
+
+    def __init__(self, ctx, L, maxp, exact=False):
+        self.ctx, self.L = ctx, tuple(L)
+        self.h = C.c_void_p()
+        _ok(lib().udx_fluforces_ini(ctx.h, L[0], L[1], L[2], maxp, C.byref(self.h)))
+        _ok(lib().udx_fluforces_set_mode(self.h, 1 if exact else 0))
+
+    def prepare(self, pp):
+        _ok(lib().udx_fluforces_prepare(self.h, pp.shape[0], _ptr(pp)))
+
+    def apply(self, params, starts_ptr, ff, periodic=True, seeds=None):
+        if seeds is None:
+            _ok(lib().udx_fluforces_apply(self.h, C.byref(params), starts_ptr, 1 if periodic else 0, _ptr(ff)))
+        else:
+            arr = (C.c_float * 27)(*[float(x) for x in seeds])
+            _ok(lib().udx_fluforces_apply_seeds(self.h, C.byref(params), starts_ptr, 1 if periodic else 0,
+                                                arr, _ptr(ff)))
+
+    def pairs(self, reset=False):
+        v = C.c_ulonglong()
+        _ok(lib().udx_fluforces_pairs(self.h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().udx_fluforces_fin(self.h)
+            self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- scheme / diag
+def scheme_move_apply(ctx, dt, mass, ff, pp):
+    _ok(lib().udx_scheme_move_apply(ctx.h, dt, mass, pp.shape[0], _ptr(ff), _ptr(pp)))
+
+
+def bforce_apply_cste(ctx, mass, accel, ff):
+    a = (C.c_float * 3)(*accel)
+    _ok(lib().udx_bforce_apply_cste(ctx.h, mass, a, ff.shape[0], _ptr(ff)))
+
+
+def wrap_pp(ctx, L, pp):
+    _ok(lib().udx_wrap_pp(ctx.h, L[0], L[1], L[2], pp.shape[0], _ptr(pp)))
+
+
+def diag_part(ctx, pp):
+    out = (C.c_double * 5)()
+    _ok(lib().udx_diag_part(ctx.h, pp.shape[0], _ptr(pp), out))
+    return list(out)
+
+
+# ---------------------------------------------------------------- solver
+class Flu:
+    """Solvent-only step pipeline (src/sim/imp/run.h:8-68): clist + reorder,
+    DPD forces, body force, kick-drift, periodic wrap."""
+
+    def __init__(self, ctx, L, maxp, params, dt, mass=1.0, accel=(0.0, 0.0, 0.0), exact=False):
+        self.ctx = ctx
+        conf = FluConf()
+        for k in range(3):
+            conf.L[k] = L[k]
+            conf.accel[k] = accel[k]
+        conf.maxp, conf.dpd, conf.dt, conf.mass = maxp, params, dt, mass
+        conf.exact = 1 if exact else 0
+        self.h = C.c_void_p()
+        _ok(lib().udx_flu_ini(ctx.h, C.byref(conf), C.byref(self.h)))
+
+    def set_pp(self, pp):
+        _ok(lib().udx_flu_set_pp(self.h, pp.shape[0], _ptr(pp)))
+
+    def get_pp(self, out):
+        n = C.c_int()
+        _ok(lib().udx_flu_get_pp(self.h, _ptr(out), C.byref(n)))
+        return n.value
+
+    def get_ff(self, out):
+        _ok(lib().udx_flu_get_ff(self.h, _ptr(out)))
+
+    def run(self, nsteps, seeds=None):
+        if seeds is None:
+            _ok(lib().udx_flu_run(self.h, nsteps, None))
+        else:
+            import numpy as np
+            s = np.ascontiguousarray(seeds, dtype=np.float32)
+            assert s.shape == (nsteps, 27)
+            _ok(lib().udx_flu_run(self.h, nsteps, s.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def pairs(self, reset=False):
+        v = C.c_ulonglong()
+        _ok(lib().udx_flu_pairs(self.h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def timing(self, enable=True):
+        out = (C.c_float * 4)()
+        _ok(lib().udx_flu_timing(self.h, 1 if enable else 0, out))
+        return list(out)
+
+    def close(self):
+        if self.h:
+            lib().udx_flu_fin(self.h)
+            self.h = C.c_void_p()

@@ -1,0 +1,137 @@
+/*
+ * udx: MI355X (gfx950) HIP implementation of the uDeviceX per-timestep DPD
+ * hot path behind a C ABI.
+ *
+ * Conventions shared by every entry point:
+ *  - particle arrays at the boundary are the reference's AoS layout
+ *    Particle{float r[3], v[3]} (24 B, src/inc/type.h:6-8) and forces are
+ *    Force{float f[3]} (12 B, src/inc/type.h:20-22), as raw device pointers;
+ *  - forces are ACCUMULATED (+=) as in the reference; the caller clears them;
+ *  - every call returns 0 on success and a non-zero status on failure, with
+ *    a message in udx_last_error() (the reference aborts through ERR/UC,
+ *    src/utils/error.h:1-17; a caller that wants that behaviour exits on a
+ *    non-zero status);
+ *  - all work is enqueued on the stream bound to the udx_ctx, nothing
+ *    synchronises unless the call returns a host value.
+ * The box is [-L/2, L/2)^3 with unit cells, x fastest, one periodic domain.
+ */
+#ifndef UDX_H
+#define UDX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { UDX_OK = 0, UDX_EINVAL = 1, UDX_EHIP = 2, UDX_ECAP = 3, UDX_ESTATE = 4 };
+
+/* message of the last failed call on this thread */
+const char *udx_last_error(void);
+
+/* ---- KISS trunk generator: replaces rnd_ini/rnd_get/rnd_fin
+ *      (src/math/rnd/imp.h:2-5; algorithm src/math/rnd/imp.cpp:9-34) ---- */
+typedef struct udx_rnd { uint32_t x, y, z, c; } udx_rnd;
+void  udx_rnd_ini(int x, int y, int z, int c, udx_rnd *r);
+float udx_rnd_get(udx_rnd *r);
+
+/* ---- DPD pair parameters: replaces pair_set_dpd / pair_compute_dpd_sigma /
+ *      pair_get_view_dpd (src/pair/imp.h:11-35, grey single-colour view) ---- */
+typedef struct udx_pair_dpd { float a, g, s, spow; } udx_pair_dpd;
+/* s = sqrt(2 kBT g / dt) (src/pair/imp/main.h:54-62) */
+void udx_pair_compute_dpd_sigma(float kBT, float dt, udx_pair_dpd *p);
+
+/* ---- context: one per host thread, bound to a HIP stream ---- */
+typedef struct udx_ctx udx_ctx;
+int udx_ctx_ini(void *hip_stream /* hipStream_t, NULL = default */, udx_ctx **ctx);
+int udx_ctx_fin(udx_ctx *ctx);
+int udx_ctx_sync(udx_ctx *ctx);
+
+/* ---- branch generator on the GPU: rnd::mean0var1ii / mean0var1uu
+ *      (src/math/rnd/dev.h:57-72) for n index pairs (device arrays);
+ *      uu != 0 selects the unsigned variant ---- */
+int udx_rnd_branch(udx_ctx *ctx, float seed, int n, const int *u, const int *v, int uu, float *out);
+
+/* ---- cell list: replaces clist_ini / clist_ini_map / clist_build /
+ *      clists_get_ss / clists_get_cc (src/clist/imp.h:5-33) ----
+ * Stable counting sort: particles are ordered by cell, ties keep their
+ * input order; particles outside the box are dropped. */
+typedef struct udx_clist udx_clist;
+int udx_clist_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_clist **c);
+int udx_clist_fin(udx_clist *c);
+/* pp: n input particles (device AoS); ppout: cell-ordered output (device AoS,
+ * capacity maxp); *nout (host) receives the number kept (synchronises). */
+int udx_clist_build(udx_clist *c, int n, const float *pp, float *ppout, int *nout);
+const int *udx_clist_starts(const udx_clist *c);   /* device, ncells + 1 */
+const int *udx_clist_counts(const udx_clist *c);   /* device, ncells     */
+const int *udx_clist_order(const udx_clist *c);    /* device, input index per output slot */
+
+/* ---- fluid-fluid DPD forces: replaces fluforces_bulk_ini / prepare / apply
+ *      and fluforces_halo_ini / prepare / apply (src/fluforces/imp.h:8-25) ----
+ * One periodic domain: pairs across the box boundary are computed in the
+ * same pass with the halo RNG of src/fluforces/halo/dev/main.h:7-12. */
+typedef struct udx_fluforces udx_fluforces;
+int udx_fluforces_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_fluforces **f);
+int udx_fluforces_fin(udx_fluforces *f);
+/* AoS -> internal float4 position/velocity streams (the reference's
+ * zipped_pp conversion, src/fluforces/imp/main.h:1-8) */
+int udx_fluforces_prepare(udx_fluforces *f, int n, const float *pp);
+/* pp given to prepare must be cell ordered by the clist whose starts are
+ * passed.  periodic = 0 skips cells outside the box (the bulk-only unit
+ * harness, src/u/bulk/main.cu).  Draws one seed per trunk on the host
+ * (bulk trunk (0,0,0,0), 26 halo trunks) and adds the forces into ff. */
+int udx_fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
+                        float *ff);
+/* same with explicit seeds: seeds[fid] for halo fragment fid (0..25),
+ * seeds[26] for the bulk */
+int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int *starts,
+                              int periodic, const float seeds[27], float *ff);
+/* exact = 1: correctly rounded sqrt and division in the pair force, so the
+ * forces equal the CPU restatement bit for bit (slower); 0 (default):
+ * hardware reciprocal square root and square root */
+int udx_fluforces_set_mode(udx_fluforces *f, int exact);
+/* unordered pair interactions (r^2 < 1) counted by apply calls since the last reset */
+int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset);
+
+/* ---- integrator: replaces scheme_move_apply (src/scheme/move/imp.h:5-6) and
+ *      bforce_apply for a constant body force (src/scheme/force/imp.h) ---- */
+/* kick-drift v += f/m dt; r += v dt (src/scheme/move/dev/vv.h:1-10) */
+int udx_scheme_move_apply(udx_ctx *ctx, float dt, float mass, int n, const float *ff, float *pp);
+/* f += m a (src/scheme/force/dev/main.h:1-4,65-80) */
+int udx_bforce_apply_cste(udx_ctx *ctx, float mass, const float accel[3], int n, float *ff);
+/* periodic wrap of positions into [-L/2, L/2) (single-domain redistribution) */
+int udx_wrap_pp(udx_ctx *ctx, int Lx, int Ly, int Lz, int n, float *pp);
+
+/* ---- diagnostics (src/io/diag/part/imp/main.h:37-63), FP64 on the GPU ----
+ * out[0..2] = sum v, out[3] = kBT = 0.5 sum|v|^2 / (1.5 n), out[4] = max |v|^2
+ * (synchronises) */
+int udx_diag_part(udx_ctx *ctx, int n, const float *pp, double out[5]);
+
+/* ---- fluid solver: the per-step pipeline of src/sim/imp/run.h:8-68 for a
+ *      solvent-only box (cell list, DPD forces, body force, kick-drift,
+ *      periodic wrap) on internal float4 state ---- */
+typedef struct udx_flu_conf {
+    int L[3];
+    int maxp;
+    udx_pair_dpd dpd;
+    float dt, mass;
+    float accel[3];     /* constant body force per unit mass */
+    int exact;          /* 1: correctly rounded pair force (see udx_fluforces_set_mode) */
+} udx_flu_conf;
+typedef struct udx_flu udx_flu;
+int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **f);
+int udx_flu_fin(udx_flu *f);
+int udx_flu_set_pp(udx_flu *f, int n, const float *pp);       /* device AoS in  */
+int udx_flu_get_pp(udx_flu *f, float *pp, int *n);            /* device AoS out */
+int udx_flu_get_ff(udx_flu *f, float *ff);                    /* last step forces, AoS */
+/* advance nsteps steps; seeds come from the internal trunks unless a table
+ * is given (seeds_host: nsteps x 27 floats, layout as udx_fluforces_apply_seeds) */
+int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host);
+int udx_flu_pairs(udx_flu *f, unsigned long long *npairs, int reset);
+/* per-step kernel timing of the last udx_flu_run (ms, HIP events on the
+ * bound stream): [0] cell list, [1] forces, [2] integrate, [3] whole steps */
+int udx_flu_timing(udx_flu *f, int enable, float out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

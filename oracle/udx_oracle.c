@@ -333,8 +333,10 @@ int orc_pair_force(const orc_dpd *p, const float *a, const float *b, float rnd, 
     if (r2 < 1e-6f) {
         ex = ey = ez = r = 0.0f;
     } else {
-        float invr = 1.0f / sqrtf(r2);
-        r = r2 * invr;
+        /* the reference's r = r2 * rsqrtf(r2) evaluated as the correctly
+         * rounded square root, 1/r correctly rounded */
+        r = sqrtf(r2);
+        float invr = 1.0f / r;
         ex = dx * invr; ey = dy * invr; ez = dz * invr;
     }
     float dvx = a[3] - b[3], dvy = a[4] - b[4], dvz = a[5] - b[5];

@@ -1,0 +1,245 @@
+"""GPU parity of the DPD hot path against the CPU oracle, through the C ABI.
+
+Bars: bit-exact for the RNG, the cell list (starts, counts, order) and, in
+exact mode, the forces and multi-step trajectories; the production (fast)
+force kernel within the stated FP32 tolerance.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from ctc_amd import udx as U  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
+
+# Production-kernel force tolerance.  The kernel uses the hardware reciprocal
+# square root and square root (about 1 ulp each) where the oracle rounds
+# correctly; summation order is identical.  Per particle:
+#   |f_gpu - f_oracle| <= FTOL * max(|f_oracle|, 0.1 * F_rms)
+FTOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return U.Context()
+
+
+def dev(a, dtype=torch.float32):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def random_box(L, rho, seed, vel=1.0, outside=0.0):
+    rng = np.random.default_rng(seed)
+    n = int(rho * np.prod(L))
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L) * (1 + outside)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, vel, (n, 3)).astype(np.float32)
+    return pp
+
+
+# ------------------------------------------------------------------ RNG
+def test_branch_rng_bits(ctx):
+    rng = np.random.default_rng(0)
+    n = 1 << 18
+    u = rng.integers(0, 1 << 24, n).astype(np.int32)
+    v = rng.integers(0, 1 << 24, n).astype(np.int32)
+    u[:1024] = np.arange(1024)        # small indices, the & 1023 branch
+    v[:1024] = 0
+    for k, seed in enumerate(O.kiss_fill(0, 0, 0, 0, 3)):
+        for uu in (False, True):
+            say("rng seed", k, "uu", uu)
+            out = torch.empty(n, dtype=torch.float32, device="cuda")
+            U.rnd_branch(ctx, float(seed), dev(u, torch.int32), dev(v, torch.int32), out, uu=uu)
+            want = O.rng_uu(seed, u.view(np.uint32), v.view(np.uint32)) if uu else O.rng_ii(seed, u, v)
+            got = host(out)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), \
+                "mismatches: %d" % np.count_nonzero(got.view(np.uint32) != want.view(np.uint32))
+
+
+# ------------------------------------------------------------------ clist
+@pytest.mark.parametrize("L,rho,outside", [((4, 6, 8), 3, 0.0), ((16, 16, 16), 3, 0.1), ((5, 7, 9), 10, 0.0),
+                                           ((33, 20, 17), 6, 0.05)])
+def test_clist_bit_exact(ctx, L, rho, outside):
+    say("clist", L, rho)
+    pp = random_box(L, rho, 1, outside=outside)
+    s_ref, c_ref, out_ref, order_ref = O.clist_build(L, pp)
+    cl = U.Clist(ctx, L, len(pp))
+    ppd = dev(pp)
+    out = torch.zeros_like(ppd)
+    nout = cl.build(ppd, out)
+    assert nout == len(out_ref)
+    assert np.array_equal(host(cl.starts()), s_ref)
+    assert np.array_equal(host(cl.counts()), c_ref)
+    assert np.array_equal(host(cl.order(nout)), order_ref)
+    assert np.array_equal(host(out)[:nout].view(np.uint32), out_ref.view(np.uint32))
+    cl.close()
+
+
+@pytest.mark.parametrize("name,gen", [("t0", (4, 6, 8, 128)), ("t1", (2, 6, 8, 64)), ("t2", (20, 6, 8, 64))])
+def test_clist_golden_on_gpu(ctx, name, gen):
+    say("clist golden", name)
+    pp = O.gen_random(*gen)
+    cl = U.Clist(ctx, (4, 6, 8), len(pp))
+    ppd = dev(pp)
+    out = torch.zeros_like(ppd)
+    cl.build(ppd, out)
+    ref = np.loadtxt(os.path.join(GOLD, "clist.ref.clist.%s.txt" % name), dtype=np.int64)
+    assert np.array_equal(host(cl.starts())[:-1], ref[1::2])
+    assert np.array_equal(host(cl.counts()), ref[2::2])
+    cl.close()
+
+
+# ------------------------------------------------------------------ forces
+def gpu_forces(ctx, L, pp_sorted, starts, params, seeds, periodic=True, exact=False):
+    n = len(pp_sorted)
+    ff = FluForcesRun(ctx, L, n, exact)
+    return ff(pp_sorted, starts, params, seeds, periodic)
+
+
+class FluForcesRun:
+    def __init__(self, ctx, L, n, exact):
+        self.ctx, self.L = ctx, L
+        self.f = U.FluForces(ctx, L, max(n, 1), exact=exact)
+
+    def __call__(self, pp_sorted, starts, params, seeds, periodic):
+        ppd = dev(pp_sorted)
+        sd = dev(starts, torch.int32)
+        ff = torch.zeros((len(pp_sorted), 3), dtype=torch.float32, device="cuda")
+        self.f.prepare(ppd)
+        self.f.apply(params, U.C.c_void_p(sd.data_ptr()), ff, periodic=periodic, seeds=seeds)
+        out = host(ff)
+        npairs = self.f.pairs(reset=True)
+        self.f.close()
+        return out, npairs
+
+
+CASES = [
+    # L, rho, a, gamma, kBT, dt, spow
+    ((16, 16, 16), 3, 75.0, 4.5, 1.0, 1e-3, 0.25),     # the plain periodic fluid
+    ((12, 10, 9), 10, 2.5, 8.0, 0.0444, 5e-4, 0.25),   # density 10 channel parameters
+    ((7, 5, 3), 5, 25.0, 4.5, 0.5, 1e-3, 1.0),         # thin box, spow = 1
+    ((3, 3, 3), 4, 25.0, 4.5, 1.0, 1e-3, 0.5),         # smallest periodic box
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_forces_exact_mode_bit_exact(ctx, case):
+    L, rho, a, g, kBT, dt, spow = case
+    say("forces exact", case)
+    pp = random_box(L, rho, 7)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    seeds = O.seeds_table(5)[4]
+    par = (a, g, O.dpd_sigma(kBT, g, dt), spow)
+    f_ref, np_ref = O.dpd_forces(L, srt, starts, counts, par, seeds)
+    f_gpu, np_gpu = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, exact=True)
+    assert np_gpu == np_ref // 2
+    assert np.array_equal(f_gpu, f_ref), "max |df| = %g" % np.abs(f_gpu - f_ref).max()
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_forces_fast_mode_tolerance(ctx, case):
+    L, rho, a, g, kBT, dt, spow = case
+    say("forces fast", case)
+    pp = random_box(L, rho, 8)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    seeds = O.seeds_table(3)[2]
+    par = (a, g, O.dpd_sigma(kBT, g, dt), spow)
+    f_ref, np_ref = O.dpd_forces(L, srt, starts, counts, par, seeds)
+    f_gpu, np_gpu = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, exact=False)
+    assert np_gpu == np_ref // 2           # cutoff decisions are exact arithmetic
+    mag = np.linalg.norm(f_ref.astype(np.float64), axis=1)
+    frms = np.sqrt((mag ** 2).mean())
+    err = np.linalg.norm(f_gpu.astype(np.float64) - f_ref, axis=1) / np.maximum(mag, 0.1 * frms)
+    say("  max rel err", err.max(), "F_rms", frms)
+    assert err.max() <= FTOL
+
+
+@pytest.mark.parametrize("name,n,scale", [("t0", 64, 1e4), ("t1", 640, 1e2)])
+def test_bulk_golden_on_gpu(ctx, name, n, scale):
+    """bulk.t0/t1 fixtures: kBT = 0, bulk kernel only (no periodic images)."""
+    say("bulk golden", name)
+    pp = O.gen_random(4, 4, 4, n)
+    starts, counts, srt, _ = O.clist_build((4, 4, 4), pp)
+    par = U.PairDpd(np.float32(2.6666667), 8.0, 0.0, 0.25)
+    f_gpu, _ = gpu_forces(ctx, (4, 4, 4), srt, starts, par, np.zeros(27, np.float32), periodic=False)
+    ref = np.loadtxt(os.path.join(GOLD, "ff.ref.bulk.%s.txt" % name)) / scale
+    table = {tuple(np.round(r[:6], 4)): r[6:] for r in ref}
+    for p, f in zip(srt, f_gpu):
+        fr = table[tuple(np.round(p.astype(np.float64), 4))]
+        assert np.linalg.norm(f - fr) <= 1e-5 * max(np.linalg.norm(fr), 1.0)
+
+
+# ------------------------------------------------------------------ steps
+def run_solver(ctx, L, pp, params, dt, nsteps, seeds, accel=(0.0, 0.0, 0.0), exact=True):
+    flu = U.Flu(ctx, L, len(pp), U.PairDpd(*params), dt, 1.0, accel, exact=exact)
+    ppd = dev(pp)
+    flu.set_pp(ppd)
+    flu.run(nsteps, seeds)
+    out = torch.zeros_like(ppd)
+    flu.get_pp(out)
+    pairs = flu.pairs()
+    res = host(out)
+    flu.close()
+    return res, pairs
+
+
+@pytest.mark.parametrize("L,rho,accel", [((16, 16, 16), 3, (0.0, 0.0, 0.0)), ((10, 8, 12), 10, (0.5, 0.0, 0.0))])
+def test_solver_trajectory_bit_exact(ctx, L, rho, accel):
+    """clist + forces + body force + kick-drift + wrap over 20 steps, exact
+    mode: positions and velocities equal the oracle's bit for bit."""
+    say("trajectory", L, rho)
+    nsteps = 20
+    params = (25.0, 4.5, O.dpd_sigma(1.0, 4.5, 1e-3), 0.25)
+    pp = random_box(L, rho, 3)
+    seeds = O.seeds_table(nsteps)
+    ref = pp.copy()
+    pairs_ref = 0
+    for k in range(nsteps):
+        ref, npair = O.flu_step(L, ref, params, seeds[k], 1e-3, accel=accel)
+        pairs_ref += npair
+    got, pairs = run_solver(ctx, L, pp, params, 1e-3, nsteps, seeds, accel)
+    assert pairs == pairs_ref
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_solver_fast_mode_conservation(ctx):
+    """Production kernel over 50 steps of the 16^3, rho = 3 fluid: FP64
+    momentum and temperature agree with the exact-mode oracle run."""
+    L, rho = (16, 16, 16), 3
+    nsteps = 50
+    params = (75.0, 4.5, O.dpd_sigma(1.0, 4.5, 1e-3), 0.25)
+    pp = random_box(L, rho, 5)
+    pp[:, 3:] -= pp[:, 3:].mean(0)
+    seeds = O.seeds_table(nsteps)
+    say("fast solver 50 steps")
+    got, _ = run_solver(ctx, L, pp, params, 1e-3, nsteps, seeds, exact=False)
+    ref = pp.copy()
+    for k in range(nsteps):
+        ref, _ = O.flu_step(L, ref, params, seeds[k], 1e-3, nthreads=8)
+    dg, dr = O.diag(got), O.diag(ref)
+    n = len(pp)
+    assert abs(dg[3] - dr[3]) < 1e-3 * dr[3]
+    assert np.all(np.abs(dg[:3] - dr[:3]) < 1e-3 * np.sqrt(n))
+    # GPU FP64 diagnostics equal the oracle's on the same state
+    ctx_out = U.diag_part(ctx, dev(got))
+    assert np.allclose(ctx_out, dg, rtol=1e-12, atol=1e-9)
