@@ -23,7 +23,8 @@ namespace udx {
 namespace {
 
 constexpr int DPD_THREADS = 256;
-constexpr int QCAP = 32;
+constexpr int QCAP = 24;   // queued pairs per lane before a drain
+constexpr int SCAP = 160;  // staged candidates per wave and row
 
 struct DpdK {
     const float4 *pos, *vel;
@@ -54,9 +55,70 @@ __device__ __forceinline__ bool xrun(int sg, int cx, int Lx, int periodic, int *
 
 __device__ __forceinline__ int wrap_dir(int c, int L) { return c < 0 ? -1 : (c >= L ? 1 : 0); }
 
+// Queue entry of a candidate that passed the cutoff: the neighbour's index j
+// (27 bits) and the periodic wrap w of its cell seen from the home cell,
+// packed as (wx + 1) + 3 (wy + 1) + 9 (wz + 1); 13 means no wrap.
+constexpr unsigned W_NONE = 13u;
+
+__device__ __forceinline__ unsigned wcode(int wx, int wy, int wz) {
+    return (unsigned)((wx + 1) + 3 * (wy + 1) + 9 * (wz + 1));
+}
+
+// inputs of one queued pair: separation, relative velocity and the logistic
+// map argument seed - p of its noise
+struct PairIn {
+    float dx, dy, dz, r2, dvx, dvy, dvz, x;
+};
+
+__device__ __forceinline__ PairIn pair_in(const DpdK &A, unsigned ent, int i, const float4 &pi, const float4 &vi,
+                                          int cx, int cy, int cz, int istart, float seed_bulk) {
+    const int j = (int)(ent & 0x7FFFFFFu);
+    const unsigned w = ent >> 27;
+    const float4 pj = A.pos[j], vj = A.vel[j];
+    float ax = pi.x, ay = pi.y, az = pi.z;
+    // bulk pair: mean0var1ii(seed, max, min) (src/fluforces/bulk/dev/main.h:32-38)
+    int u = i > j ? i : j, v = i > j ? j : i;
+    float seed = seed_bulk;
+    if (w != W_NONE) {
+        // pair across the periodic boundary: the local particle is moved into
+        // the neighbour frame (src/fluforces/halo/dev/main.h:40-48) and the
+        // noise is mean0var1uu(seed_d, ...) of fragment d = w with positions
+        // in the packed lists (src/fluforces/halo/dev/main.h:7-12,
+        // src/exch/flu/dev/pack.h:2-40)
+        const int Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
+        const int wx = (int)(w % 3u) - 1, wy = (int)((w / 3u) % 3u) - 1, wz = (int)(w / 9u) - 1;
+        ax = pi.x - (float)(wx * Lx);
+        ay = pi.y - (float)(wy * Ly);
+        az = pi.z - (float)(wz * Lz);
+        const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
+        const int aid = A.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] + (i - istart);
+        int jx = 0, jy = 0, jz = 0;
+        cell_coord(pj.x, Lx, &jx);
+        cell_coord(pj.y, Ly, &jy);
+        cell_coord(pj.z, Lz, &jz);
+        const int jstart = A.starts[jx + Lx * (jy + Ly * jz)];
+        const int bid = A.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] + (j - jstart);
+        const bool mask = fid < anti;
+        u = mask ? aid : bid;
+        v = mask ? bid : aid;
+        seed = A.seeds[fid];
+    }
+    PairIn o;
+    o.dx = ax - pj.x;
+    o.dy = ay - pj.y;
+    o.dz = az - pj.z;
+    o.r2 = __builtin_fmaf(o.dz, o.dz, __builtin_fmaf(o.dx, o.dx, o.dy * o.dy));
+    o.dvx = vi.x - vj.x;
+    o.dvy = vi.y - vj.y;
+    o.dvz = vi.z - vj.z;
+    o.x = seed - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
+    return o;
+}
+
 template <int WR, bool EXACT>
 __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     __shared__ unsigned q[QCAP * DPD_THREADS];
+    __shared__ float4 sbuf[DPD_THREADS / 64][SCAP + 4];
     __shared__ unsigned long long lpairs[DPD_THREADS / 64];
     const int tid = threadIdx.x;
     const int Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
@@ -77,89 +139,119 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     const float seed_bulk = A.seeds[26];
     float fx = 0.f, fy = 0.f, fz = 0.f;
     int npair = 0, fill = 0;
+    unsigned *qlane = q + tid;
 
-    // drain the per-lane queues: pair body for every queued neighbour
+    // Drain the per-lane queues in queue order (= candidate order), two
+    // pairs per lane at a time so two noise chains share each
+    // round-toward-zero window.
     auto drain = [&]() {
-        for (int e = 0; wave_any(e < fill); e++) {
-            if (e < fill) {
-                const unsigned ent = q[e * DPD_THREADS + tid];
-                const int j = (int)(ent & 0x7FFFFFFu);
-                const int code = (int)(ent >> 27);
-                const int row = code / 3, wx = code % 3 - 1;
-                const int dz = row / 3 - 1, dy = row % 3 - 1;
-                const int yy = cy + dy, zz = cz + dz;
-                const int wy = wrap_dir(yy, Ly), wz = wrap_dir(zz, Lz);
-                const float4 pj = A.pos[j], vj = A.vel[j];
-                const float ax = pi.x - (float)(wx * Lx);
-                const float ay = pi.y - (float)(wy * Ly);
-                const float az = pi.z - (float)(wz * Lz);
-                const float ddx = ax - pj.x, ddy = ay - pj.y, ddz = az - pj.z;
-                const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));
-                float rnd;
-                if ((wx | wy | wz) == 0) {
-                    const int hi = i > j ? i : j, lo = i > j ? j : i;
-                    rnd = mean0var1ii(seed_bulk, hi, lo);
-                } else {
-                    const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
-                    const int aid = A.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] + (i - istart);
-                    int jx = 0;
-                    cell_coord(pj.x, Lx, &jx);
-                    const int jy = yy - wy * Ly, jz = zz - wz * Lz;
-                    const int jstart = A.starts[jx + Lx * (jy + Ly * jz)];
-                    const int bid =
-                        A.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] + (j - jstart);
-                    const bool mask = fid < anti;
-                    rnd = mean0var1uu(A.seeds[fid], (uint32_t)(mask ? aid : bid), (uint32_t)(mask ? bid : aid));
-                }
-                const float3 f = dpd_pair<WR, EXACT>(A.par, ddx, ddy, ddz, r2, vi.x - vj.x, vi.y - vj.y,
-                                                     vi.z - vj.z, rnd);
-                fx += f.x;
-                fy += f.y;
-                fz += f.z;
-                npair++;
-            }
+        for (int e = 0; wave_any(e < fill); e += 2) {
+            const bool va = e < fill, vb = e + 1 < fill;
+            const unsigned ea = va ? qlane[e * DPD_THREADS] : (unsigned)i | (W_NONE << 27);
+            const unsigned eb = vb ? qlane[(e + 1) * DPD_THREADS] : (unsigned)i | (W_NONE << 27);
+            PairIn a = pair_in(A, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+            PairIn b = pair_in(A, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+            float xa = a.x, xb = b.x;
+            logistic18x2(&xa, &xb);
+            const float3 f1 = dpd_pair<WR, EXACT>(A.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+            const float3 f2 = dpd_pair<WR, EXACT>(A.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+            fx += va ? f1.x : 0.f;
+            fy += va ? f1.y : 0.f;
+            fz += va ? f1.z : 0.f;
+            fx += vb ? f2.x : 0.f;
+            fy += vb ? f2.y : 0.f;
+            fz += vb ? f2.z : 0.f;
         }
+        npair += fill;
         fill = 0;
     };
 
-    int k = 0, j = 0, je = 0, code = 0;
-    float xa = 0.f, ya = 0.f, za = 0.f;
-    for (;;) {
-        // next non-empty run of candidate cells
-        while (active && j >= je && k < 18) {
-            const int row = k >> 1, sg = k & 1;
-            k++;
-            const int dz = row / 3 - 1, dy = row % 3 - 1;
-            int yy = cy + dy, zz = cz + dz;
-            const int wy = wrap_dir(yy, Ly), wz = wrap_dir(zz, Lz);
-            if ((wy | wz) && !A.periodic) continue;
-            int x0, x1, wx;
-            if (!xrun(sg, cx, Lx, A.periodic, &x0, &x1, &wx)) continue;
-            yy -= wy * Ly;
-            zz -= wz * Lz;
-            const int base = Lx * (yy + Ly * zz);
-            j = A.starts[base + x0];
-            je = A.starts[base + x1 + 1];
-            code = row * 3 + (wx + 1);
-            xa = pi.x - (float)(wx * Lx);
-            ya = pi.y - (float)(wy * Ly);
-            za = pi.z - (float)(wz * Lz);
+    auto push = [&](bool hit, int j, unsigned w) {
+        if (hit) {
+            qlane[fill * DPD_THREADS] = (unsigned)j | (w << 27);
+            fill++;
         }
-        const bool has = active && j < je;
-        if (!wave_any(has)) break;
-        if (has) {
-            if (j != i) {
-                const float4 pj = A.pos[j];
-                const float ddx = xa - pj.x, ddy = ya - pj.y, ddz = za - pj.z;
-                const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));
-                if (r2 < 1.0f) {
-                    q[fill * DPD_THREADS + tid] = (unsigned)j | ((unsigned)code << 27);
-                    fill++;
-                }
+    };
+
+    // Candidate pass, one row (dz, dy) of the 27-cell shell at a time: a row
+    // is one run of up to 3 cells, or two runs where x wraps; the loop shape
+    // is the same for every lane.  The union of the wave's runs of a row is
+    // one contiguous index range when no lane's row wraps (the lanes are
+    // consecutive cell-ordered particles): it is staged in LDS with coalesced
+    // loads and tested four candidates per iteration; otherwise the wave reads
+    // its candidates from global memory.
+    float4 *sb = sbuf[tid >> 6];
+    for (int row = 0; row < 9; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        int yy = cy + dy, zz = cz + dz;
+        const int wy = yy < 0 ? -1 : (yy >= Ly ? 1 : 0), wz = zz < 0 ? -1 : (zz >= Lz ? 1 : 0);
+        const bool rowok = active && (A.periodic || (wy | wz) == 0);
+        yy -= wy * Ly;
+        zz -= wz * Lz;
+        const int base = Lx * (yy + Ly * zz);
+        int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
+        if (rowok) {
+            if (xrun(0, cx, Lx, A.periodic, &x0, &x1, &wx0)) {
+                j0 = A.starts[base + x0];
+                n0 = A.starts[base + x1 + 1] - j0;
             }
-            j++;
+            if (xrun(1, cx, Lx, A.periodic, &x0, &x1, &wx1)) {
+                j1 = A.starts[base + x0];
+                n1 = A.starts[base + x1 + 1] - j1;
+            }
         }
-        if (wave_any(fill == QCAP)) drain();
+        const bool simple = !rowok || ((wy | wz | wx0) == 0 && n1 == 0);
+        int lo = rowok ? j0 : 0x7fffffff, hi = rowok ? j0 + n0 : 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+        }
+        const bool lds_row = !wave_any(!simple) && hi > lo && hi - lo <= SCAP;
+        if (lds_row) {
+            __builtin_amdgcn_wave_barrier();
+            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = A.pos[lo + k];
+            __builtin_amdgcn_wave_barrier();
+            const float4 *sp = sb + (rowok ? j0 - lo : 0);
+            for (int t = 0; wave_any(t < n0); t += 4) {
+                if (wave_any(fill + 4 > QCAP)) drain();
+                const float4 c0 = sp[t], c1 = sp[t + 1], c2 = sp[t + 2], c3 = sp[t + 3];
+#define UDX_TEST(C, U)                                                                             \
+    {                                                                                              \
+        const float ddx = pi.x - C.x, ddy = pi.y - C.y, ddz = pi.z - C.z;                          \
+        const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));            \
+        const int j = j0 + t + U;                                                                  \
+        push(t + U < n0 && j != i && r2 < 1.0f, j, W_NONE);                                       \
+    }
+                UDX_TEST(c0, 0)
+                UDX_TEST(c1, 1)
+                UDX_TEST(c2, 2)
+                UDX_TEST(c3, 3)
+#undef UDX_TEST
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
+            const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
+            const unsigned w0 = wcode(wx0, wy, wz), w1 = wcode(wx1, wy, wz);
+            const int len = n0 + n1;
+            for (int t = 0; wave_any(t < len); t += 2) {
+                if (wave_any(fill + 2 > QCAP)) drain();
+                const int ta = t, tb = t + 1;
+                const bool oka = ta < len, okb = tb < len;
+                const int ja = ta >= n0 ? j1 + (ta - n0) : j0 + ta;
+                const int jb = tb >= n0 ? j1 + (tb - n0) : j0 + tb;
+                const float4 pa = A.pos[oka ? ja : 0];
+                const float4 pb = A.pos[okb ? jb : 0];
+                const float axa = ta >= n0 ? ax1 : ax0, axb = tb >= n0 ? ax1 : ax0;
+                const float dxa = axa - pa.x, dya = ay - pa.y, dza = az - pa.z;
+                const float dxb = axb - pb.x, dyb = ay - pb.y, dzb = az - pb.z;
+                const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
+                const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
+                push(oka && ja != i && r2a < 1.0f, ja, ta >= n0 ? w1 : w0);
+                push(okb && jb != i && r2b < 1.0f, jb, tb >= n0 ? w1 : w0);
+            }
+        }
     }
     drain();
 

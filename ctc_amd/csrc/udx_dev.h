@@ -40,6 +40,27 @@ __device__ __forceinline__ float t4_round_rz(float x2, float c8, float cm8) {
     return r;
 }
 
+// the same round for two independent chains sharing one round-toward-zero
+// window (halves the mode switches and interleaves the dependent fma)
+__device__ __forceinline__ void t4_round_rz2(float a2, float b2, float c8, float cm8, float *ra, float *rb) {
+    float ta, tb, oa, ob;
+    asm volatile(
+        "s_nop 1\n\t"
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 3\n\t"
+        "s_nop 1\n\t"
+        "v_fma_f32 %2, %6, %4, %7\n\t"
+        "v_fma_f32 %3, %6, %5, %7\n\t"
+        "v_fma_f32 %0, %2, %4, 1.0\n\t"
+        "v_fma_f32 %1, %3, %5, 1.0\n\t"
+        "s_nop 1\n\t"
+        "s_setreg_imm32_b32 hwreg(HW_REG_MODE, 0, 2), 0\n\t"
+        "s_nop 1"
+        : "=&v"(oa), "=&v"(ob), "=&v"(ta), "=&v"(tb)
+        : "v"(a2), "v"(b2), "v"(c8), "v"(cm8));
+    *ra = oa;
+    *rb = ob;
+}
+
 // nine rounds (N = 18), the square rounded to nearest
 __device__ __forceinline__ float logistic18(float x) {
     const float c8 = 8.0f, cm8 = -8.0f;
@@ -49,6 +70,18 @@ __device__ __forceinline__ float logistic18(float x) {
         x = t4_round_rz(x2, c8, cm8);
     }
     return x;
+}
+
+__device__ __forceinline__ void logistic18x2(float *xa, float *xb) {
+    const float c8 = 8.0f, cm8 = -8.0f;
+    float a = *xa, b = *xb;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        float a2 = a * a, b2 = b * b;
+        t4_round_rz2(a2, b2, c8, cm8, &a, &b);
+    }
+    *xa = a;
+    *xb = b;
 }
 
 // p = frac((u&1023) gold + u bronze + (v&1023) silver + v tin), evaluated as
