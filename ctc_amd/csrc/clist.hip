@@ -108,20 +108,32 @@ struct PosAos {
     }
 };
 
+// Cell of each particle and a provisional slot in it.  Consecutive lanes
+// with the same cell (the usual case: the input is the previous step's
+// cell-ordered array) form a segment that takes its slots with one atomic.
 template <typename Pos>
 __global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int Ly, int Lz, int *counts,
                                                      int2 *cellslot) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float3 r = pos(i);
-    int cx, cy, cz;
-    bool ok = cell_coord(r.x, Lx, &cx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz);
-    int2 cs = make_int2(-1, 0);
-    if (ok) {
-        int cid = cx + Lx * (cy + Ly * cz);
-        cs = make_int2(cid, atomicAdd(counts + cid, 1));
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int cid = -1;
+    if (i < n) {
+        const float3 r = pos(i);
+        int cx, cy, cz;
+        if (cell_coord(r.x, Lx, &cx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz))
+            cid = cx + Lx * (cy + Ly * cz);
     }
-    cellslot[i] = cs;
+    const int prev = __shfl_up(cid, 1, 64);
+    const bool head = lane == 0 || prev != cid;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int hlane = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll(above) - 1 : 64;
+    int base = 0;
+    if (head && cid >= 0) base = atomicAdd(counts + cid, next - lane);
+    base = __shfl(base, hlane, 64);
+    if (i < n) cellslot[i] = make_int2(cid, base + (lane - hlane));
 }
 
 __global__ __launch_bounds__(256) void k_clist_scatter(int n, const int2 *cellslot, const int *starts,

@@ -91,13 +91,14 @@ __device__ __forceinline__ PairIn pair_in(const DpdK &A, unsigned ent, int i, co
         ay = pi.y - (float)(wy * Ly);
         az = pi.z - (float)(wz * Lz);
         const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
-        const int aid = A.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] + (i - istart);
+        const int aid = A.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] - A.fss[A.foff[fid]] + (i - istart);
         int jx = 0, jy = 0, jz = 0;
         cell_coord(pj.x, Lx, &jx);
         cell_coord(pj.y, Ly, &jy);
         cell_coord(pj.z, Lz, &jz);
         const int jstart = A.starts[jx + Lx * (jy + Ly * jz)];
-        const int bid = A.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] + (j - jstart);
+        const int bid = A.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] - A.fss[A.foff[anti]] +
+                        (j - jstart);
         const bool mask = fid < anti;
         u = mask ? aid : bid;
         v = mask ? bid : aid;
@@ -283,45 +284,28 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     }
 }
 
-// per-fragment exclusive scan of cell counts in fragment-cell order
-// (src/exch/flu/dev/map.h:22-98); one block per fragment
-__global__ __launch_bounds__(256) void k_frag_tables(const int *starts, int Lx, int Ly, int Lz, int *fss,
-                                                     const int *foff_dev) {
-    __shared__ int lw[4];
-    __shared__ int carry;
-    const int f = blockIdx.x;
+// Counts of the cells of the 26 fragments in fragment-cell order (x
+// fastest, src/exch/flu/dev/map.h:22-42), one extra 0 after each fragment.
+// An exclusive scan over this array gives every fragment's packed-list
+// offsets (src/exch/flu/dev/map.h:44-98) as differences G[off + h] - G[off].
+__global__ __launch_bounds__(256) void k_frag_counts(const int *starts, int Lx, int Ly, int Lz, FragOff fo,
+                                                     int *cnt) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= fo.off[26]) return;
+    int f = 0;
+    while (f < 25 && g >= fo.off[f + 1]) f++;
+    const int h = g - fo.off[f];
     const int dx = (f + 2) % 3 - 1, dy = (f / 3 + 2) % 3 - 1, dz = (f / 9 + 2) % 3 - 1;
-    const int ox = dx == 1 ? Lx - 1 : 0, oy = dy == 1 ? Ly - 1 : 0, oz = dz == 1 ? Lz - 1 : 0;
     const int ex = dx ? 1 : Lx, ey = dy ? 1 : Ly, ez = dz ? 1 : Lz;
-    const int nc = ex * ey * ez;
-    int *out = fss + foff_dev[f];
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int h0 = 0; h0 < nc; h0 += 256) {
-        const int h = h0 + threadIdx.x;
-        int v = 0;
-        if (h < nc) {
-            const int x = ox + h % ex, y = oy + (h / ex) % ey, z = oz + h / (ex * ey);
-            const int cid = x + Lx * (y + Ly * z);
-            v = starts[cid + 1] - starts[cid];
-        }
-        int inc = v;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            int t = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += t;
-        }
-        if (lane == 63) lw[wave] = inc;
-        __syncthreads();
-        int add = carry;
-        for (int w = 0; w < wave; w++) add += lw[w];
-        if (h < nc) out[h] = add + inc - v;
-        __syncthreads();
-        if (threadIdx.x == 255) carry = add + inc;
-        __syncthreads();
+    int v = 0;
+    if (h < ex * ey * ez) {
+        const int x = (dx == 1 ? Lx - 1 : 0) + h % ex;
+        const int y = (dy == 1 ? Ly - 1 : 0) + (h / ex) % ey;
+        const int z = (dz == 1 ? Lz - 1 : 0) + h / (ex * ey);
+        const int cid = x + Lx * (y + Ly * z);
+        v = starts[cid + 1] - starts[cid];
     }
-    if (threadIdx.x == 0) out[nc] = carry;
+    cnt[g] = v;
 }
 
 template <int WR, bool EXACT>
@@ -343,18 +327,25 @@ int frag_alloc(FragTables *t, const Box &b) {
     }
     t->off[26] = off;
     t->size = off;
-    UDX_TRY(dev_alloc(&t->fss, off + 27));
-    UDX_HIP(hipMemcpy(t->fss + off, t->off, 27 * sizeof(int), hipMemcpyHostToDevice));
+    UDX_TRY(dev_alloc(&t->cnt, off));
+    UDX_TRY(dev_alloc(&t->fss, off + 1));
+    UDX_TRY(dev_alloc(&t->scan_tmp, scan_scratch_size(off)));
     return UDX_OK;
 }
 
-void frag_free(FragTables *t) { dev_free(t->fss); }
+void frag_free(FragTables *t) {
+    dev_free(t->cnt);
+    dev_free(t->fss);
+    dev_free(t->scan_tmp);
+}
 
 int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *starts) {
-    hipLaunchKernelGGL(k_frag_tables, dim3(26), dim3(256), 0, s, starts, b.L[0], b.L[1], b.L[2], t->fss,
-                       t->fss + t->size);
+    FragOff fo;
+    for (int f = 0; f < 27; f++) fo.off[f] = t->off[f];
+    hipLaunchKernelGGL(k_frag_counts, dim3(ceil_div(t->size, 256)), dim3(256), 0, s, starts, b.L[0], b.L[1],
+                       b.L[2], fo, t->cnt);
     UDX_LAUNCHED();
-    return UDX_OK;
+    return scan_exclusive(s, t->cnt, t->size, t->fss, t->scan_tmp);
 }
 
 // exact = 1 selects correctly rounded sqrt/division in the pair force

@@ -90,9 +90,14 @@ int gather_aos(hipStream_t s, const ClistBufs *c, int n, const float *pin, float
 
 // ------------------------------------------------------------------ forces
 struct FragTables {
-    int *fss;        // concatenated per-fragment exclusive scans
-    int off[27];     // offsets of each fragment's table
+    int *cnt;        // cell counts of the 26 fragments, concatenated (one 0 after each)
+    int *fss;        // exclusive scan of cnt: fragment f's offsets are fss[off[f] + h] - fss[off[f]]
+    int *scan_tmp;
+    int off[27];     // start of each fragment in cnt
     int size;
+};
+struct FragOff {
+    int off[27];
 };
 int frag_alloc(FragTables *t, const Box &b);
 void frag_free(FragTables *t);
