@@ -33,10 +33,12 @@ static uint32_t kiss_int(udx_rnd *r) {
     return r->x + r->y + r->z;
 }
 
-// the 27 per-step trunks: halo fragments 0..25 (src/fluforces/imp/ini.h:11-58,
-// one rank: seed_base = 0) and the bulk trunk (0,0,0,0) at index 26
+// the per-step trunks: halo fragments 0..25 (src/fluforces/imp/ini.h:11-58,
+// one rank: seed_base = 0), the bulk trunk (0,0,0,0) at index 26 and the
+// wall trunk (42,42,42,42) of src/wall/imp/ini.h:11 at index 27
+constexpr int NSEED = 28;
 struct Trunks {
-    udx_rnd t[27];
+    udx_rnd t[NSEED];
     void ini() {
         for (int f = 0; f < 26; f++) {
             int d[3] = {(f + 2) % 3 - 1, (f / 3 + 2) % 3 - 1, (f / 9 + 2) % 3 - 1};
@@ -48,9 +50,10 @@ struct Trunks {
             udx_rnd_ini(390 + seed, seed + 615, 12309, 23094, &t[f]);
         }
         udx_rnd_ini(0, 0, 0, 0, &t[26]);
+        udx_rnd_ini(42, 42, 42, 42, &t[27]);
     }
-    void next(float out[27]) {
-        for (int f = 0; f < 27; f++) out[f] = udx_rnd_get(&t[f]);
+    void next(float out[NSEED]) {
+        for (int f = 0; f < NSEED; f++) out[f] = udx_rnd_get(&t[f]);
     }
 };
 
@@ -83,6 +86,25 @@ struct udx_fluforces {
     unsigned long long *pairs_dev;
 };
 
+struct udx_sdf {
+    udx_ctx *ctx;
+    SdfField f;
+};
+
+struct udx_wall {
+    udx_ctx *ctx;
+    const udx_sdf *sdf;
+    WallSet w;
+    udx_rnd trunk;              // (42,42,42,42), src/wall/imp/ini.h:11
+    int exact, cap;
+    float *seed_dev;
+    float4 *pos, *vel;          // AoS -> float4 staging for udx_wall_force
+    unsigned long long *pairs_dev;
+};
+
+static const SdfField *wall_sdf(const udx_wall *w) { return &w->sdf->f; }
+static const WallSet *wall_set(const udx_wall *w) { return &w->w; }
+
 struct udx_flu {
     udx_ctx *ctx;
     udx_flu_conf conf;
@@ -93,10 +115,11 @@ struct udx_flu {
     ClistBufs cl;
     FragTables frag;
     Trunks trunks;
-    float *seeds_dev;   // seed_cap x 27
+    float *seeds_dev;   // seed_cap x NSEED
     int seed_cap;
     float *seeds_host;  // pinned staging
-    unsigned long long *pairs_dev;
+    unsigned long long *pairs_dev;   // [0] ordered solvent pairs, [1] solvent-wall pairs
+    struct udx_wall *wall;
     int timing;
     hipEvent_t ev[4];
     float tms[4];
@@ -266,7 +289,7 @@ int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int
 
 int udx_fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic, float *ff) {
     UDX_CHECK(f, UDX_EINVAL, "udx_fluforces_apply: null handle");
-    float seeds[27];
+    float seeds[NSEED];
     f->trunks.next(seeds);
     return udx_fluforces_apply_seeds(f, p, starts, periodic, seeds, ff);
 }
@@ -329,8 +352,8 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
     if (!s) s = dev_alloc(&f->ff, conf->maxp);
     if (!s) s = clist_alloc(&f->cl, f->box, conf->maxp);
     if (!s) s = frag_alloc(&f->frag, f->box);
-    if (!s) s = dev_alloc(&f->pairs_dev, 1);
-    if (!s && hipMemset(f->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess) {
+    if (!s) s = dev_alloc(&f->pairs_dev, 2);
+    if (!s && hipMemset(f->pairs_dev, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
         set_error("udx_flu_ini: memset failed");
         s = UDX_EHIP;
     }
@@ -393,8 +416,8 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
         if (f->seeds_host) (void)hipHostFree(f->seeds_host);
         f->seeds_host = nullptr;
         f->seed_cap = 0;
-        UDX_TRY(dev_alloc(&f->seeds_dev, 27 * (size_t)nsteps));
-        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, 27 * sizeof(float) * (size_t)nsteps));
+        UDX_TRY(dev_alloc(&f->seeds_dev, NSEED * (size_t)nsteps));
+        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, NSEED * sizeof(float) * (size_t)nsteps));
         f->seed_cap = nsteps;
     } else {
         // the staging buffer may still feed an earlier copy
@@ -402,11 +425,11 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
     }
     for (int k = 0; k < nsteps; k++) {
         if (seeds_host)
-            memcpy(f->seeds_host + 27 * k, seeds_host + 27 * k, 27 * sizeof(float));
+            memcpy(f->seeds_host + NSEED * k, seeds_host + NSEED * k, NSEED * sizeof(float));
         else
-            f->trunks.next(f->seeds_host + 27 * k);
+            f->trunks.next(f->seeds_host + NSEED * k);
     }
-    UDX_HIP(hipMemcpyAsync(f->seeds_dev, f->seeds_host, 27 * sizeof(float) * (size_t)nsteps,
+    UDX_HIP(hipMemcpyAsync(f->seeds_dev, f->seeds_host, NSEED * sizeof(float) * (size_t)nsteps,
                            hipMemcpyHostToDevice, s));
     return UDX_OK;
 }
@@ -437,9 +460,15 @@ static int flu_step(udx_flu *f, const float *seeds_dev) {
     d.accumulate = 0;
     d.pairs = f->pairs_dev;
     UDX_TRY(dpd_forces_mode(s, d, f->conf.exact));
+    const SdfField *sdf = nullptr;
+    if (f->wall) {
+        sdf = wall_sdf(f->wall);
+        UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
+                            f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1, f->conf.exact));
+    }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
-    UDX_TRY(move_f4(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff, f->pos[b],
-                    f->vel[b]));
+    UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
+                         f->pos[b], f->vel[b], sdf));
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     f->cur = b;
     return UDX_OK;
@@ -451,7 +480,7 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
     UDX_TRY(flu_seed_table(f, nsteps, seeds_host));
     float acc[3] = {0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
-        UDX_TRY(flu_step(f, f->seeds_dev + 27 * k));
+        UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k));
         if (f->timing) {
             UDX_HIP(hipEventSynchronize(f->ev[3]));
             float t;
@@ -471,11 +500,194 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
 
 int udx_flu_pairs(udx_flu *f, unsigned long long *npairs, int reset) {
     UDX_CHECK(f && npairs, UDX_EINVAL, "udx_flu_pairs: null argument");
-    unsigned long long v = 0;
-    UDX_HIP(hipMemcpyAsync(&v, f->pairs_dev, sizeof v, hipMemcpyDeviceToHost, f->ctx->stream));
+    unsigned long long v[2] = {0, 0};
+    UDX_HIP(hipMemcpyAsync(v, f->pairs_dev, sizeof v, hipMemcpyDeviceToHost, f->ctx->stream));
     UDX_HIP(hipStreamSynchronize(f->ctx->stream));
-    *npairs = v / 2;
+    npairs[0] = v[0] / 2;
     if (reset) UDX_HIP(hipMemsetAsync(f->pairs_dev, 0, sizeof v, f->ctx->stream));
+    return UDX_OK;
+}
+
+int udx_flu_wall_pairs(udx_flu *f, unsigned long long *npairs) {
+    UDX_CHECK(f && npairs, UDX_EINVAL, "udx_flu_wall_pairs: null argument");
+    UDX_HIP(hipMemcpyAsync(npairs, f->pairs_dev + 1, sizeof *npairs, hipMemcpyDeviceToHost, f->ctx->stream));
+    UDX_HIP(hipStreamSynchronize(f->ctx->stream));
+    return UDX_OK;
+}
+
+// ------------------------------------------------------------------ sdf / walls
+int udx_sdf_fin(udx_sdf *q) {
+    if (!q) return UDX_OK;
+    dev_free(q->f.tex);
+    delete q;
+    return UDX_OK;
+}
+
+int udx_sdf_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, udx_sdf **out) {
+    UDX_CHECK(ctx && out && Lx > 0 && Ly > 0 && Lz > 0, UDX_EINVAL, "udx_sdf_ini: bad arguments");
+    udx_sdf *q = new udx_sdf();
+    q->ctx = ctx;
+    q->f.L[0] = Lx;
+    q->f.L[1] = Ly;
+    q->f.L[2] = Lz;
+    sdf_texture_size(q->f.L, q->f.T);
+    const long nt = (long)q->f.T[0] * q->f.T[1] * q->f.T[2];
+    int s = dev_alloc(&q->f.tex, nt);
+    if (s) {
+        udx_sdf_fin(q);
+        return s;
+    }
+    q->f.view.tex = q->f.tex;
+    for (int c = 0; c < 3; c++) q->f.view.T[c] = q->f.T[c];
+    *out = q;
+    return UDX_OK;
+}
+
+int udx_sdf_gen(udx_sdf *q, const float ext[3], const int N[3], const float *D) {
+    UDX_CHECK(q && ext && N && D, UDX_EINVAL, "udx_sdf_gen: null argument");
+    UDX_CHECK(N[0] > 0 && N[1] > 0 && N[2] > 0 && ext[0] > 0, UDX_EINVAL, "udx_sdf_gen: bad field");
+    return sdf_build(q->ctx->stream, &q->f, ext, N, D);
+}
+
+// sdf.dat: "ex ey ez\n" "nx ny nz\n" then nx ny nz float32, x fastest
+// (src/wall/sdf/field/imp/ini.h:19-43)
+int udx_sdf_gen_file(udx_sdf *q, const char *path) {
+    UDX_CHECK(q && path, UDX_EINVAL, "udx_sdf_gen_file: null argument");
+    FILE *fp = fopen(path, "rb");
+    UDX_CHECK(fp, UDX_EINVAL, "udx_sdf_gen_file: cannot open '%s'", path);
+    char line[2048];
+    float ext[3];
+    int N[3], ok = 1;
+    if (!fgets(line, sizeof line, fp) || sscanf(line, "%f %f %f", &ext[0], &ext[1], &ext[2]) != 3) ok = 0;
+    if (ok && (!fgets(line, sizeof line, fp) || sscanf(line, "%d %d %d", &N[0], &N[1], &N[2]) != 3)) ok = 0;
+    std::vector<float> D;
+    if (ok && N[0] > 0 && N[1] > 0 && N[2] > 0) {
+        D.resize((size_t)N[0] * N[1] * N[2]);
+        if (fread(D.data(), sizeof(float), D.size(), fp) != D.size()) ok = 0;
+    } else {
+        ok = 0;
+    }
+    fclose(fp);
+    UDX_CHECK(ok, UDX_EINVAL, "udx_sdf_gen_file: '%s' is not an sdf.dat file", path);
+    return udx_sdf_gen(q, ext, N, D.data());
+}
+
+int udx_sdf_size(const udx_sdf *q, int T[3], float *cheap_threshold) {
+    UDX_CHECK(q, UDX_EINVAL, "udx_sdf_size: null handle");
+    for (int c = 0; c < 3; c++) T[c] = q->f.T[c];
+    if (cheap_threshold) *cheap_threshold = q->f.view.cheap_threshold;
+    return UDX_OK;
+}
+
+const float *udx_sdf_texture(const udx_sdf *q) { return q ? q->f.tex : nullptr; }
+
+int udx_sdf_probe(const udx_sdf *q, int n, const float *r3, int what, float *out) {
+    UDX_CHECK(q && q->f.ready && n >= 0 && what >= 0 && what <= 2, UDX_EINVAL, "udx_sdf_probe: bad arguments");
+    return sdf_probe(q->ctx->stream, &q->f, n, r3, what, out);
+}
+
+int udx_wall_fin(udx_wall *w) {
+    if (!w) return UDX_OK;
+    wall_free(&w->w);
+    dev_free(w->seed_dev);
+    dev_free(w->pos);
+    dev_free(w->vel);
+    dev_free(w->pairs_dev);
+    delete w;
+    return UDX_OK;
+}
+
+int udx_wall_gen(udx_ctx *ctx, const udx_sdf *q, int *n, float *pp, udx_wall **out) {
+    UDX_CHECK(ctx && q && q->f.ready && n && out, UDX_EINVAL, "udx_wall_gen: bad arguments");
+    udx_wall *w = new udx_wall();
+    w->ctx = ctx;
+    w->sdf = q;
+    udx_rnd_ini(42, 42, 42, 42, &w->trunk);
+    int s = wall_build(ctx->stream, &w->w, &q->f, n, pp);
+    if (!s) s = dev_alloc(&w->seed_dev, 1);
+    if (!s) s = dev_alloc(&w->pairs_dev, 1);
+    if (!s && hipMemset(w->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess) s = UDX_EHIP;
+    if (s) {
+        udx_wall_fin(w);
+        return s;
+    }
+    *out = w;
+    return UDX_OK;
+}
+
+int udx_wall_counts(const udx_wall *w, int *nfrozen, int *ntotal) {
+    UDX_CHECK(w, UDX_EINVAL, "udx_wall_counts: null handle");
+    if (nfrozen) *nfrozen = w->w.nfrozen;
+    if (ntotal) *ntotal = w->w.n;
+    return UDX_OK;
+}
+
+const float *udx_wall_pp(const udx_wall *w) { return w ? w->w.pp_aos : nullptr; }
+const int *udx_wall_starts(const udx_wall *w) { return w ? w->w.cl.starts : nullptr; }
+
+int udx_wall_set_mode(udx_wall *w, int exact) {
+    UDX_CHECK(w, UDX_EINVAL, "udx_wall_set_mode: null handle");
+    w->exact = exact ? 1 : 0;
+    return UDX_OK;
+}
+
+int udx_wall_force_seed(udx_wall *w, const udx_pair_dpd *p, int n, const float *pp, float seed, float *ff) {
+    UDX_CHECK(w && p && n >= 0, UDX_EINVAL, "udx_wall_force_seed: bad arguments");
+    if (n == 0) return UDX_OK;
+    hipStream_t s = w->ctx->stream;
+    if (n > w->cap) {
+        UDX_HIP(hipStreamSynchronize(s));
+        dev_free(w->pos);
+        dev_free(w->vel);
+        w->cap = 0;
+        UDX_TRY(dev_alloc(&w->pos, n));
+        UDX_TRY(dev_alloc(&w->vel, n));
+        w->cap = n;
+    }
+    UDX_TRY(aos_to_f4(s, n, pp, w->pos, w->vel));
+    UDX_HIP(hipMemcpyAsync(w->seed_dev, &seed, sizeof seed, hipMemcpyHostToDevice, s));
+    // n as the device-side count: the first element of a one-int buffer
+    int *n_dev = nullptr;
+    UDX_TRY(dev_alloc(&n_dev, 1));
+    int st = UDX_OK;
+    if (hipMemcpyAsync(n_dev, &n, sizeof n, hipMemcpyHostToDevice, s) != hipSuccess) st = UDX_EHIP;
+    if (!st)
+        st = wall_forces(s, &w->sdf->f, &w->w, n, n_dev, w->pos, w->vel, *p, w->seed_dev, ff, 3, w->pairs_dev,
+                         w->exact);
+    if (!st && hipStreamSynchronize(s) != hipSuccess) st = UDX_EHIP;
+    dev_free(n_dev);
+    if (st) set_error("udx_wall_force: launch failed");
+    return st;
+}
+
+int udx_wall_force(udx_wall *w, const udx_pair_dpd *p, int n, const float *pp, float *ff) {
+    UDX_CHECK(w, UDX_EINVAL, "udx_wall_force: null handle");
+    return udx_wall_force_seed(w, p, n, pp, udx_rnd_get(&w->trunk), ff);
+}
+
+int udx_wall_pairs(udx_wall *w, unsigned long long *npairs, int reset) {
+    UDX_CHECK(w && npairs, UDX_EINVAL, "udx_wall_pairs: null argument");
+    UDX_HIP(hipMemcpyAsync(npairs, w->pairs_dev, sizeof *npairs, hipMemcpyDeviceToHost, w->ctx->stream));
+    UDX_HIP(hipStreamSynchronize(w->ctx->stream));
+    if (reset) UDX_HIP(hipMemsetAsync(w->pairs_dev, 0, sizeof *npairs, w->ctx->stream));
+    return UDX_OK;
+}
+
+int udx_wall_bounce(const udx_sdf *q, float dt, int n, float *pp) {
+    UDX_CHECK(q && q->f.ready, UDX_EINVAL, "udx_wall_bounce: sdf not generated");
+    return wall_bounce_aos(q->ctx->stream, &q->f, dt, n, pp);
+}
+
+int udx_wall_repulse(const udx_sdf *q, float lambda, int n, const float *pp, float *ff) {
+    UDX_CHECK(q && q->f.ready, UDX_EINVAL, "udx_wall_repulse: sdf not generated");
+    return wall_repulse_aos(q->ctx->stream, &q->f, lambda, n, pp, ff);
+}
+
+int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_wall: null handle");
+    UDX_CHECK(!w || (w->sdf->f.L[0] == f->box.L[0] && w->sdf->f.L[1] == f->box.L[1] && w->sdf->f.L[2] == f->box.L[2]),
+              UDX_EINVAL, "udx_flu_set_wall: wall and solvent boxes differ");
+    f->wall = w;
     return UDX_OK;
 }
 

@@ -41,6 +41,31 @@ __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, flo
     vel[i] = v;
 }
 
+// the same with the wall bounce-back between the drift and the wrap (the
+// reference bounces after update_solvent, before the next redistribution,
+// src/sim/imp/run.h:43-54)
+__global__ __launch_bounds__(256) void k_move_f4_wall(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
+                                                      float dt, float mass, float ax, float ay, float az,
+                                                      const float4 *ff, float4 *pos, float4 *vel, SdfView sdf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nmax || i >= *n_dev) return;
+    float4 p = pos[i], v = vel[i];
+    const float4 f = ff[i];
+    kick_drift(dt, mass, ax, f.x, &p.x, &v.x);
+    kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
+    kick_drift(dt, mass, az, f.z, &p.z, &v.z);
+    float r[3] = {p.x, p.y, p.z}, u[3] = {v.x, v.y, v.z};
+    if (sdf_bounce(sdf, dt, r, u)) {
+        p.x = r[0]; p.y = r[1]; p.z = r[2];
+        v.x = u[0]; v.y = u[1]; v.z = u[2];
+    }
+    p.x = wrap_coord(p.x, Lx);
+    p.y = wrap_coord(p.y, Ly);
+    p.z = wrap_coord(p.z, Lz);
+    pos[i] = p;
+    vel[i] = v;
+}
+
 // reference kick-drift on AoS particles, no wrap (src/scheme/move/dev/main.h:1-12)
 __global__ __launch_bounds__(256) void k_move_aos(int n, float dt, float mass, const float *ff, float *pp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -182,6 +207,16 @@ int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, f
     if (nmax <= 0) return UDX_OK;
     hipLaunchKernelGGL(k_move_f4, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
                        (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q) {
+    if (!q) return move_f4(s, nmax, n_dev, b, dt, mass, accel, ff, pos, vel);
+    if (nmax <= 0) return UDX_OK;
+    hipLaunchKernelGGL(k_move_f4_wall, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
+                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, q->view);
     UDX_LAUNCHED();
     return UDX_OK;
 }

@@ -174,4 +174,137 @@ __device__ __forceinline__ int frag_cell(int dx, int dy, int dz, int cx, int cy,
 // ---------------------------------------------------------------- waves
 __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 
+// ---------------------------------------------------------------- SDF
+// wall margin in cells (XWM/YWM/ZWM, src/inc/def.h:15-18)
+constexpr int WALL_MARGIN = 6;
+
+// texture of the signed distance (positive inside the walls) with the map
+// from local position to texture coordinate t = s r + o
+struct SdfView {
+    const float *tex;
+    int T[3];
+    float o[3], s[3];
+    float cheap_threshold;
+};
+
+// point fetch with wrap addressing (src/wall/sdf/tex3d/imp/main.h:12-29)
+__device__ __forceinline__ float sdf_fetch(const SdfView &v, int i, int j, int k) {
+    i %= v.T[0]; if (i < 0) i += v.T[0];
+    j %= v.T[1]; if (j < 0) j += v.T[1];
+    k %= v.T[2]; if (k < 0) k += v.T[2];
+    return v.tex[i + v.T[0] * ((long)j + (long)v.T[1] * k)];
+}
+
+__device__ __forceinline__ void sdf_tex(const SdfView &v, float x, float y, float z, float t[3]) {
+    t[0] = __builtin_fmaf(v.s[0], x, v.o[0]);
+    t[1] = __builtin_fmaf(v.s[1], y, v.o[1]);
+    t[2] = __builtin_fmaf(v.s[2], z, v.o[2]);
+}
+
+// trilinear interpolation between texel centres (src/wall/sdf/dev.h:79-112)
+__device__ __forceinline__ float sdf_eval(const SdfView &v, float x, float y, float z) {
+    float t[3], l[3];
+    int c0[3];
+    sdf_tex(v, x, y, z, t);
+    for (int c = 0; c < 3; c++) {
+        c0[c] = (int)t[c];
+        l[c] = t[c] - (float)c0[c];
+    }
+    const float s000 = sdf_fetch(v, c0[0], c0[1], c0[2]), s001 = sdf_fetch(v, c0[0] + 1, c0[1], c0[2]);
+    const float s010 = sdf_fetch(v, c0[0], c0[1] + 1, c0[2]), s011 = sdf_fetch(v, c0[0] + 1, c0[1] + 1, c0[2]);
+    const float s100 = sdf_fetch(v, c0[0], c0[1], c0[2] + 1), s101 = sdf_fetch(v, c0[0] + 1, c0[1], c0[2] + 1);
+    const float s110 = sdf_fetch(v, c0[0], c0[1] + 1, c0[2] + 1), s111 = sdf_fetch(v, c0[0] + 1, c0[1] + 1, c0[2] + 1);
+    // A (1 - p) + p B with the first product fused into the sum
+    auto wav = [](float A, float B, float p) { return __builtin_fmaf(A, 1 - p, p * B); };
+    const float s00x = wav(s000, s001, l[0]), s01x = wav(s010, s011, l[0]);
+    const float s10x = wav(s100, s101, l[0]), s11x = wav(s110, s111, l[0]);
+    const float s0yx = wav(s00x, s01x, l[1]), s1yx = wav(s10x, s11x, l[1]);
+    return wav(s0yx, s1yx, l[2]);
+}
+
+// forward difference (src/wall/sdf/dev.h:32-46)
+__device__ __forceinline__ void sdf_grad(const SdfView &v, float x, float y, float z, float g[3]) {
+    float t[3];
+    sdf_tex(v, x, y, z, t);
+    const int i = (int)t[0], j = (int)t[1], k = (int)t[2];
+    const float s0 = sdf_fetch(v, i, j, k);
+    g[0] = v.s[0] * (sdf_fetch(v, i + 1, j, k) - s0);
+    g[1] = v.s[1] * (sdf_fetch(v, i, j + 1, k) - s0);
+    g[2] = v.s[2] * (sdf_fetch(v, i, j, k + 1) - s0);
+}
+
+__device__ __forceinline__ void sdf_ugrad(const SdfView &v, float x, float y, float z, float g[3]) {
+    sdf_grad(v, x, y, z, g);
+    const float mag = __builtin_sqrtf(__builtin_fmaf(g[2], g[2], __builtin_fmaf(g[0], g[0], g[1] * g[1])));
+    if (mag > 1e-6f) {
+        g[0] = g[0] / mag;
+        g[1] = g[1] / mag;
+        g[2] = g[2] / mag;
+    }
+}
+
+__device__ __forceinline__ int sdf_iround(float x) { return (x > 0.5f) ? (int)(x + 0.5f) : (int)(x - 0.5f); }
+
+// nearest texel (src/wall/sdf/dev.h:61-75)
+__device__ __forceinline__ float sdf_cheap(const SdfView &v, float x, float y, float z) {
+    float t[3];
+    sdf_tex(v, x, y, z, t);
+    return sdf_fetch(v, sdf_iround(t[0]), sdf_iround(t[1]), sdf_iround(t[2]));
+}
+
+__device__ __forceinline__ bool sdf_far(const SdfView &v, float x, float y, float z) {
+    return sdf_cheap(v, x, y, z) <= v.cheap_threshold - 1;
+}
+
+// bounce-back with a resting wall (src/wall/sdf/bounce/dev/main.h:1-114);
+// returns true when the particle was inside and has been moved
+__device__ __forceinline__ bool sdf_bounce(const SdfView &v, float dt, float r[3], float u[3]) {
+    if (sdf_cheap(v, r[0], r[1], r[2]) < v.cheap_threshold) return false;
+    const float currsdf = sdf_eval(v, r[0], r[1], r[2]);
+    if (!(currsdf >= 0)) return false;
+    auto axpy = [](float a, const float *x, float *y) {
+        for (int c = 0; c < 3; c++) y[c] = __builtin_fmaf(a, x[c], y[c]);
+    };
+    float r0[3] = {r[0], r[1], r[2]};
+    axpy(-dt, u, r0);
+    if (sdf_eval(v, r0[0], r0[1], r0[2]) >= 0) {
+        // rescue: walk back along the gradient
+        float g[3];
+        sdf_ugrad(v, r[0], r[1], r[2], g);
+        const float sdf0 = currsdf;
+        axpy(-sdf0, g, r);
+        for (int l = 8; l >= 1; --l) {
+            if (sdf_eval(v, r[0], r[1], r[2]) < 0) {
+                for (int c = 0; c < 3; c++) u[c] = 0.0f - u[c];
+                return true;
+            }
+            const float jump = 1.1f * sdf0 / (float)(1 << l);
+            axpy(-jump, g, r);
+        }
+        return true;
+    }
+    // Newton iterations on sdf(r + t u) = 0, t in [-dt, 0]
+    float t = 0;
+    for (int l = 0; l < 2; l++) {
+        float rc[3] = {r[0], r[1], r[2]}, g[3];
+        axpy(t, u, rc);
+        const float phi = sdf_eval(v, rc[0], rc[1], rc[2]);
+        sdf_grad(v, rc[0], rc[1], rc[2], g);
+        const float dphi = __builtin_fmaf(u[2], g[2], __builtin_fmaf(u[0], g[0], u[1] * g[1]));
+        if (__builtin_fabsf(dphi) < 1e-6f) break;
+        t -= phi / dphi;
+        if (t < -dt) t = -dt;
+        if (t > 0) t = 0;
+    }
+    float rw[3] = {r[0], r[1], r[2]};
+    axpy(t, u, rw);
+    for (int c = 0; c < 3; c++) u[c] = 0.0f - u[c];
+    r[0] = rw[0]; r[1] = rw[1]; r[2] = rw[2];
+    axpy(-t, u, r);
+    if (sdf_eval(v, r[0], r[1], r[2]) >= 0) {
+        r[0] = r0[0]; r[1] = r0[1]; r[2] = r0[2];
+    }
+    return true;
+}
+
 }  // namespace udx

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/udx.h"
+#include "udx_dev.h"
 
 namespace udx {
 
@@ -130,6 +131,32 @@ int move_aos(hipStream_t s, int n, float dt, float mass, const float *ff, float 
 int bforce_aos(hipStream_t s, int n, float mass, const float accel[3], float *ff);
 int wrap_aos(hipStream_t s, const Box &b, int n, float *pp);
 int rnd_branch(hipStream_t s, float seed, int n, const int *u, const int *v, int uu, float *out);
+
+// ------------------------------------------------------------------ walls
+struct SdfField {
+    int L[3], T[3];
+    float *tex;      // T[0] T[1] T[2] texels, x fastest
+    SdfView view;    // device view (tex pointer, grid, transform, threshold)
+    int ready;
+};
+struct WallSet {
+    ClistBufs cl;    // cell list of the wall particles on the margin box
+    float4 *pos;     // cell-ordered wall positions
+    float *pp_aos;   // the same in AoS (velocity 0)
+    int n, nfrozen;  // all wall particles (frozen + periodic images), frozen ones
+};
+void sdf_texture_size(const int L[3], int T[3]);
+int sdf_build(hipStream_t s, SdfField *q, const float ext[3], const int N[3], const float *D_host);
+int sdf_probe(hipStream_t s, const SdfField *q, int n, const float *r3, int what, float *out);
+int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp);
+void wall_free(WallSet *w);
+int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, const int *n_dev, const float4 *pos,
+                const float4 *vel, const udx_pair_dpd &p, const float *seed_dev, float *ff, int ff_stride,
+                unsigned long long *pairs, int exact);
+int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp);
+int wall_repulse_aos(hipStream_t s, const SdfField *q, float lambda, int n, const float *pp, float *ff);
+int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q);
 int diag_aos(hipStream_t s, int n, const float *pp, double *partials_dev, double out[5]);
 
 }  // namespace udx

@@ -65,6 +65,26 @@ _SIGS = {
     "udx_flu_run": (_I, [_VP, _I, C.POINTER(_F)]),
     "udx_flu_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
     "udx_flu_timing": (_I, [_VP, _I, C.POINTER(_F)]),
+    "udx_flu_wall_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong)]),
+    "udx_flu_set_wall": (_I, [_VP, _VP]),
+    "udx_sdf_ini": (_I, [_VP, _I, _I, _I, C.POINTER(_VP)]),
+    "udx_sdf_fin": (_I, [_VP]),
+    "udx_sdf_gen": (_I, [_VP, C.POINTER(_F), C.POINTER(_I), C.POINTER(_F)]),
+    "udx_sdf_gen_file": (_I, [_VP, C.c_char_p]),
+    "udx_sdf_size": (_I, [_VP, C.POINTER(_I), C.POINTER(_F)]),
+    "udx_sdf_texture": (_VP, [_VP]),
+    "udx_sdf_probe": (_I, [_VP, _I, _VP, _I, _VP]),
+    "udx_wall_gen": (_I, [_VP, _VP, C.POINTER(_I), _VP, C.POINTER(_VP)]),
+    "udx_wall_fin": (_I, [_VP]),
+    "udx_wall_counts": (_I, [_VP, C.POINTER(_I), C.POINTER(_I)]),
+    "udx_wall_pp": (_VP, [_VP]),
+    "udx_wall_starts": (_VP, [_VP]),
+    "udx_wall_set_mode": (_I, [_VP, _I]),
+    "udx_wall_force": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _VP]),
+    "udx_wall_force_seed": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _F, _VP]),
+    "udx_wall_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
+    "udx_wall_bounce": (_I, [_VP, _F, _I, _VP]),
+    "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -279,13 +299,29 @@ class Flu:
         _ok(lib().udx_flu_get_ff(self.h, _ptr(out)))
 
     def run(self, nsteps, seeds=None):
+        """seeds: None (internal trunks) or an (nsteps, 27|28) table; a
+        27-column table gets the wall trunk (42,42,42,42) as column 27."""
         if seeds is None:
             _ok(lib().udx_flu_run(self.h, nsteps, None))
         else:
             import numpy as np
             s = np.ascontiguousarray(seeds, dtype=np.float32)
-            assert s.shape == (nsteps, 27)
+            if s.shape == (nsteps, 27):
+                if not hasattr(self, "_wall_trunk"):
+                    self._wall_trunk = RndTrunk(42, 42, 42, 42)
+                w = np.array([self._wall_trunk.get() for _ in range(nsteps)], np.float32)
+                s = np.ascontiguousarray(np.concatenate([s, w[:, None]], axis=1))
+            assert s.shape == (nsteps, 28)
             _ok(lib().udx_flu_run(self.h, nsteps, s.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def set_wall(self, wall):
+        self.wall = wall
+        _ok(lib().udx_flu_set_wall(self.h, wall.h if wall is not None else None))
+
+    def wall_pairs(self):
+        v = C.c_ulonglong()
+        _ok(lib().udx_flu_wall_pairs(self.h, C.byref(v)))
+        return v.value
 
     def pairs(self, reset=False):
         v = C.c_ulonglong()
@@ -300,4 +336,97 @@ class Flu:
     def close(self):
         if self.h:
             lib().udx_flu_fin(self.h)
+            self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- walls
+class Sdf:
+    """sdf_ini / sdf_gen (src/wall/sdf/imp.h): signed distance of the walls."""
+
+    def __init__(self, ctx, L, ext=None, N=None, D=None, path=None):
+        self.ctx, self.L = ctx, tuple(L)
+        self.h = C.c_void_p()
+        _ok(lib().udx_sdf_ini(ctx.h, L[0], L[1], L[2], C.byref(self.h)))
+        if path is not None:
+            _ok(lib().udx_sdf_gen_file(self.h, os.fsencode(path)))
+        elif D is not None:
+            import numpy as np
+            e = (C.c_float * 3)(*[float(x) for x in ext])
+            n = (C.c_int * 3)(*[int(x) for x in N])
+            d = np.ascontiguousarray(D, np.float32)
+            _ok(lib().udx_sdf_gen(self.h, e, n, d.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def size(self):
+        T = (C.c_int * 3)()
+        thr = C.c_float()
+        _ok(lib().udx_sdf_size(self.h, T, C.byref(thr)))
+        return tuple(T), thr.value
+
+    def probe(self, r3, what=0):
+        """what 0: sdf_eval, 1: nearest texel, 2: gradient."""
+        import torch
+        n = r3.shape[0]
+        out = torch.empty((n, 3) if what == 2 else (n,), dtype=torch.float32, device="cuda")
+        _ok(lib().udx_sdf_probe(self.h, n, _ptr(r3), what, _ptr(out)))
+        return out
+
+    def bounce(self, dt, pp):
+        _ok(lib().udx_wall_bounce(self.h, dt, pp.shape[0], _ptr(pp)))
+
+    def repulse(self, lam, pp, ff):
+        _ok(lib().udx_wall_repulse(self.h, lam, pp.shape[0], _ptr(pp), _ptr(ff)))
+
+    def close(self):
+        if self.h:
+            lib().udx_sdf_fin(self.h)
+            self.h = C.c_void_p()
+
+
+class Wall:
+    """wall_gen + wall_force (src/wall/imp.h): frozen wall particles."""
+
+    def __init__(self, ctx, sdf, pp, exact=False):
+        """pp: solvent (n, 6) cuda tensor; on return the first self.n_solvent
+        rows hold the solvent that stays (order kept)."""
+        self.ctx, self.sdf = ctx, sdf
+        self.h = C.c_void_p()
+        n = C.c_int(pp.shape[0])
+        _ok(lib().udx_wall_gen(ctx.h, sdf.h, C.byref(n), _ptr(pp), C.byref(self.h)))
+        self.n_solvent = n.value
+        _ok(lib().udx_wall_set_mode(self.h, 1 if exact else 0))
+
+    def counts(self):
+        a, b = C.c_int(), C.c_int()
+        _ok(lib().udx_wall_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def particles(self):
+        import torch
+        _, n = self.counts()
+        out = torch.empty((n, 6), dtype=torch.float32, device="cuda")
+        _copy_dev(out, lib().udx_wall_pp(self.h), n * 24)
+        return out
+
+    def starts(self):
+        import torch
+        W = [x + 12 for x in self.sdf.L]
+        nc = W[0] * W[1] * W[2]
+        out = torch.empty(nc + 1, dtype=torch.int32, device="cuda")
+        _copy_dev(out, lib().udx_wall_starts(self.h), (nc + 1) * 4)
+        return out
+
+    def force(self, params, pp, ff, seed=None):
+        if seed is None:
+            _ok(lib().udx_wall_force(self.h, C.byref(params), pp.shape[0], _ptr(pp), _ptr(ff)))
+        else:
+            _ok(lib().udx_wall_force_seed(self.h, C.byref(params), pp.shape[0], _ptr(pp), float(seed), _ptr(ff)))
+
+    def pairs(self, reset=False):
+        v = C.c_ulonglong()
+        _ok(lib().udx_wall_pairs(self.h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().udx_wall_fin(self.h)
             self.h = C.c_void_p()

@@ -124,9 +124,54 @@ int udx_flu_set_pp(udx_flu *f, int n, const float *pp);       /* device AoS in  
 int udx_flu_get_pp(udx_flu *f, float *pp, int *n);            /* device AoS out */
 int udx_flu_get_ff(udx_flu *f, float *ff);                    /* last step forces, AoS */
 /* advance nsteps steps; seeds come from the internal trunks unless a table
- * is given (seeds_host: nsteps x 27 floats, layout as udx_fluforces_apply_seeds) */
+ * is given (seeds_host: nsteps x 28 floats: [0..26] as in
+ * udx_fluforces_apply_seeds, [27] the wall seed) */
 int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host);
 int udx_flu_pairs(udx_flu *f, unsigned long long *npairs, int reset);
+/* solvent-wall pairs counted since the last reset of udx_flu_pairs */
+int udx_flu_wall_pairs(udx_flu *f, unsigned long long *npairs);
+
+/* ---- SDF walls: replace sdf_ini / sdf_gen / sdf_get_view / sdf_bounce
+ *      (src/wall/sdf/imp.h:9-25), wall_gen / wall_interact / wall_bounce
+ *      (src/sim/walls/imp.h:14-31) and wall_force / wall_repulse
+ *      (src/wall/imp.h:42-48); resting walls (wall velocity 0) ---- */
+typedef struct udx_sdf udx_sdf;
+int udx_sdf_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, udx_sdf **q);
+int udx_sdf_fin(udx_sdf *q);
+/* field D (host, N[0] N[1] N[2] floats, x fastest) over the box with
+ * extents ext: scaled by Lx / ext[0] and resampled with a periodic cubic
+ * B-spline onto the texture grid (src/wall/sdf/imp/gen.h) */
+int udx_sdf_gen(udx_sdf *q, const float ext[3], const int N[3], const float *D);
+int udx_sdf_gen_file(udx_sdf *q, const char *path);            /* sdf.dat format */
+int udx_sdf_size(const udx_sdf *q, int T[3], float *cheap_threshold);
+const float *udx_sdf_texture(const udx_sdf *q);                /* device, T[0] T[1] T[2] */
+/* r3: n positions (device, 3 floats each); what = 0: sdf_eval, 1: nearest
+ * texel, 2: gradient (3 floats per position) (src/wall/sdf/dev.h) */
+int udx_sdf_probe(const udx_sdf *q, int n, const float *r3, int what, float *out);
+
+typedef struct udx_wall udx_wall;
+/* freeze (src/wall/imp/generate.h:1-60): pp (device AoS, *n particles) keeps
+ * the solvent (sdf < 0) in order and *n is updated; particles with
+ * 0 <= sdf <= 2 become wall particles, deeper ones are dropped; the wall
+ * set holds them and their periodic images in the 6-cell margin, cell-sorted */
+int udx_wall_gen(udx_ctx *ctx, const udx_sdf *q, int *n, float *pp, udx_wall **w);
+int udx_wall_fin(udx_wall *w);
+int udx_wall_counts(const udx_wall *w, int *nfrozen, int *ntotal);
+const float *udx_wall_pp(const udx_wall *w);       /* device AoS, cell order on the margin box */
+const int *udx_wall_starts(const udx_wall *w);     /* device, margin-box cells + 1 */
+int udx_wall_set_mode(udx_wall *w, int exact);     /* as udx_fluforces_set_mode */
+/* solvent-wall DPD on n cell-ordered solvent particles (device AoS), forces
+ * added to ff; the seed is one draw of the wall trunk, or given */
+int udx_wall_force(udx_wall *w, const udx_pair_dpd *p, int n, const float *pp, float *ff);
+int udx_wall_force_seed(udx_wall *w, const udx_pair_dpd *p, int n, const float *pp, float seed, float *ff);
+int udx_wall_pairs(udx_wall *w, unsigned long long *npairs, int reset);
+/* bounce-back of particles inside the walls (src/wall/sdf/bounce/dev/main.h) */
+int udx_wall_bounce(const udx_sdf *q, float dt, int n, float *pp);
+/* wall repulsion of object particles, f += -min(1e4, e^{l(s+1)} - 1) grad s / |grad s| */
+int udx_wall_repulse(const udx_sdf *q, float lambda, int n, const float *pp, float *ff);
+/* walls in the solvent step: wall DPD after the solvent forces, bounce-back
+ * after the drift (NULL removes them) */
+int udx_flu_set_wall(udx_flu *f, udx_wall *w);
 /* per-step kernel timing of the last udx_flu_run (ms, HIP events on the
  * bound stream): [0] cell list, [1] forces, [2] integrate, [3] whole steps */
 int udx_flu_timing(udx_flu *f, int enable, float out[4]);

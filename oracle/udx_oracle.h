@@ -98,6 +98,43 @@ long orc_flu_step(const int L[3], int n, float *pp, float *scratch_pp, int *star
                   float *ff, orc_dpd p, const float seeds[27], float dt, float mass,
                   const float accel[3], int nthreads);
 
+/* ---- SDF walls (udx_oracle_wall.c) -------------------------------------- */
+/* wall margin in cells (XWM/YWM/ZWM, src/inc/def.h:15-18) */
+#define ORC_WALL_MARGIN 6
+typedef struct {
+    int L[3], T[3];          /* box, texture grid                          */
+    float o[3], s[3];        /* local position -> texture coordinate        */
+    float cheap_threshold;
+    float *tex;              /* T[0] T[1] T[2] values, x fastest (caller)   */
+} orc_sdf;
+void  orc_tform_grid2grid(const float flo[3], const float fhi[3], const int fn[3], const float tlo[3],
+                          const float thi[3], const int tn[3], float o[3], float s[3]);
+/* texture grid of src/wall/sdf/imp/main.h:1-18 */
+void  orc_sdf_texture_size(const int L[3], int T[3]);
+/* sdf_gen: scale the sdf.dat field (extents ext, grid N) to the box and
+ * resample it with a periodic cubic B-spline onto the texture grid
+ * (src/wall/sdf/imp/gen.h, src/wall/sdf/field/imp/sample.h) */
+void  orc_sdf_gen(const int L[3], const float ext[3], const int N[3], const float *D, orc_sdf *q);
+void  orc_sdf_sample_box(const int L[3], const float ext[3], const int N[3], const float *D, float *out);
+float orc_sdf_eval(const orc_sdf *q, float x, float y, float z);          /* src/wall/sdf/dev.h:79-112 */
+void  orc_sdf_grad(const orc_sdf *q, float x, float y, float z, float g[3]); /* :32-46 */
+float orc_sdf_cheap(const orc_sdf *q, float x, float y, float z);         /* :61-66 */
+int   orc_sdf_far(const orc_sdf *q, float x, float y, float z);           /* :70-75 */
+/* freeze (src/wall/imp/generate.h:1-60): solvent keeps s < 0, returns the
+ * number of frozen wall particles (0 <= s <= 2) written to wall */
+int   orc_wall_freeze(const orc_sdf *q, int *n, float *pp, float *wall);
+/* frozen particles + periodic images in the margin box (src/wall/exch/imp.cpp) */
+int   orc_wall_images(const int L[3], int nw, const float *wall, float *out);
+/* solvent-wall DPD (src/wall/force/dev/main.h:1-50); wall clist on the
+ * margin box L + 2 margin */
+long  orc_wall_forces(const int L[3], int n, const float *pp, const orc_sdf *q, int nw, const float *wpp,
+                      const int *wstarts, orc_dpd p, float seed, float *ff);
+void  orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp);     /* src/wall/sdf/bounce/dev/main.h */
+/* kick-drift (+ body force), bounce-back when q != NULL, periodic wrap */
+void  orc_integrate_wall(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
+                         float *pp, const orc_sdf *q);
+void  orc_wall_repulse(const orc_sdf *q, float lambda, int n, const float *pp, float *ff);
+
 #ifdef __cplusplus
 }
 #endif

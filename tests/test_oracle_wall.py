@@ -1,0 +1,169 @@
+"""The wall oracle (oracle/udx_oracle_wall.c) against the reference's SDF
+golden files, closed forms and brute force.
+
+Fixtures: tests/golden/ref/sdf/{cyl,wall}.dat are the reference's
+src/data/sdf/{cyl1/cyl.dat, wall1/wall.dat}; sdf.ref.sdf.0 and
+wall.ref.sdf.1 are its src/test_data files for src/test/sdf/{0,1}.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import wall as W
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
+
+
+def cyl():
+    return W.read_sdf_dat(os.path.join(GOLD, "sdf", "cyl.dat"))
+
+
+def test_sdf_eval_golden():
+    """sdf.0: sdf_eval at (0, 0, z) of cyl.dat in a 16^3 box, printed %g."""
+    ext, N, D = cyl()
+    s = W.Sdf((16, 16, 16), ext, N, D)
+    want = [l.split() for l in open(os.path.join(GOLD, "sdf.ref.sdf.0.txt")).read().split("\n") if l.strip()]
+    for row in want:
+        z = float(row[2])
+        got = "%g" % s.eval([0.0, 0.0, z])[0]
+        assert got == row[3], (z, got, row[3])
+
+
+def test_sdf_resample_golden():
+    """sdf.1: the scaled field B-spline-sampled at the 16^3 cell centres
+    (the wall.h5 dump), printed %.10e."""
+    ext, N, D = cyl()
+    got = W.sample_box((16, 16, 16), ext, N, D)
+    ref = np.loadtxt(os.path.join(GOLD, "wall.ref.sdf.1.txt"), skiprows=1)
+    assert ref.size == got.size
+    assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
+
+
+def test_texture_grid_size():
+    """src/wall/sdf/imp/main.h:1-18 for the benchmark box and odd sizes."""
+    for L, T in [((96, 96, 96), (256, 256, 256)), ((16, 16, 16), (256, 256, 256)), ((32, 16, 8), (256, 176, 128))]:
+        ext = np.array(L, np.float32)
+        s = W.Sdf(L, ext, (4, 4, 4), np.zeros(64, np.float32))
+        assert s.T == T
+
+
+def test_plane_channel_linear_precision():
+    """A cubic B-spline reproduces linear functions and trilinear
+    interpolation of a linear field is exact, so away from the kinks of
+    |z| - h the texture gives |z| - h."""
+    L = (24, 20, 32)
+    Dn = W.plane_channel_sdf(L, 64, 10.0)
+    s = W.Sdf(L, np.array(L, np.float32), (64, 64, 64), Dn)
+    rng = np.random.default_rng(0)
+    r = np.zeros((400, 3), np.float32)
+    r[:, 0] = rng.uniform(-12, 12, 400)
+    r[:, 1] = rng.uniform(-10, 10, 400)
+    r[:, 2] = rng.choice([-1, 1], 400) * rng.uniform(2, 14, 400)
+    got = s.eval(r)
+    assert np.max(np.abs(got - (np.abs(r[:, 2]) - 10.0))) < 2e-4
+    g = s.grad(r)
+    assert np.allclose(g[:, 2], np.sign(r[:, 2]), atol=2e-3)
+    # tangential components: FP32 rounding of the resampled texels divided by
+    # the texel size
+    assert np.allclose(g[:, :2], 0, atol=5e-4)
+
+
+def channel(L=(16, 12, 24), h=8.0, rho=4, seed=1):
+    Dn = W.plane_channel_sdf(L, 48, h)
+    s = W.Sdf(L, np.array(L, np.float32), (48, 48, 48), Dn)
+    rng = np.random.default_rng(seed)
+    n = int(rho * np.prod(L))
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, 0.3, (n, 3)).astype(np.float32)
+    return s, pp
+
+
+def test_freeze_labels_and_order():
+    s, pp = channel()
+    keep, wall = s.freeze(pp)
+    sd = s.eval(pp[:, :3])
+    assert np.array_equal(keep, pp[sd < 0])
+    assert np.array_equal(wall, pp[(sd >= 0) & (sd <= 2)])
+
+
+def test_wall_images_brute_force():
+    L = (16, 12, 24)
+    s, pp = channel(L)
+    _, wall = s.freeze(pp)
+    out = W.wall_images(L, wall)
+    assert np.array_equal(out[:len(wall)], wall)
+    lw = 0.5 * (np.array(L, np.float64) + 12)
+    want = []
+    for j in range(26):
+        d = -np.array(O.frag_i2d3(j))
+        for p in wall:
+            r = p[:3] + (d * np.array(L)).astype(np.float32)
+            if np.all((r.astype(np.float64) >= -lw) & (r.astype(np.float64) < lw)):
+                want.append(np.concatenate([r, p[3:]]))
+    assert np.array_equal(out[len(wall):], np.array(want, np.float32).reshape(-1, 6))
+
+
+def wall_setup(L=(16, 12, 24), h=8.0, rho=4, seed=1):
+    s, pp = channel(L, h, rho, seed)
+    keep, wall = s.freeze(pp)
+    allw = W.wall_images(L, wall)
+    ws, wc, wsorted, _ = O.clist_build(W.margin_box(L), allw)
+    starts, counts, srt, _ = O.clist_build(L, keep)
+    return s, srt, wsorted, ws
+
+
+def test_wall_forces_brute_force():
+    """Solvent-wall DPD == all pairs within the cutoff (wall velocity 0),
+    with the noise mean0var1ii(seed, solvent index, wall index)."""
+    L = (16, 12, 24)
+    s, srt, wpp, ws = wall_setup(L)
+    par = (25.0, 4.5, 3.0, 0.25)
+    seed = np.float32(0.37)
+    ff, npair = W.wall_forces(L, srt, s, wpp, ws, par, seed)
+    far = s.far(srt[:, :3])
+    want = np.zeros((len(srt), 3))
+    count = 0
+    rw = wpp[:, :3].astype(np.float64)
+    for a in range(len(srt)):
+        if far[a]:
+            continue
+        d = srt[a, :3].astype(np.float64) - rw
+        r2 = (d * d).sum(1)
+        for b in np.nonzero(r2 < 1)[0]:
+            r = np.sqrt(r2[b])
+            e = d[b] / r
+            ev = e.dot(srt[a, 3:].astype(np.float64))
+            xi = O.rng_ii(seed, [a], [b])[0]
+            w = (1 - r) ** 0.25
+            want[a] += (-par[1] * w * w * ev + par[2] * xi * w + par[0] * (1 - r)) * e
+            count += 1
+    assert npair == count and count > 0
+    scale = np.sqrt((want ** 2).sum(1).mean())
+    # float64 brute force vs the FP32 formula: w_R = (1-r)^(1/4) has an
+    # unbounded slope at the cutoff, so a pair at r ~ 1 - 1e-6 turns the FP32
+    # rounding of r into ~1e-4 of the noise term
+    assert np.max(np.linalg.norm(ff - want, axis=1)) <= 1e-4 * scale
+    # no particle far from the walls has a wall particle within the cutoff
+    for a in np.nonzero(far)[0][:200]:
+        assert np.min(((srt[a, :3] - wpp[:, :3]) ** 2).sum(1)) >= 1
+
+
+def test_bounce_returns_particles_to_the_fluid():
+    L = (16, 12, 24)
+    s, _ = channel(L)
+    rng = np.random.default_rng(4)
+    n = 500
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, 0] = rng.uniform(-8, 8, n)
+    pp[:, 1] = rng.uniform(-6, 6, n)
+    side = rng.choice([-1, 1], n)
+    pp[:, 2] = side * rng.uniform(8.0, 8.05, n)       # just inside a wall
+    pp[:, 5] = side * rng.uniform(0.5, 2.0, n)        # moving into it
+    dt = 0.05
+    out = s.bounce(pp, dt)
+    assert np.all(s.eval(out[:, :3]) < 0)
+    assert np.all(out[:, 5] * side < 0)               # normal velocity reversed
+    assert np.array_equal(out[:, 3:], -pp[:, 3:])     # resting wall: v -> -v
