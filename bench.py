@@ -1,10 +1,17 @@
 """Benchmark of the DPD per-timestep hot path on one MI355X.
 
-One step = cell list (counting sort + reorder), DPD forces (conservative +
-dissipative + random with the step-seeded hash RNG), constant body force,
-kick-drift and periodic wrap, for the 96^3, density-10 box with the channel
-flow parameters (a = 2.5, gamma = 8, kBT = 0.0444, dt = 5e-4, body force 0.01
-x).  Prints one JSON line.
+Workload: wall-bounded Poiseuille flow in a 96^3 box at number density 10
+(a = 2.5, gamma = 8, kBT = 0.0444, dt = 5e-4, spow = 0.25).  The walls come
+from a 128^3 sdf.dat-style field of a plane channel, s(z) = |z| - 40 (positive
+inside the walls), resampled onto the wall texture.  The box is filled with
+uniform random positions and Maxwell velocities, equilibrated as a periodic
+fluid, then the wall particles are frozen (0 <= s <= 2), the particles inside
+the walls dropped and the body force 0.01 x switched on.
+
+One step = cell list (counting sort + reorder + fragment tables), solvent DPD
+(conservative + dissipative + random with the step-seeded hash RNG), wall
+DPD, body force + kick-drift, bounce-back on the walls and periodic wrap.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -18,6 +25,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DPD pair-interactions/s & steps/s on 1 MI355X; batched vs per-box launches"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_dpd_gather_96cube_wall.md")
 
 
 def parse():
@@ -27,16 +35,42 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--L", type=int, default=96)
     ap.add_argument("--rho", type=float, default=10.0)
+    ap.add_argument("--half-width", type=float, default=40.0, help="channel half-width (0: periodic, no walls)")
+    ap.add_argument("--sdf-grid", type=int, default=128)
+    ap.add_argument("--equil", type=int, default=200, help="periodic equilibration steps before the freeze")
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=20, help="steps timed per kernel phase")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="MiB copied by one plain device copy before the workload (calibrates HBM counters)")
     return ap.parse_args()
+
+
+def plane_channel(L, N, h):
+    """s(z) = |z| - h at the cell centres of an N^3 grid over the box, x
+    fastest (the sdf.dat layout)."""
+    import numpy as np
+    z = (np.arange(N) + 0.5) * (L[2] / N) - 0.5 * L[2]
+    s = (np.abs(z) - h).astype(np.float32)
+    return np.ascontiguousarray(np.broadcast_to(s[:, None, None], (N, N, N)))
+
+
+def read_traffic():
+    """HBM bytes per k_dpd launch (FETCH_SIZE + WRITE_SIZE, copy-kernel
+    calibrated) from the committed rocprofv3 counter summary."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the path would shard by "
-              "3-D domain decomposition, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "3-D domain decomposition)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -45,56 +79,88 @@ def main():
     from ctc_amd import udx as U
 
     torch.cuda.set_device(0)
+    if args.copy_calib > 0:
+        # a plain copy of known size: reads and writes copy_calib MiB each
+        src = torch.ones(args.copy_calib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     L = (args.L, args.L, args.L)
     ncells = L[0] * L[1] * L[2]
-    n = int(round(args.rho * ncells))
+    n0 = int(round(args.rho * ncells))
     a, g, kBT, dt, fx = 2.5, 8.0, 0.0444, 5e-4, 0.01
     params = U.dpd_params(a, g, kBT, dt, 0.25)
+    walls = args.half_width > 0
 
     # synthetic state: uniform positions, Maxwell velocities with zero net momentum
     gen = torch.Generator(device="cuda").manual_seed(1234)
-    pp = torch.empty((n, 6), dtype=torch.float32, device="cuda")
-    pp[:, :3] = (torch.rand((n, 3), generator=gen, device="cuda") - 0.5) * float(args.L)
+    pp = torch.empty((n0, 6), dtype=torch.float32, device="cuda")
+    pp[:, :3] = (torch.rand((n0, 3), generator=gen, device="cuda") - 0.5) * float(args.L)
     pp[:, :3].clamp_(-0.5 * args.L, 0.5 * args.L - 1e-4)
-    v = torch.randn((n, 3), generator=gen, device="cuda", dtype=torch.float64) * np.sqrt(kBT)
+    v = torch.randn((n0, 3), generator=gen, device="cuda", dtype=torch.float64) * np.sqrt(kBT)
     v -= v.mean(0)
     pp[:, 3:] = v.float()
 
     ctx = U.Context()
-    flu = U.Flu(ctx, L, n, params, dt, 1.0, (fx, 0.0, 0.0))
-    flu.set_pp(pp)
+    t_setup = time.perf_counter()
+    # equilibration as a periodic fluid, no body force
+    eq = U.Flu(ctx, L, n0, params, dt, 1.0, (0.0, 0.0, 0.0))
+    eq.set_pp(pp)
+    eq.run(args.equil)
+    eq.get_pp(pp)
+    eq.close()
 
-    # warm-up
+    sdf = wall = None
+    n = n0
+    D = None
+    if walls:
+        N = args.sdf_grid
+        D = plane_channel(L, N, args.half_width)
+        sdf = U.Sdf(ctx, L, ext=[float(x) for x in L], N=(N, N, N), D=D)
+        wall = U.Wall(ctx, sdf, pp)             # freeze: pp[:n_solvent] is the solvent
+        n = wall.n_solvent
+    flu = U.Flu(ctx, L, n, params, dt, 1.0, (fx, 0.0, 0.0))
+    flu.set_pp(pp[:n].contiguous())
+    if wall is not None:
+        flu.set_wall(wall)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+
     flu.run(args.warmup)
     torch.cuda.synchronize()
 
-    # per-phase kernel times (HIP events on the bound stream, one step at a time)
+    # per-phase kernel times: HIP events on the library's stream, one step at a time
     flu.timing(True)
     flu.run(args.profile_steps)
-    t_phase = flu.timing(False)          # ms: clist, forces, integrate, sum
+    t_phase = flu.timing(False)          # ms: clist, solvent DPD, wall DPD, integrate, sum
     torch.cuda.synchronize()
 
     # timed region
     flu.pairs(reset=True)
+    w0 = flu.wall_pairs() if walls else 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     flu.run(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     pairs = flu.pairs()
+    wpairs = (flu.wall_pairs() - w0) if walls else 0
     sec = t1 - t0
     ms_step = 1e3 * sec / args.steps
     steps_s = args.steps / sec
-    pairs_step = pairs / args.steps
-    pairs_s = pairs / sec
+    pairs_step = (pairs + wpairs) / args.steps
+    pairs_s = (pairs + wpairs) / sec
 
-    # roofline of the dominant kernel (the force gather): algorithmic bytes
-    # per launch = N * 36 B (24 B particle in, 12 B force out)
+    # dominant kernel: the solvent force gather k_dpd, one launch per step.
+    # Algorithmic bytes per launch = n * 36 B (24 B particle in, 12 B force out)
     t_force = t_phase[1] * 1e-3
     force_bytes = n * 36.0
     force_gbs = force_bytes / t_force / 1e9
-    flops = pairs_step * 100.0 + n * 13.5 * args.rho * 9.0
+    # algorithmic FLOPs: 100 per pair (RNG included) + 9 per half-shell distance test
+    flops = (pairs / args.steps) * 100.0 + n * 13.5 * args.rho * 9.0
     step_bytes = n * 164.0 + ncells * 16.0
+    traffic = read_traffic()
 
     out = {
         "metric": METRIC,
@@ -106,64 +172,93 @@ def main():
         "ms_per_step": ms_step,
         "steps_per_s": steps_s,
         "pairs_per_step": pairs_step,
+        "wall_pairs_per_step": wpairs / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (uniform positions, Maxwell velocities, seed 1234)",
+        "data": "synthetic (uniform positions, Maxwell velocities, seed 1234; %d periodic equilibration steps "
+                "before the wall freeze)" % args.equil,
         "config": {
-            "workload": "dpd_%d^3_rho%g_channel_params_periodic" % (args.L, args.rho),
-            "box": list(L), "particles": n, "a": a, "gamma": g, "kBT": kBT, "dt": dt, "body_force_x": fx,
-            "walls": False,
+            "workload": ("wall_poiseuille_%d^3_rho%g_plane_channel_h%g" % (args.L, args.rho, args.half_width)
+                         if walls else "dpd_%d^3_rho%g_periodic" % (args.L, args.rho)),
+            "box": list(L), "particles_before_freeze": n0, "solvent_particles": n,
+            "wall_particles": wall.counts()[1] if walls else 0,
+            "a": a, "gamma": g, "kBT": kBT, "dt": dt, "spow": 0.25, "body_force_x": fx,
+            "walls": walls, "sdf_grid": args.sdf_grid if walls else None,
             "parallelism": "single GPU",
         },
-        "phase_ms": {"clist": t_phase[0], "forces": t_phase[1], "integrate": t_phase[2]},
+        "phase_ms": {"clist": t_phase[0], "dpd": t_phase[1], "wall_dpd": t_phase[2], "integrate": t_phase[3]},
+        "setup_s": t_setup,
         "roofline": {
-            "bound": "hbm", "kernel": "k_dpd (force gather)",
+            "bound": "hbm", "kernel": "k_dpd (solvent force gather)",
             "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": force_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
         },
         "step_roofline": {
             "achieved": step_bytes / (ms_step * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "bytes_per_step": step_bytes,
         },
         "valu": {
+            "kernel": "k_dpd",
             "achieved": flops / t_force / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": flops / t_force / 1e12 / FP32_PEAK_TFLOPS, "flop_per_step": flops,
+            "frac": flops / t_force / 1e12 / FP32_PEAK_TFLOPS, "flop_per_launch": flops,
         },
     }
 
     if args.cpu_steps > 0:
-        out["cpu_baseline"] = cpu_baseline(pp, L, (a, g, params.s, 0.25), dt, fx, args.cpu_steps)
-        cb = out["cpu_baseline"]
-        out["cpu_baseline"]["gpu_over_cpu"] = pairs_s / cb["value"] if cb["value"] else None
+        cur = torch.empty((n, 6), dtype=torch.float32, device="cuda")
+        flu.get_pp(cur)
+        wdata = None
+        if walls:
+            wdata = (D, host(wall.particles()), host(wall.starts()))
+        cb = cpu_baseline(host(cur), L, (a, g, params.s, 0.25), dt, fx, args.cpu_steps, wdata, args.sdf_grid)
+        cb["gpu_over_cpu"] = pairs_s / cb["value"] if cb["value"] else None
+        out["cpu_baseline"] = cb
 
     flu.close()
+    if wall is not None:
+        wall.close()
+        sdf.close()
     print(json.dumps(out))
 
 
-def cpu_baseline(pp_dev, L, params, dt, fx, nsteps):
-    """The oracle (CPU restatement, OpenMP over particles) on the same state."""
+def host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def cpu_baseline(pp, L, params, dt, fx, nsteps, wdata, sdf_grid):
+    """The oracle (CPU restatement: clist, OpenMP force gather, wall DPD,
+    integrate, bounce-back) timed on the same state and walls."""
     import numpy as np
 
     import oracle as O
+    from oracle import wall as W
 
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    pp = pp_dev.cpu().numpy()
     seeds = O.seeds_table(nsteps)
+    seeds = np.concatenate([seeds, O.kiss_fill(42, 42, 42, 42, nsteps)[:, None]], axis=1)
+    q = wpp = wst = None
+    if wdata is not None:
+        D, wpp, wst = wdata
+        q = W.Sdf(L, np.array(L, np.float32), (sdf_grid,) * 3, D.ravel())
     t0 = time.perf_counter()
     pairs = 0
     for k in range(nsteps):
-        pp, npair = O.flu_step(L, pp, params, seeds[k], dt, accel=(fx, 0.0, 0.0), nthreads=cores)
-        pairs += npair
+        pp, npair, nw = W.flu_wall_step(L, pp, params, seeds[k], dt, q, wpp, wst, accel=(fx, 0.0, 0.0),
+                                        nthreads=cores)
+        pairs += npair + nw
     sec = time.perf_counter() - t0
     return {"value": pairs / sec, "unit": "pair-interactions/s", "steps_per_s": nsteps / sec,
             "cores": cores, "kind": "port",
-            "sample": "%d full step(s) of the same %dx%dx%d box (clist + forces + integrate)" % (nsteps, *L)}
+            "sample": "%d full step(s) of the same %dx%dx%d %s box (clist + solvent DPD + wall DPD + integrate "
+                      "+ bounce)" % (nsteps, *L, "wall-bounded" if wdata is not None else "periodic")}
 
 
 if __name__ == "__main__":

@@ -121,8 +121,8 @@ struct udx_flu {
     unsigned long long *pairs_dev;   // [0] ordered solvent pairs, [1] solvent-wall pairs
     struct udx_wall *wall;
     int timing;
-    hipEvent_t ev[4];
-    float tms[4];
+    hipEvent_t ev[5];
+    float tms[5];
 };
 
 extern "C" {
@@ -357,7 +357,7 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
         set_error("udx_flu_ini: memset failed");
         s = UDX_EHIP;
     }
-    for (int e = 0; e < 4 && !s; e++)
+    for (int e = 0; e < 5 && !s; e++)
         if (hipEventCreate(&f->ev[e]) != hipSuccess) {
             set_error("udx_flu_ini: event create failed");
             s = UDX_EHIP;
@@ -382,7 +382,7 @@ int udx_flu_fin(udx_flu *f) {
     dev_free(f->seeds_dev);
     dev_free(f->pairs_dev);
     if (f->seeds_host) (void)hipHostFree(f->seeds_host);
-    for (int e = 0; e < 4; e++)
+    for (int e = 0; e < 5; e++)
         if (f->ev[e]) (void)hipEventDestroy(f->ev[e]);
     delete f;
     return UDX_OK;
@@ -460,16 +460,17 @@ static int flu_step(udx_flu *f, const float *seeds_dev) {
     d.accumulate = 0;
     d.pairs = f->pairs_dev;
     UDX_TRY(dpd_forces_mode(s, d, f->conf.exact));
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
     const SdfField *sdf = nullptr;
     if (f->wall) {
         sdf = wall_sdf(f->wall);
         UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
                             f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1, f->conf.exact));
     }
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
                          f->pos[b], f->vel[b], sdf));
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
+    if (f->timing) UDX_HIP(hipEventRecord(f->ev[4], s));
     f->cur = b;
     return UDX_OK;
 }
@@ -478,23 +479,22 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
     UDX_CHECK(f && nsteps >= 0, UDX_EINVAL, "udx_flu_run: bad arguments");
     if (nsteps == 0) return UDX_OK;
     UDX_TRY(flu_seed_table(f, nsteps, seeds_host));
-    float acc[3] = {0, 0, 0};
+    float acc[4] = {0, 0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
         UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k));
         if (f->timing) {
-            UDX_HIP(hipEventSynchronize(f->ev[3]));
-            float t;
-            UDX_HIP(hipEventElapsedTime(&t, f->ev[0], f->ev[1]));
-            acc[0] += t;
-            UDX_HIP(hipEventElapsedTime(&t, f->ev[1], f->ev[2]));
-            acc[1] += t;
-            UDX_HIP(hipEventElapsedTime(&t, f->ev[2], f->ev[3]));
-            acc[2] += t;
+            UDX_HIP(hipEventSynchronize(f->ev[4]));
+            for (int c = 0; c < 4; c++) {
+                float t;
+                UDX_HIP(hipEventElapsedTime(&t, f->ev[c], f->ev[c + 1]));
+                acc[c] += t;
+            }
         }
     }
-    if (f->timing)
-        for (int c = 0; c < 3; c++) f->tms[c] = acc[c] / nsteps;
-    f->tms[3] = f->tms[0] + f->tms[1] + f->tms[2];
+    if (f->timing) {
+        for (int c = 0; c < 4; c++) f->tms[c] = acc[c] / nsteps;
+        f->tms[4] = f->tms[0] + f->tms[1] + f->tms[2] + f->tms[3];
+    }
     return UDX_OK;
 }
 
@@ -691,11 +691,11 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
     return UDX_OK;
 }
 
-int udx_flu_timing(udx_flu *f, int enable, float out[4]) {
+int udx_flu_timing(udx_flu *f, int enable, float out[5]) {
     UDX_CHECK(f, UDX_EINVAL, "udx_flu_timing: null handle");
     f->timing = enable ? 1 : 0;
     if (out)
-        for (int c = 0; c < 4; c++) out[c] = f->tms[c];
+        for (int c = 0; c < 5; c++) out[c] = f->tms[c];
     return UDX_OK;
 }
 

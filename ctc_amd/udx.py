@@ -329,7 +329,7 @@ class Flu:
         return v.value
 
     def timing(self, enable=True):
-        out = (C.c_float * 4)()
+        out = (C.c_float * 5)()
         _ok(lib().udx_flu_timing(self.h, 1 if enable else 0, out))
         return list(out)
 

@@ -173,8 +173,9 @@ int udx_wall_repulse(const udx_sdf *q, float lambda, int n, const float *pp, flo
  * after the drift (NULL removes them) */
 int udx_flu_set_wall(udx_flu *f, udx_wall *w);
 /* per-step kernel timing of the last udx_flu_run (ms, HIP events on the
- * bound stream): [0] cell list, [1] forces, [2] integrate, [3] whole steps */
-int udx_flu_timing(udx_flu *f, int enable, float out[4]);
+ * bound stream): [0] cell list + reorder + fragment tables, [1] solvent DPD,
+ * [2] wall DPD, [3] integrate + bounce-back, [4] their sum */
+int udx_flu_timing(udx_flu *f, int enable, float out[5]);
 
 #ifdef __cplusplus
 }
