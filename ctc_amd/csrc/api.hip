@@ -78,7 +78,7 @@ struct udx_clist {
 struct udx_fluforces {
     udx_ctx *ctx;
     Box box;
-    int maxp, n, exact;
+    int maxp, n, mode;
     float4 *pos, *vel;
     float *seeds_dev;
     FragTables frag;
@@ -96,7 +96,7 @@ struct udx_wall {
     const udx_sdf *sdf;
     WallSet w;
     udx_rnd trunk;              // (42,42,42,42), src/wall/imp/ini.h:11
-    int exact, cap;
+    int mode, cap;
     float *seed_dev;
     float4 *pos, *vel;          // AoS -> float4 staging for udx_wall_force
     unsigned long long *pairs_dev;
@@ -246,9 +246,11 @@ int udx_fluforces_fin(udx_fluforces *f) {
     return UDX_OK;
 }
 
-int udx_fluforces_set_mode(udx_fluforces *f, int exact) {
+int udx_fluforces_set_mode(udx_fluforces *f, int mode) {
     UDX_CHECK(f, UDX_EINVAL, "udx_fluforces_set_mode: null handle");
-    f->exact = exact ? 1 : 0;
+    UDX_CHECK(mode >= UDX_MODE_FAST && mode <= UDX_MODE_EXACT, UDX_EINVAL, "udx_fluforces_set_mode: mode %d",
+              mode);
+    f->mode = mode;
     return UDX_OK;
 }
 
@@ -281,7 +283,7 @@ int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int
     a.ff_stride = 3;
     a.accumulate = 1;
     a.pairs = f->pairs_dev;
-    UDX_TRY(dpd_forces_mode(s, a, f->exact));
+    UDX_TRY(dpd_forces_mode(s, a, f->mode));
     // the seeds live on the host stack: wait for the copy before returning
     UDX_HIP(hipStreamSynchronize(s));
     return UDX_OK;
@@ -338,6 +340,8 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
     UDX_CHECK(L[0] >= 3 && L[1] >= 3 && L[2] >= 3, UDX_EINVAL,
               "udx_flu_ini: a periodic box needs at least 3 cells per side (got %d %d %d)", L[0], L[1], L[2]);
     UDX_CHECK(conf->maxp > 0 && conf->mass > 0, UDX_EINVAL, "udx_flu_ini: bad capacity or mass");
+    UDX_CHECK(conf->mode >= UDX_MODE_FAST && conf->mode <= UDX_MODE_EXACT, UDX_EINVAL, "udx_flu_ini: mode %d",
+              conf->mode);
     udx_flu *f = new udx_flu();
     f->ctx = ctx;
     f->conf = *conf;
@@ -459,13 +463,14 @@ static int flu_step(udx_flu *f, const float *seeds_dev) {
     d.ff_stride = 4;
     d.accumulate = 0;
     d.pairs = f->pairs_dev;
-    UDX_TRY(dpd_forces_mode(s, d, f->conf.exact));
+    UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
     const SdfField *sdf = nullptr;
     if (f->wall) {
         sdf = wall_sdf(f->wall);
         UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
-                            f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1, f->conf.exact));
+                            f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1,
+                            f->conf.mode == UDX_MODE_EXACT));
     }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
@@ -625,9 +630,10 @@ int udx_wall_counts(const udx_wall *w, int *nfrozen, int *ntotal) {
 const float *udx_wall_pp(const udx_wall *w) { return w ? w->w.pp_aos : nullptr; }
 const int *udx_wall_starts(const udx_wall *w) { return w ? w->w.cl.starts : nullptr; }
 
-int udx_wall_set_mode(udx_wall *w, int exact) {
+int udx_wall_set_mode(udx_wall *w, int mode) {
     UDX_CHECK(w, UDX_EINVAL, "udx_wall_set_mode: null handle");
-    w->exact = exact ? 1 : 0;
+    UDX_CHECK(mode >= UDX_MODE_FAST && mode <= UDX_MODE_EXACT, UDX_EINVAL, "udx_wall_set_mode: mode %d", mode);
+    w->mode = mode;
     return UDX_OK;
 }
 
@@ -653,7 +659,7 @@ int udx_wall_force_seed(udx_wall *w, const udx_pair_dpd *p, int n, const float *
     if (hipMemcpyAsync(n_dev, &n, sizeof n, hipMemcpyHostToDevice, s) != hipSuccess) st = UDX_EHIP;
     if (!st)
         st = wall_forces(s, &w->sdf->f, &w->w, n, n_dev, w->pos, w->vel, *p, w->seed_dev, ff, 3, w->pairs_dev,
-                         w->exact);
+                         w->mode == UDX_MODE_EXACT);
     if (!st && hipStreamSynchronize(s) != hipSuccess) st = UDX_EHIP;
     dev_free(n_dev);
     if (st) set_error("udx_wall_force: launch failed");

@@ -116,6 +116,14 @@ __device__ __forceinline__ PairIn pair_in(const DpdK &A, unsigned ent, int i, co
     return o;
 }
 
+// The hardware hands workgroups to the 8 XCDs round robin; renumbering them
+// gives each XCD one contiguous run of cells, so neighbour rows are reused
+// from that XCD's L2 instead of being fetched again by the others.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int per = nb >> 3, rem = nb & 7, x = b & 7, k = b >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 template <int WR, bool EXACT>
 __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     __shared__ unsigned q[QCAP * DPD_THREADS];
@@ -124,7 +132,7 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     const int tid = threadIdx.x;
     const int Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
     const int n = A.starts[Lx * Ly * Lz];
-    const int i = blockIdx.x * DPD_THREADS + tid;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS + tid;
     const bool active = i < n && i < A.nmax;
 
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
@@ -145,11 +153,14 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     // Drain the per-lane queues in queue order (= candidate order), two
     // pairs per lane at a time so two noise chains share each
     // round-toward-zero window.
+    // an idle slot evaluates a discarded pair with an in-bounds particle
+    // (i itself, or particle 0 for a lane past the end)
+    const unsigned idle = (unsigned)(active ? i : 0) | (W_NONE << 27);
     auto drain = [&]() {
         for (int e = 0; wave_any(e < fill); e += 2) {
             const bool va = e < fill, vb = e + 1 < fill;
-            const unsigned ea = va ? qlane[e * DPD_THREADS] : (unsigned)i | (W_NONE << 27);
-            const unsigned eb = vb ? qlane[(e + 1) * DPD_THREADS] : (unsigned)i | (W_NONE << 27);
+            const unsigned ea = va ? qlane[e * DPD_THREADS] : idle;
+            const unsigned eb = vb ? qlane[(e + 1) * DPD_THREADS] : idle;
             PairIn a = pair_in(A, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
             PairIn b = pair_in(A, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
             float xa = a.x, xb = b.x;
@@ -315,6 +326,7 @@ int launch_dpd(hipStream_t s, const DpdK &k, int nmax) {
     return UDX_OK;
 }
 
+
 }  // namespace
 
 int frag_alloc(FragTables *t, const Box &b) {
@@ -348,8 +360,9 @@ int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *star
     return scan_exclusive(s, t->cnt, t->size, t->fss, t->scan_tmp);
 }
 
-// exact = 1 selects correctly rounded sqrt/division in the pair force
-int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int exact) {
+// mode UDX_MODE_EXACT selects correctly rounded sqrt/division in the pair force
+int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
+    const bool exact = mode == UDX_MODE_EXACT;
     UDX_CHECK(!a.periodic || (a.box.L[0] >= 3 && a.box.L[1] >= 3 && a.box.L[2] >= 3), UDX_EINVAL,
               "dpd: a periodic box needs at least 3 cells per side (got %d %d %d)", a.box.L[0], a.box.L[1],
               a.box.L[2]);

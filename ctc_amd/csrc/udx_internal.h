@@ -119,7 +119,8 @@ struct DpdLaunch {
     unsigned long long *pairs;   // ordered-pair counter or null
 };
 int dpd_forces(hipStream_t s, const DpdLaunch &a);
-int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int exact);
+// mode: UDX_MODE_FAST or UDX_MODE_EXACT
+int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode);
 
 // ------------------------------------------------------------------ scheme
 int aos_to_f4(hipStream_t s, int n, const float *pp, float4 *pos, float4 *vel);

@@ -27,7 +27,7 @@ class PairDpd(C.Structure):
 
 class FluConf(C.Structure):
     _fields_ = [("L", C.c_int * 3), ("maxp", C.c_int), ("dpd", PairDpd), ("dt", C.c_float),
-                ("mass", C.c_float), ("accel", C.c_float * 3), ("exact", C.c_int)]
+                ("mass", C.c_float), ("accel", C.c_float * 3), ("mode", C.c_int)]
 
 
 _VP, _I, _F = C.c_void_p, C.c_int, C.c_float
@@ -131,6 +131,17 @@ class RndTrunk:
         return lib().udx_rnd_get(C.byref(self.s))
 
 
+MODE_FAST, MODE_EXACT = 0, 1
+
+
+def _mode(exact, mode):
+    """Force kernel mode: an explicit MODE_* wins, else exact selects
+    MODE_EXACT over MODE_FAST."""
+    if mode is not None:
+        return int(mode)
+    return MODE_EXACT if exact else MODE_FAST
+
+
 def dpd_params(a, g, kBT, dt, spow=0.25):
     """pair_set_dpd + pair_compute_dpd_sigma (src/pair/imp/main.h:54-62)."""
     p = PairDpd(a, g, 0.0, spow)
@@ -223,11 +234,11 @@ class FluForces:
     """fluforces_bulk_* + fluforces_halo_* (src/fluforces/imp.h) on one
     periodic domain."""
 
-    def __init__(self, ctx, L, maxp, exact=False):
+    def __init__(self, ctx, L, maxp, exact=False, mode=None):
         self.ctx, self.L = ctx, tuple(L)
         self.h = C.c_void_p()
         _ok(lib().udx_fluforces_ini(ctx.h, L[0], L[1], L[2], maxp, C.byref(self.h)))
-        _ok(lib().udx_fluforces_set_mode(self.h, 1 if exact else 0))
+        _ok(lib().udx_fluforces_set_mode(self.h, _mode(exact, mode)))
 
     def prepare(self, pp):
         _ok(lib().udx_fluforces_prepare(self.h, pp.shape[0], _ptr(pp)))
@@ -276,14 +287,14 @@ class Flu:
     """Solvent-only step pipeline (src/sim/imp/run.h:8-68): clist + reorder,
     DPD forces, body force, kick-drift, periodic wrap."""
 
-    def __init__(self, ctx, L, maxp, params, dt, mass=1.0, accel=(0.0, 0.0, 0.0), exact=False):
+    def __init__(self, ctx, L, maxp, params, dt, mass=1.0, accel=(0.0, 0.0, 0.0), exact=False, mode=None):
         self.ctx = ctx
         conf = FluConf()
         for k in range(3):
             conf.L[k] = L[k]
             conf.accel[k] = accel[k]
         conf.maxp, conf.dpd, conf.dt, conf.mass = maxp, params, dt, mass
-        conf.exact = 1 if exact else 0
+        conf.mode = _mode(exact, mode)
         self.h = C.c_void_p()
         _ok(lib().udx_flu_ini(ctx.h, C.byref(conf), C.byref(self.h)))
 
@@ -385,7 +396,7 @@ class Sdf:
 class Wall:
     """wall_gen + wall_force (src/wall/imp.h): frozen wall particles."""
 
-    def __init__(self, ctx, sdf, pp, exact=False):
+    def __init__(self, ctx, sdf, pp, exact=False, mode=None):
         """pp: solvent (n, 6) cuda tensor; on return the first self.n_solvent
         rows hold the solvent that stays (order kept)."""
         self.ctx, self.sdf = ctx, sdf
@@ -393,7 +404,7 @@ class Wall:
         n = C.c_int(pp.shape[0])
         _ok(lib().udx_wall_gen(ctx.h, sdf.h, C.byref(n), _ptr(pp), C.byref(self.h)))
         self.n_solvent = n.value
-        _ok(lib().udx_wall_set_mode(self.h, 1 if exact else 0))
+        _ok(lib().udx_wall_set_mode(self.h, _mode(exact, mode)))
 
     def counts(self):
         a, b = C.c_int(), C.c_int()

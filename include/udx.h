@@ -85,10 +85,12 @@ int udx_fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *star
  * seeds[26] for the bulk */
 int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int *starts,
                               int periodic, const float seeds[27], float *ff);
-/* exact = 1: correctly rounded sqrt and division in the pair force, so the
- * forces equal the CPU restatement bit for bit (slower); 0 (default):
- * hardware reciprocal square root and square root */
-int udx_fluforces_set_mode(udx_fluforces *f, int exact);
+/* force kernel modes; both gather the full shell of each particle in a fixed
+ * order (no atomics), so results are identical from run to run */
+#define UDX_MODE_FAST 0    /* default: hardware reciprocal and roots (1 ulp) */
+#define UDX_MODE_EXACT 1   /* correctly rounded division and roots: equal to
+                              the CPU restatement bit for bit (slower) */
+int udx_fluforces_set_mode(udx_fluforces *f, int mode);
 /* unordered pair interactions (r^2 < 1) counted by apply calls since the last reset */
 int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset);
 
@@ -115,7 +117,7 @@ typedef struct udx_flu_conf {
     udx_pair_dpd dpd;
     float dt, mass;
     float accel[3];     /* constant body force per unit mass */
-    int exact;          /* 1: correctly rounded pair force (see udx_fluforces_set_mode) */
+    int mode;           /* UDX_MODE_* of the solvent force kernel */
 } udx_flu_conf;
 typedef struct udx_flu udx_flu;
 int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **f);
@@ -159,7 +161,7 @@ int udx_wall_fin(udx_wall *w);
 int udx_wall_counts(const udx_wall *w, int *nfrozen, int *ntotal);
 const float *udx_wall_pp(const udx_wall *w);       /* device AoS, cell order on the margin box */
 const int *udx_wall_starts(const udx_wall *w);     /* device, margin-box cells + 1 */
-int udx_wall_set_mode(udx_wall *w, int exact);     /* as udx_fluforces_set_mode */
+int udx_wall_set_mode(udx_wall *w, int mode);      /* UDX_MODE_EXACT or not */
 /* solvent-wall DPD on n cell-ordered solvent particles (device AoS), forces
  * added to ff; the seed is one draw of the wall trunk, or given */
 int udx_wall_force(udx_wall *w, const udx_pair_dpd *p, int n, const float *pp, float *ff);

@@ -23,8 +23,8 @@ from ctc_amd import udx as U  # noqa: E402
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
 
 # Production-kernel force tolerance.  The kernel uses the hardware reciprocal
-# square root and square root (about 1 ulp each) where the oracle rounds
-# correctly; summation order is identical.  Per particle:
+# and square roots (about 1 ulp each) where the oracle rounds correctly;
+# summation order is identical.  Per particle:
 #   |f_gpu - f_oracle| <= FTOL * max(|f_oracle|, 0.1 * F_rms)
 FTOL = 1e-5
 
@@ -110,16 +110,16 @@ def test_clist_golden_on_gpu(ctx, name, gen):
 
 
 # ------------------------------------------------------------------ forces
-def gpu_forces(ctx, L, pp_sorted, starts, params, seeds, periodic=True, exact=False):
+def gpu_forces(ctx, L, pp_sorted, starts, params, seeds, periodic=True, exact=False, mode=None):
     n = len(pp_sorted)
-    ff = FluForcesRun(ctx, L, n, exact)
+    ff = FluForcesRun(ctx, L, n, exact, mode)
     return ff(pp_sorted, starts, params, seeds, periodic)
 
 
 class FluForcesRun:
-    def __init__(self, ctx, L, n, exact):
+    def __init__(self, ctx, L, n, exact, mode=None):
         self.ctx, self.L = ctx, L
-        self.f = U.FluForces(ctx, L, max(n, 1), exact=exact)
+        self.f = U.FluForces(ctx, L, max(n, 1), exact=exact, mode=mode)
 
     def __call__(self, pp_sorted, starts, params, seeds, periodic):
         ppd = dev(pp_sorted)
@@ -158,6 +158,8 @@ def test_forces_exact_mode_bit_exact(ctx, case):
 
 @pytest.mark.parametrize("case", CASES)
 def test_forces_fast_mode_tolerance(ctx, case):
+    """Production kernel: the oracle's summation order with hardware
+    reciprocal and roots; identical from run to run."""
     L, rho, a, g, kBT, dt, spow = case
     say("forces fast", case)
     pp = random_box(L, rho, 8)
@@ -165,7 +167,9 @@ def test_forces_fast_mode_tolerance(ctx, case):
     seeds = O.seeds_table(3)[2]
     par = (a, g, O.dpd_sigma(kBT, g, dt), spow)
     f_ref, np_ref = O.dpd_forces(L, srt, starts, counts, par, seeds)
-    f_gpu, np_gpu = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, exact=False)
+    f_gpu, np_gpu = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, mode=U.MODE_FAST)
+    f_gpu2, _ = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, mode=U.MODE_FAST)
+    assert np.array_equal(f_gpu.view(np.uint32), f_gpu2.view(np.uint32))
     assert np_gpu == np_ref // 2           # cutoff decisions are exact arithmetic
     mag = np.linalg.norm(f_ref.astype(np.float64), axis=1)
     frms = np.sqrt((mag ** 2).mean())
