@@ -30,6 +30,12 @@ class FluConf(C.Structure):
                 ("mass", C.c_float), ("accel", C.c_float * 3), ("mode", C.c_int)]
 
 
+class RbcParams(C.Structure):
+    """src/rbc/params/imp.h:3-9"""
+    _fields_ = [(k, C.c_float) for k in ("gammaC", "gammaT", "kBT", "kb", "phi", "ks", "x0", "mpow", "ka", "kd",
+                                          "kv", "totArea", "totVolume")]
+
+
 _VP, _I, _F = C.c_void_p, C.c_int, C.c_float
 _SIGS = {
     "udx_last_error": (C.c_char_p, []),
@@ -85,6 +91,17 @@ _SIGS = {
     "udx_wall_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
     "udx_wall_bounce": (_I, [_VP, _F, _I, _VP]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
+    "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
+    "udx_rbc_fin": (_I, [_VP]),
+    "udx_rbc_md": (_I, [_VP, C.POINTER(_I)]),
+    "udx_rbc_adjacency": (_I, [_VP, C.POINTER(_I), C.POINTER(_I)]),
+    "udx_rbc_set_stressful": (_I, [_VP, _F]),
+    "udx_rbc_set_stressfree": (_I, [_VP, C.POINTER(_F)]),
+    "udx_rbc_set_bending": (_I, [_VP, _I]),
+    "udx_rbc_area_volume": (_I, [_VP, _I, _VP, _VP]),
+    "udx_rbc_av": (_VP, [_VP]),
+    "udx_rbc_force_apply": (_I, [_VP, C.POINTER(RbcParams), _F, _I, _VP, _VP]),
+    "udx_rbc_stretch_apply": (_I, [_VP, _I, _I, _VP, _VP]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -441,3 +458,84 @@ class Wall:
         if self.h:
             lib().udx_wall_fin(self.h)
             self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- RBC
+def read_off(path):
+    """OFF mesh (src/io/mesh_read/imp/off.h): vertices (nv, 3) float32 and
+    triangles (nt, 3) int32."""
+    import numpy as np
+    with open(path) as f:
+        toks = f.read().split()
+    if not toks or toks[0] != "OFF":
+        raise UdxError("%s: not an OFF file" % path)
+    nv, nt = int(toks[1]), int(toks[2])
+    k = 4
+    rr = np.array(toks[k:k + 3 * nv], np.float32).reshape(nv, 3)
+    k += 3 * nv
+    tt = np.zeros((nt, 3), np.int32)
+    for t in range(nt):
+        if toks[k] != "3":
+            raise UdxError("%s: face %d is not a triangle" % (path, t))
+        tt[t] = [int(toks[k + 1]), int(toks[k + 2]), int(toks[k + 3])]
+        k += 4
+    return rr, tt
+
+
+def rbc_params(**kw):
+    p = RbcParams()
+    for k, v in kw.items():
+        setattr(p, k, float(v))
+    return p
+
+
+class Rbc:
+    """Membrane forces of nc copies of one mesh (src/rbc/force/imp.h:9-28)."""
+
+    def __init__(self, ctx, tt, nv, max_cells):
+        import numpy as np
+        self.ctx = ctx
+        t = np.ascontiguousarray(tt, np.int32).reshape(-1, 3)
+        self.nv, self.nt = int(nv), len(t)
+        self.h = C.c_void_p()
+        _ok(lib().udx_rbc_ini(ctx.h, self.nv, self.nt, t.ctypes.data_as(C.POINTER(_I)), max_cells,
+                              C.byref(self.h)))
+        md = C.c_int()
+        _ok(lib().udx_rbc_md(self.h, C.byref(md)))
+        self.md = md.value
+
+    def adjacency(self):
+        import numpy as np
+        a0 = np.zeros(self.md * self.nv, np.int32)
+        a1 = np.zeros(self.md * self.nv, np.int32)
+        _ok(lib().udx_rbc_adjacency(self.h, a0.ctypes.data_as(C.POINTER(_I)), a1.ctypes.data_as(C.POINTER(_I))))
+        return a0, a1
+
+    def set_stressful(self, totArea):
+        _ok(lib().udx_rbc_set_stressful(self.h, float(totArea)))
+
+    def set_stressfree(self, rr):
+        import numpy as np
+        r = np.ascontiguousarray(rr, np.float32)
+        _ok(lib().udx_rbc_set_stressfree(self.h, r.ctypes.data_as(C.POINTER(_F))))
+
+    def set_bending(self, kantor=True):
+        _ok(lib().udx_rbc_set_bending(self.h, 1 if kantor else 0))
+
+    def area_volume(self, nc, pp):
+        import torch
+        av = torch.empty(2 * nc, dtype=torch.float64, device="cuda")
+        _ok(lib().udx_rbc_area_volume(self.h, nc, _ptr(pp), _ptr(av)))
+        return av
+
+    def force(self, params, dt, nc, pp, ff):
+        _ok(lib().udx_rbc_force_apply(self.h, C.byref(params), float(dt), nc, _ptr(pp), _ptr(ff)))
+
+    def close(self):
+        if self.h:
+            lib().udx_rbc_fin(self.h)
+            self.h = C.c_void_p()
+
+
+def rbc_stretch_apply(ctx, nc, nv, f, ff):
+    _ok(lib().udx_rbc_stretch_apply(ctx.h, nc, nv, _ptr(f), _ptr(ff)))

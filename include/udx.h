@@ -179,6 +179,42 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w);
  * [2] wall DPD, [3] integrate + bounce-back, [4] their sum */
 int udx_flu_timing(udx_flu *f, int enable, float out[5]);
 
+/* ---- RBC membrane: replaces rbc_force_ini / rbc_force_set_conf /
+ *      rbc_force_apply / rbc_force_fin (src/rbc/force/imp.h:9-28),
+ *      area_volume_ini / area_volume_compute (src/rbc/force/area_volume/imp.h:6-10),
+ *      bending_apply with the Kantor model (src/rbc/force/bending/imp.h:16)
+ *      and rbc_stretch_apply (src/rbc/stretch/imp.h) ----
+ * Cells are nc consecutive copies of one template mesh (nv vertices, nt
+ * triangles); pp holds nc * nv Particles, cell-major. */
+typedef struct udx_rbc_params {        /* src/rbc/params/imp.h:3-9 */
+    float gammaC, gammaT, kBT;
+    float kb, phi;                     /* bending rigidity, spontaneous angle (degrees) */
+    float ks, x0, mpow;                /* WLC + POW springs */
+    float ka, kd, kv;                  /* global area, local area, volume */
+    float totArea, totVolume;
+} udx_rbc_params;
+typedef struct udx_rbc udx_rbc;
+/* tt: host, 3 * nt vertex indices; builds the ring adjacency */
+int udx_rbc_ini(udx_ctx *ctx, int nv, int nt, const int *tt, int max_cells, udx_rbc **r);
+int udx_rbc_fin(udx_rbc *r);
+int udx_rbc_md(const udx_rbc *r, int *md);                 /* maximum vertex degree */
+/* host copies of the ring adjacency (md * nv each, src/rbc/adj/imp.h) */
+int udx_rbc_adjacency(const udx_rbc *r, int *adj0, int *adj1);
+/* stress-full references l0, a0 from the total area (src/rbc/force/imp/main.h:50-59) */
+int udx_rbc_set_stressful(udx_rbc *r, float totArea);
+/* stress-free references from a shape (host, 3 * nv floats; src/rbc/force/imp/main.h:61-92) */
+int udx_rbc_set_stressfree(udx_rbc *r, const float *rr);
+int udx_rbc_set_bending(udx_rbc *r, int kantor);           /* 1: "kantor", 0: "none" */
+/* per cell [area, volume] in FP64 into av (device, 2 * nc; NULL: the
+ * handle's own buffer, see udx_rbc_av) */
+int udx_rbc_area_volume(udx_rbc *r, int nc, const float *pp, double *av);
+const double *udx_rbc_av(const udx_rbc *r);                /* device, 2 * max_cells */
+/* area/volume, bending and membrane forces ADDED to ff (nc * nv Forces);
+ * the membrane's thermal noise (rnd = true) is not supported */
+int udx_rbc_force_apply(udx_rbc *r, const udx_rbc_params *p, float dt, int nc, const float *pp, float *ff);
+/* ff[i] += f[i % nv] for nc * nv vertices (f: device, 3 * nv) */
+int udx_rbc_stretch_apply(udx_ctx *ctx, int nc, int nv, const float *f, float *ff);
+
 #ifdef __cplusplus
 }
 #endif

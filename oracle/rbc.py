@@ -1,0 +1,173 @@
+"""ctypes bindings of the RBC membrane part of the C oracle
+(oracle/udx_oracle_rbc.c).  Test infrastructure only."""
+import ctypes as C
+
+import numpy as np
+
+from .oracle import F32P, F64P, I32P, _f32, _i32, _ptr, lib
+
+
+class RbcPar(C.Structure):
+    """src/rbc/params/imp.h:3-9"""
+    _fields_ = [(k, C.c_float) for k in ("gammaC", "gammaT", "kBT", "kb", "phi", "ks", "x0", "mpow", "ka", "kd",
+                                          "kv", "totArea", "totVolume")]
+
+
+# conf/rbc.test.cfg
+RBC_TEST = dict(gammaC=30.0, gammaT=90.0, kBT=0.0444, kb=17.78, phi=6.97, ks=14.8, x0=0.5, mpow=2.0, ka=2450.0,
+                kd=100.0, kv=3750.0)
+
+_declared = False
+
+
+def _lib():
+    global _declared
+    L = lib()
+    if not _declared:
+        P = C.POINTER(RbcPar)
+        sig = {
+            "orc_mesh_md": (C.c_int, [C.c_int, C.c_int, I32P]),
+            "orc_adj_ini": (C.c_int, [C.c_int, C.c_int, C.c_int, I32P, I32P, I32P]),
+            "orc_adj_map": (C.c_int, [C.c_int, C.c_int, I32P, I32P, C.c_int, I32P]),
+            "orc_rbc_area_volume": (None, [C.c_int, C.c_int, C.c_int, I32P, F32P, F64P]),
+            "orc_rbc_shape": (None, [C.c_int, C.c_int, I32P, I32P, F32P, F32P, F32P]),
+            "orc_rbc_stressful": (None, [C.c_int, C.c_float, F32P, F32P]),
+            "orc_rbc_forces": (None, [P, C.c_int, C.c_int, C.c_int, I32P, I32P, F32P, F64P, F32P, F32P, C.c_float,
+                                      C.c_float, F64P]),
+            "orc_rbc_bending": (None, [P, C.c_int, C.c_int, C.c_int, I32P, I32P, F32P, F64P]),
+            "orc_rbc_energy": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, I32P, F64P, F32P,
+                                            F32P, C.c_float, C.c_float, C.c_int]),
+            "orc_rbc_stretch": (None, [C.c_int, C.c_int, F32P, F32P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _declared = True
+    return L
+
+
+def read_off(path):
+    """OFF mesh: (vertices (nv, 3) float32, triangles (nt, 3) int32)
+    (src/io/mesh_read/imp/off.h)."""
+    with open(path) as f:
+        toks = f.read().split()
+    assert toks[0] == "OFF"
+    nv, nt = int(toks[1]), int(toks[2])
+    k = 4
+    rr = np.array(toks[k:k + 3 * nv], np.float32).reshape(nv, 3)
+    k += 3 * nv
+    tt = np.zeros((nt, 3), np.int32)
+    for t in range(nt):
+        assert toks[k] == "3"
+        tt[t] = [int(toks[k + 1]), int(toks[k + 2]), int(toks[k + 3])]
+        k += 4
+    return rr, tt
+
+
+class Mesh:
+    """Template mesh with its ring adjacency."""
+
+    def __init__(self, rr, tt):
+        self.rr = _f32(rr).reshape(-1, 3)
+        self.tt = _i32(tt).reshape(-1, 3)
+        self.nv, self.nt = len(self.rr), len(self.tt)
+        self.md = _lib().orc_mesh_md(self.nv, self.nt, _ptr(self.tt, I32P))
+        self.adj0 = np.zeros(self.md * self.nv, np.int32)
+        self.adj1 = np.zeros(self.md * self.nv, np.int32)
+        _lib().orc_adj_ini(self.md, self.nt, self.nv, _ptr(self.tt, I32P), _ptr(self.adj0, I32P),
+                           _ptr(self.adj1, I32P))
+
+    def map(self, i):
+        m = np.zeros(6, np.int32)
+        ok = _lib().orc_adj_map(self.md, self.nv, _ptr(self.adj0, I32P), _ptr(self.adj1, I32P), i, _ptr(m, I32P))
+        return m if ok else None
+
+    def shape(self):
+        """stress-free (ll, aa) per slot of the template itself"""
+        n = self.md * self.nv
+        ll, aa = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        _lib().orc_rbc_shape(self.md, self.nv, _ptr(self.adj0, I32P), _ptr(self.adj1, I32P), _ptr(self.rr, F32P),
+                             _ptr(ll, F32P), _ptr(aa, F32P))
+        return ll, aa
+
+    def stressful(self, totArea):
+        l0, a0 = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        _lib().orc_rbc_stressful(self.nt, np.float32(totArea), _ptr(l0, F32P), _ptr(a0, F32P))
+        return float(l0[0]), float(a0[0])
+
+
+def par(**kw):
+    p = RbcPar()
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def area_volume(mesh, pp, nc):
+    pp = _f32(pp).reshape(-1, 6)
+    av = np.zeros(2 * nc, np.float64)
+    _lib().orc_rbc_area_volume(mesh.nt, mesh.nv, nc, _ptr(mesh.tt, I32P), _ptr(pp, F32P), _ptr(av, F64P))
+    return av
+
+
+def forces(p, mesh, pp, nc, av, stress_free=None, l0=0.0, a0=0.0, bending=True):
+    """membrane (+ Kantor bending) forces, double (nc*nv, 3)"""
+    pp = _f32(pp).reshape(-1, 6)
+    ff = np.zeros((nc * mesh.nv, 3), np.float64)
+    ll = aa = None
+    if stress_free is not None:
+        ll, aa = (_f32(x) for x in stress_free)
+    _lib().orc_rbc_forces(C.byref(p), mesh.md, mesh.nv, nc, _ptr(mesh.adj0, I32P), _ptr(mesh.adj1, I32P),
+                          _ptr(pp, F32P), _ptr(_f64(av), F64P), _ptr(ll, F32P) if ll is not None else None,
+                          _ptr(aa, F32P) if aa is not None else None, np.float32(l0), np.float32(a0),
+                          _ptr(ff, F64P))
+    if bending:
+        _lib().orc_rbc_bending(C.byref(p), mesh.md, mesh.nv, nc, _ptr(mesh.adj0, I32P), _ptr(mesh.adj1, I32P),
+                               _ptr(pp, F32P), _ptr(ff, F64P))
+    return ff
+
+
+def bending(p, mesh, pp, nc):
+    pp = _f32(pp).reshape(-1, 6)
+    ff = np.zeros((nc * mesh.nv, 3), np.float64)
+    _lib().orc_rbc_bending(C.byref(p), mesh.md, mesh.nv, nc, _ptr(mesh.adj0, I32P), _ptr(mesh.adj1, I32P),
+                           _ptr(pp, F32P), _ptr(ff, F64P))
+    return ff
+
+
+def energy(p, mesh, r, nc, terms, stress_free=None, l0=0.0, a0=0.0):
+    r = np.ascontiguousarray(r, np.float64).reshape(-1, 3)
+    ll = aa = None
+    if stress_free is not None:
+        ll, aa = (_f32(x) for x in stress_free)
+    return _lib().orc_rbc_energy(C.byref(p), mesh.md, mesh.nv, nc, mesh.nt, _ptr(mesh.tt, I32P),
+                                 _ptr(mesh.adj0, I32P), _ptr(mesh.adj1, I32P), _ptr(r, F64P),
+                                 _ptr(ll, F32P) if ll is not None else None,
+                                 _ptr(aa, F32P) if aa is not None else None, np.float32(l0), np.float32(a0), terms)
+
+
+def stretch(nc, nv, f, ff):
+    ff = _f32(ff).reshape(-1, 3).copy()
+    _lib().orc_rbc_stretch(nc, nv, _ptr(_f32(f), F32P), _ptr(ff, F32P))
+    return ff
+
+
+def place(rr, matrices, L):
+    """cells from the template by 4x4 affine matrices (row major), in double,
+    shifted to local coordinates (src/mesh/gen/imp/main.h:1-56)"""
+    out = []
+    for A in matrices:
+        A = np.asarray(A, np.float64).reshape(4, 4)
+        h = np.concatenate([rr.astype(np.float64), np.ones((len(rr), 1))], axis=1)
+        g = (h @ A.T)[:, :3].astype(np.float32)
+        g = g - (np.array(L, np.float32) / np.float32(2))
+        out.append(g)
+    r = np.concatenate(out).astype(np.float32)
+    pp = np.zeros((len(r), 6), np.float32)
+    pp[:, :3] = r
+    return pp
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)

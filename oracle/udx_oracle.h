@@ -135,6 +135,37 @@ void  orc_integrate_wall(const int L[3], int n, float dt, float mass, const floa
                          float *pp, const orc_sdf *q);
 void  orc_wall_repulse(const orc_sdf *q, float lambda, int n, const float *pp, float *ff);
 
+/* ---- RBC membrane (udx_oracle_rbc.c) ------------------------------------ */
+/* parameters of src/rbc/params/imp.h:3-9 */
+typedef struct {
+    float gammaC, gammaT, kBT, kb, phi, ks, x0, mpow, ka, kd, kv, totArea, totVolume;
+} orc_rbc_par;
+int    orc_mesh_md(int nv, int nt, const int *tt);                 /* src/io/mesh_read/imp/main.h:109-129 */
+/* ring adjacency, md*nv entries each (src/rbc/adj/imp/ini.h:1-45) */
+int    orc_adj_ini(int md, int nt, int nv, const int *tt, int *adj0, int *adj1);
+/* (vertex, slot) -> i0 i1 i2 i3 i4 cell; 0 for an empty slot (src/rbc/adj/dev.h:2-42) */
+int    orc_adj_map(int md, int nv, const int *adj0, const int *adj1, int i, int m[6]);
+/* per cell [area, volume] in FP64 (src/rbc/force/area_volume/dev/main.h) */
+void   orc_rbc_area_volume(int nt, int nv, int nc, const int *tt, const float *pp, double *av);
+/* stress-free edge lengths and triangle areas per slot (src/rbc/shape/imp/main.h) */
+void   orc_rbc_shape(int md, int nv, const int *adj0, const int *adj1, const float *rr, float *ll, float *aa);
+void   orc_rbc_stressful(int nt, float totArea, float *l0, float *a0);  /* src/rbc/force/imp/main.h:50-59 */
+/* area + volume + spring + viscous forces ADDED to ff (double, 3 per
+ * vertex); ll/aa NULL selects the stress-full l0/a0
+ * (src/rbc/force/dev/main.h:48-77, common.h:1-108) */
+void   orc_rbc_forces(const orc_rbc_par *p, int md, int nv, int nc, const int *adj0, const int *adj1,
+                      const float *pp, const double *av, const float *ll, const float *aa, float l0, float a0,
+                      double *ff);
+/* Kantor bending ADDED to ff (src/rbc/force/bending/kantor/dev/main.h:23-66) */
+void   orc_rbc_bending(const orc_rbc_par *p, int md, int nv, int nc, const int *adj0, const int *adj1,
+                       const float *pp, double *ff);
+/* membrane energy at double positions r (3 per vertex); terms bit mask:
+ * 1 global area, 2 volume, 4 local area, 8 springs, 16 bending */
+double orc_rbc_energy(const orc_rbc_par *p, int md, int nv, int nc, int nt, const int *tt, const int *adj0,
+                      const int *adj1, const double *r, const float *ll, const float *aa, float l0, float a0,
+                      int terms);
+void   orc_rbc_stretch(int nc, int nv, const float *f, float *ff);     /* src/rbc/stretch/dev/main.h */
+
 #ifdef __cplusplus
 }
 #endif
