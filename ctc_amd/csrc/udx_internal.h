@@ -160,6 +160,23 @@ int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float 
                  const float4 *ff, float4 *pos, float4 *vel, const SdfField *q);
 int diag_aos(hipStream_t s, int n, const float *pp, double *partials_dev, double out[5]);
 
+// ------------------------------------------------------------------ FSI
+struct FsiBufs {
+    Box box;
+    int max_obj;
+    int *flag, *aid, *scan_tmp;   // 26 * max_obj halo flags and their exclusive scan
+    float *inst, *inst_sorted;    // vertex and image instances (AoS), 8 * max_obj
+    int2 *meta, *meta_sorted;     // (noise index, image?)
+    ClistBufs cl;                 // instances on the (L+2)^3 grid of cells -1..L
+};
+int fsi_alloc(FsiBufs *f, const Box &b, int max_obj);
+void fsi_free(FsiBufs *f);
+// +f on the vertices (fo, stride 3) and -f on the solvent (fs, fs_stride),
+// both ADDED; seeds_dev = [bulk, halo]
+int fsi_forces(hipStream_t s, FsiBufs *f, int ns, const float4 *spos, const float4 *svel, const int *starts,
+               int no, const float *opp, const udx_pair_dpd &p, const float *seeds_dev, float *fo, float *fs,
+               int fs_stride, unsigned long long *pairs, int exact);
+
 }  // namespace udx
 
 struct udx_ctx {

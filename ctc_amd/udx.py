@@ -102,6 +102,13 @@ _SIGS = {
     "udx_rbc_av": (_VP, [_VP]),
     "udx_rbc_force_apply": (_I, [_VP, C.POINTER(RbcParams), _F, _I, _VP, _VP]),
     "udx_rbc_stretch_apply": (_I, [_VP, _I, _I, _VP, _VP]),
+    "udx_fsi_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "udx_fsi_fin": (_I, [_VP]),
+    "udx_fsi_set_mode": (_I, [_VP, _I]),
+    "udx_fsi_bind_solvent": (_I, [_VP, _VP, _VP, _I, _VP]),
+    "udx_fsi_apply": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _VP]),
+    "udx_fsi_apply_seeds": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _VP, C.POINTER(_F)]),
+    "udx_fsi_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -539,3 +546,35 @@ class Rbc:
 
 def rbc_stretch_apply(ctx, nc, nv, f, ff):
     _ok(lib().udx_rbc_stretch_apply(ctx.h, nc, nv, _ptr(f), _ptr(ff)))
+
+
+# ---------------------------------------------------------------- FSI
+class Fsi:
+    """Solvent-membrane DPD (src/fsi/imp.h:6-13)."""
+
+    def __init__(self, ctx, L, max_vertices, exact=False, mode=None):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _ok(lib().udx_fsi_ini(ctx.h, L[0], L[1], L[2], max_vertices, C.byref(self.h)))
+        _ok(lib().udx_fsi_set_mode(self.h, _mode(exact, mode)))
+
+    def bind_solvent(self, pp, ff, starts_ptr):
+        self._bound = (pp, ff)
+        _ok(lib().udx_fsi_bind_solvent(self.h, _ptr(pp), _ptr(ff), pp.shape[0], starts_ptr))
+
+    def apply(self, params, opp, off, seeds=None):
+        if seeds is None:
+            _ok(lib().udx_fsi_apply(self.h, C.byref(params), opp.shape[0], _ptr(opp), _ptr(off)))
+        else:
+            s = (C.c_float * 2)(*[float(x) for x in seeds])
+            _ok(lib().udx_fsi_apply_seeds(self.h, C.byref(params), opp.shape[0], _ptr(opp), _ptr(off), s))
+
+    def pairs(self, reset=False):
+        v = C.c_ulonglong()
+        _ok(lib().udx_fsi_pairs(self.h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().udx_fsi_fin(self.h)
+            self.h = C.c_void_p()

@@ -215,6 +215,29 @@ int udx_rbc_force_apply(udx_rbc *r, const udx_rbc_params *p, float dt, int nc, c
 /* ff[i] += f[i % nv] for nc * nv vertices (f: device, 3 * nv) */
 int udx_rbc_stretch_apply(udx_ctx *ctx, int nc, int nv, const float *f, float *ff);
 
+/* ---- solvent-membrane interactions: replaces fsi_ini / fsi_bind_solvent /
+ *      fsi_bulk / fsi_halo for one membrane type (src/fsi/imp.h:6-13) ----
+ * One periodic domain with even sides: the vertices outside the domain act
+ * through their periodic images, with the second seed of the step and their
+ * position in the halo list (fragment-major, vertex order within a
+ * fragment) as noise index.  Both sides are gathers in a fixed order (no
+ * atomics); +f goes to the vertex, -f to the solvent particle. */
+typedef struct udx_fsi udx_fsi;
+int udx_fsi_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int max_vertices, udx_fsi **f);
+int udx_fsi_fin(udx_fsi *f);
+int udx_fsi_set_mode(udx_fsi *f, int mode);      /* UDX_MODE_FAST or UDX_MODE_EXACT */
+/* pp: cell-ordered solvent (AoS) with the starts of its cell list (ncells + 1);
+ * ff: solvent forces (Force) the interactions are ADDED to */
+int udx_fsi_bind_solvent(udx_fsi *f, const float *pp, float *ff, int n, const int *starts);
+/* nv membrane vertices opp (AoS, any position) with forces off (ADDED);
+ * draws the step's two seeds (bulk, halo) from the FSI trunk
+ * (1908, 1409, 290, 12968) (src/fsi/imp/main.h:1-8) */
+int udx_fsi_apply(udx_fsi *f, const udx_pair_dpd *p, int nv, const float *opp, float *off);
+int udx_fsi_apply_seeds(udx_fsi *f, const udx_pair_dpd *p, int nv, const float *opp, float *off,
+                        const float seeds[2]);
+/* vertex-solvent pairs (r^2 < 1) since the last reset */
+int udx_fsi_pairs(udx_fsi *f, unsigned long long *npairs, int reset);
+
 #ifdef __cplusplus
 }
 #endif

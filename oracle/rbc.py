@@ -4,7 +4,7 @@ import ctypes as C
 
 import numpy as np
 
-from .oracle import F32P, F64P, I32P, _f32, _i32, _ptr, lib
+from .oracle import Dpd, F32P, F64P, I32P, _f32, _i32, _ptr, lib
 
 
 class RbcPar(C.Structure):
@@ -38,6 +38,8 @@ def _lib():
             "orc_rbc_energy": (C.c_double, [P, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, I32P, F64P, F32P,
                                             F32P, C.c_float, C.c_float, C.c_int]),
             "orc_rbc_stretch": (None, [C.c_int, C.c_int, F32P, F32P]),
+            "orc_fsi_forces": (C.c_long, [I32P, C.c_int, F32P, I32P, C.c_int, F32P, Dpd, F32P, F32P, F32P]),
+            "orc_fsi_seeds": (None, [C.c_int, F32P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -171,3 +173,21 @@ def place(rr, matrices, L):
 
 def _f64(a):
     return np.ascontiguousarray(a, np.float64)
+
+
+def fsi_forces(L, spp, starts, opp, params, seeds, fo=None, fs=None):
+    """solvent-membrane DPD; returns (fo, fs, pairs)"""
+    spp = _f32(spp).reshape(-1, 6)
+    opp = _f32(opp).reshape(-1, 6)
+    fo = np.zeros((len(opp), 3), np.float32) if fo is None else _f32(fo).copy()
+    fs = np.zeros((len(spp), 3), np.float32) if fs is None else _f32(fs).copy()
+    n = _lib().orc_fsi_forces(_ptr(_i32(L), I32P), len(spp), _ptr(spp, F32P), _ptr(_i32(starts), I32P), len(opp),
+                              _ptr(opp, F32P), Dpd(*params), _ptr(_f32(seeds), F32P), _ptr(fo, F32P),
+                              _ptr(fs, F32P))
+    return fo, fs, int(n)
+
+
+def fsi_seeds(nsteps):
+    out = np.zeros(2 * nsteps, np.float32)
+    _lib().orc_fsi_seeds(nsteps, _ptr(out, F32P))
+    return out.reshape(nsteps, 2)

@@ -166,6 +166,14 @@ double orc_rbc_energy(const orc_rbc_par *p, int md, int nv, int nc, int nt, cons
                       int terms);
 void   orc_rbc_stretch(int nc, int nv, const float *f, float *ff);     /* src/rbc/stretch/dev/main.h */
 
+/* ---- solvent-membrane interactions (udx_oracle_fsi.c) ------------------- */
+/* spp: cell-ordered solvent with starts (ncells + 1); opp: no membrane
+ * vertices (any position); seeds[0] bulk, seeds[1] halo.  Forces ADDED to fo
+ * (vertices) and fs (solvent); returns the number of pairs (src/fsi/dev/bulk.h, halo.h) */
+long   orc_fsi_forces(const int L[3], int ns, const float *spp, const int *starts, int no, const float *opp,
+                      orc_dpd p, const float seeds[2], float *fo, float *fs);
+void   orc_fsi_seeds(int nsteps, float *out /* 2 per step: bulk, halo */);
+
 #ifdef __cplusplus
 }
 #endif

@@ -155,3 +155,81 @@ def test_stretch(ctx):
     ffd = dev(ff0)
     U.rbc_stretch_apply(ctx, nc, nv, dev(f), ffd)
     assert np.array_equal(host(ffd), R.stretch(nc, nv, f, ff0))
+
+
+# ------------------------------------------------------------------ FSI
+def fsi_case(L, rho, seed, out):
+    import oracle as O
+    rng = np.random.default_rng(seed)
+    n = int(rho * np.prod(L))
+    spp = np.zeros((n, 6), np.float32)
+    spp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    spp[:, 3:] = rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+    starts, counts, srt, _ = O.clist_build(L, spp)
+    m = mesh(498)
+    opp = R.place(m.rr, rotations(3, seed), (0, 0, 0))
+    # three cells: one near the upper corner (outside on every axis), one
+    # near the lower corner, one inside
+    c = [np.array(L, np.float32) / 2 - np.array(out, np.float32), -np.array(L, np.float32) / 2 + 0.7,
+         np.zeros(3, np.float32)]
+    for k in range(3):
+        sl = slice(k * m.nv, (k + 1) * m.nv)
+        opp[sl, :3] = (opp[sl, :3] - opp[sl, :3].mean(0) + c[k]).astype(np.float32)
+    opp[:, 3:] = rng.normal(0, 0.3, (len(opp), 3)).astype(np.float32)
+    return srt, starts, opp
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_fsi_forces(ctx, exact):
+    """Vertex and solvent forces added to nonzero arrays: exact mode bit for
+    bit with the oracle (same order on both sides), production mode within
+    the force tolerance; pair counts equal."""
+    import oracle as O
+    L = (16, 12, 14)
+    srt, starts, opp = fsi_case(L, 5, 2, (1.0, 1.2, 0.4))
+    par = (2.6666667, 8.0, O.dpd_sigma(0.0444, 8.0, 5e-4), 0.25)
+    seeds = np.array([0.31, 0.77], np.float32)
+    rng = np.random.default_rng(9)
+    fo0 = rng.normal(0, 1, (len(opp), 3)).astype(np.float32)
+    fs0 = rng.normal(0, 1, (len(srt), 3)).astype(np.float32)
+    fo_ref, fs_ref, n_ref = R.fsi_forces(L, srt, starts, opp, par, seeds, fo0, fs0)
+    say("fsi pairs", n_ref, "exact", exact)
+    f = U.Fsi(ctx, L, len(opp), exact=exact)
+    spp_d, fs_d, st_d = dev(srt), dev(fs0), dev(starts, torch.int32)
+    f.bind_solvent(spp_d, fs_d, U.C.c_void_p(st_d.data_ptr()))
+    fo_d = dev(fo0)
+    f.apply(U.PairDpd(*par), dev(opp), fo_d, seeds=seeds)
+    fo, fs = host(fo_d), host(fs_d)
+    assert f.pairs() == n_ref and n_ref > 1000
+    if exact:
+        assert np.array_equal(fo.view(np.uint32), fo_ref.view(np.uint32))
+        assert np.array_equal(fs.view(np.uint32), fs_ref.view(np.uint32))
+    else:
+        for got, want, base in ((fo, fo_ref, fo0), (fs, fs_ref, fs0)):
+            df = want.astype(np.float64) - base
+            frms = np.sqrt((df ** 2).sum(1).mean())
+            tol = 1e-5 * np.maximum(np.abs(df), 0.1 * frms) + 2.4e-7 * np.abs(want)
+            assert np.all(np.abs(got - want) <= tol)
+    f.close()
+
+
+def test_fsi_trunk(ctx):
+    """Two apply calls draw (bulk, halo) seeds from KISS(1908, 1409, 290, 12968)."""
+    import oracle as O
+    L = (12, 12, 12)
+    srt, starts, opp = fsi_case(L, 3, 4, (1.0, 1.0, 1.0))
+    par = (2.6666667, 8.0, 1.0, 0.25)
+    seeds = R.fsi_seeds(2)
+    fo_ref = np.zeros((len(opp), 3), np.float32)
+    fs_ref = np.zeros((len(srt), 3), np.float32)
+    for k in range(2):
+        fo_ref, fs_ref, _ = R.fsi_forces(L, srt, starts, opp, par, seeds[k], fo_ref, fs_ref)
+    f = U.Fsi(ctx, L, len(opp), exact=True)
+    spp_d, fs_d, st_d = dev(srt), torch.zeros((len(srt), 3), device="cuda"), dev(starts, torch.int32)
+    f.bind_solvent(spp_d, fs_d, U.C.c_void_p(st_d.data_ptr()))
+    fo_d = torch.zeros((len(opp), 3), device="cuda")
+    opp_d = dev(opp)
+    f.apply(U.PairDpd(*par), opp_d, fo_d)
+    f.apply(U.PairDpd(*par), opp_d, fo_d)
+    assert np.array_equal(host(fo_d), fo_ref) and np.array_equal(host(fs_d), fs_ref)
+    f.close()

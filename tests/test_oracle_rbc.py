@@ -216,3 +216,91 @@ def test_stretch():
     ff = np.ones((8, 3), np.float32)
     out = R.stretch(2, 4, f, ff)
     assert np.array_equal(out, 1 + np.concatenate([f, f]))
+
+
+# ------------------------------------------------------------------ FSI
+def fsi_setup(L=(12, 10, 14), rho=4, seed=3, out=(1.0, 1.2, 0.4)):
+    """solvent in the box and a 498-vertex cell centred near the upper
+    corner so that some vertices lie outside the box on every axis"""
+    import oracle as O
+    rng = np.random.default_rng(seed)
+    n = int(rho * np.prod(L))
+    spp = np.zeros((n, 6), np.float32)
+    spp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    spp[:, 3:] = rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+    starts, counts, srt, _ = O.clist_build(L, spp)
+    m = mesh(498)
+    c = np.array(L, np.float32) / 2 - np.array(out, np.float32)
+    opp = np.zeros((m.nv, 6), np.float32)
+    opp[:, :3] = (m.rr + c).astype(np.float32)
+    opp[:, 3:] = rng.normal(0, 0.3, (m.nv, 3)).astype(np.float32)
+    return srt, starts, opp
+
+
+def fsi_brute(L, srt, opp, par, seeds):
+    """all (instance, solvent) pairs: vertices and their periodic images
+    (vertices outside the box, shifted back by a period on every subset of
+    their outside axes), the solvent cell within one cell of the instance's
+    cell, r^2 < 1; noise index = vertex or position in the halo list"""
+    import oracle as O
+    Lh = np.array([x // 2 for x in L])
+    inst = [(opp[v], v, v, 0) for v in range(len(opp))]
+    aid = 0
+    for i in range(26):
+        di = np.array(O.frag_i2d3(i))
+        for v in range(len(opp)):
+            d = -1 + (opp[v, :3] >= -Lh).astype(int) + (opp[v, :3] >= Lh).astype(int)
+            s = -di
+            if not np.any(d) or np.any((s != 0) & (s != d)) or not np.any(s):
+                continue
+            q = opp[v].copy()
+            q[:3] = (q[:3] + (np.array(L) * di).astype(np.float32)).astype(np.float32)
+            inst.append((q, v, aid, 1))
+            aid += 1
+    fo = np.zeros((len(opp), 3))
+    fs = np.zeros((len(srt), 3))
+    cs = np.floor(srt[:, :3].astype(np.float64) + np.array(L) / 2).astype(int)
+    npair = 0
+    for q, v, lid, halo in inst:
+        cq = Lh + np.floor(q[:3]).astype(int)
+        near = np.all(np.abs(cs - cq) <= 1, axis=1)
+        for b in np.nonzero(near)[0]:
+            xi = O.rng_uu(seeds[halo], np.array([lid], np.uint32), np.array([b], np.uint32))[0]
+            f, ok = O.pair_force(par, q, srt[b], xi)
+            npair += ok
+            fo[v] += f
+            fs[b] -= f
+    return fo, fs, npair, aid
+
+
+def test_fsi_forces_brute_force():
+    import oracle as O
+    L = (12, 10, 14)
+    srt, starts, opp = fsi_setup(L)
+    par = (2.6666667, 8.0, O.dpd_sigma(0.0444, 8.0, 5e-4), 0.25)
+    seeds = np.array([0.31, 0.77], np.float32)
+    fo, fs, n = R.fsi_forces(L, srt, starts, opp, par, seeds)
+    bo, bs, nb, nimg = fsi_brute(L, srt, opp, par, seeds)
+    assert nimg > 50 and n == nb and n > 1000
+    scale = max(np.abs(bo).max(), np.abs(bs).max())
+    assert np.abs(fo - bo).max() <= 1e-5 * scale
+    assert np.abs(fs - bs).max() <= 1e-5 * scale
+    # momentum: every pair acts on both sides
+    assert np.abs(fo.astype(np.float64).sum(0) + fs.astype(np.float64).sum(0)).max() <= 1e-4 * scale
+
+
+def test_fsi_inside_vertices_make_no_images():
+    import oracle as O
+    L = (12, 10, 14)
+    srt, starts, opp = fsi_setup(L, out=(4.0, 3.5, 2.0))      # the whole cell inside
+    assert np.all(np.abs(opp[:, :3]) < np.array(L) / 2)
+    par = (2.6666667, 8.0, 1.0, 0.25)
+    fo1, fs1, n1 = R.fsi_forces(L, srt, starts, opp, par, np.array([0.3, 0.1], np.float32))
+    fo2, fs2, n2 = R.fsi_forces(L, srt, starts, opp, par, np.array([0.3, 0.9], np.float32))
+    assert n1 == n2 and np.array_equal(fo1, fo2) and np.array_equal(fs1, fs2)   # the halo seed is unused
+
+
+def test_fsi_seeds_trunk():
+    import oracle as O
+    s = R.fsi_seeds(3)
+    assert np.array_equal(s.ravel(), O.kiss_fill(1908, 1409, 290, 12968, 6))
