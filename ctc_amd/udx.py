@@ -109,6 +109,26 @@ _SIGS = {
     "udx_fsi_apply": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _VP]),
     "udx_fsi_apply_seeds": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _VP, C.POINTER(_F)]),
     "udx_fsi_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
+    "udx_conf_ini": (_I, [C.POINTER(_VP)]),
+    "udx_conf_fin": (_I, [_VP]),
+    "udx_conf_set_include_dir": (_I, [_VP, C.c_char_p]),
+    "udx_conf_read": (_I, [_VP, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "udx_conf_has": (_I, [_VP, C.c_char_p]),
+    "udx_conf_lookup_int": (_I, [_VP, C.c_char_p, C.POINTER(_I)]),
+    "udx_conf_lookup_float": (_I, [_VP, C.c_char_p, C.POINTER(_F)]),
+    "udx_conf_lookup_bool": (_I, [_VP, C.c_char_p, C.POINTER(_I)]),
+    "udx_conf_lookup_string": (_I, [_VP, C.c_char_p, C.POINTER(C.c_char_p)]),
+    "udx_conf_lookup_vint": (_I, [_VP, C.c_char_p, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "udx_conf_lookup_vfloat": (_I, [_VP, C.c_char_p, _I, C.POINTER(_I), C.POINTER(_F)]),
+    "udx_conf_lookup_vstring": (_I, [_VP, C.c_char_p, _I, C.POINTER(_I), C.POINTER(C.c_char_p)]),
+    "udx_conf_lookup_int3": (_I, [_VP, C.c_char_p, C.POINTER(_I)]),
+    "udx_conf_lookup_float3": (_I, [_VP, C.c_char_p, C.POINTER(_F)]),
+    "udx_conf_set_int": (_I, [_VP, C.c_char_p, _I]),
+    "udx_conf_set_float": (_I, [_VP, C.c_char_p, _F]),
+    "udx_conf_set_bool": (_I, [_VP, C.c_char_p, _I]),
+    "udx_conf_set_string": (_I, [_VP, C.c_char_p, C.c_char_p]),
+    "udx_pair_set_conf": (_I, [_VP, C.c_char_p, _F, _F, C.POINTER(PairDpd)]),
+    "udx_rbc_params_set_conf": (_I, [_VP, C.c_char_p, C.POINTER(RbcParams)]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -578,3 +598,97 @@ class Fsi:
         if self.h:
             lib().udx_fsi_fin(self.h)
             self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- configuration
+class Conf:
+    """Layered libconfig-subset configuration (src/conf/imp.h)."""
+
+    def __init__(self, default=None, opt=None, args=None, include_dir=None):
+        self.h = C.c_void_p()
+        _ok(lib().udx_conf_ini(C.byref(self.h)))
+        if include_dir is not None:
+            _ok(lib().udx_conf_set_include_dir(self.h, os.fsencode(include_dir)))
+        enc = (lambda x: None if x is None else (os.fsencode(x) if not isinstance(x, bytes) else x))
+        if default is not None or opt is not None or args is not None:
+            _ok(lib().udx_conf_read(self.h, enc(default), enc(opt), None if args is None else args.encode()))
+
+    def has(self, key):
+        return bool(lib().udx_conf_has(self.h, key.encode()))
+
+    def int(self, key):
+        v = C.c_int()
+        _ok(lib().udx_conf_lookup_int(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def float(self, key):
+        v = C.c_float()
+        _ok(lib().udx_conf_lookup_float(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def bool(self, key):
+        v = C.c_int()
+        _ok(lib().udx_conf_lookup_bool(self.h, key.encode(), C.byref(v)))
+        return bool(v.value)
+
+    def string(self, key):
+        v = C.c_char_p()
+        _ok(lib().udx_conf_lookup_string(self.h, key.encode(), C.byref(v)))
+        return v.value.decode()
+
+    def vint(self, key, maxn=64):
+        a, n = (C.c_int * maxn)(), C.c_int()
+        _ok(lib().udx_conf_lookup_vint(self.h, key.encode(), maxn, C.byref(n), a))
+        return list(a[:n.value])
+
+    def vfloat(self, key, maxn=64):
+        a, n = (C.c_float * maxn)(), C.c_int()
+        _ok(lib().udx_conf_lookup_vfloat(self.h, key.encode(), maxn, C.byref(n), a))
+        return list(a[:n.value])
+
+    def vstring(self, key, maxn=64):
+        a, n = (C.c_char_p * maxn)(), C.c_int()
+        _ok(lib().udx_conf_lookup_vstring(self.h, key.encode(), maxn, C.byref(n), a))
+        return [x.decode() for x in a[:n.value]]
+
+    def int3(self, key):
+        a = (C.c_int * 3)()
+        _ok(lib().udx_conf_lookup_int3(self.h, key.encode(), a))
+        return tuple(a)
+
+    def float3(self, key):
+        a = (C.c_float * 3)()
+        _ok(lib().udx_conf_lookup_float3(self.h, key.encode(), a))
+        return tuple(a)
+
+    def set(self, key, value):
+        k = key.encode()
+        if isinstance(value, bool):
+            _ok(lib().udx_conf_set_bool(self.h, k, int(value)))
+        elif isinstance(value, int):
+            _ok(lib().udx_conf_set_int(self.h, k, value))
+        elif isinstance(value, float):
+            _ok(lib().udx_conf_set_float(self.h, k, value))
+        else:
+            _ok(lib().udx_conf_set_string(self.h, k, str(value).encode()))
+
+    def pair(self, ns, kBT, dt):
+        p = PairDpd()
+        _ok(lib().udx_pair_set_conf(self.h, ns.encode(), kBT, dt, C.byref(p)))
+        return p
+
+    def rbc_params(self, ns="rbc"):
+        p = RbcParams()
+        _ok(lib().udx_rbc_params_set_conf(self.h, ns.encode(), C.byref(p)))
+        return p
+
+    def close(self):
+        if self.h:
+            lib().udx_conf_fin(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -238,6 +238,41 @@ int udx_fsi_apply_seeds(udx_fsi *f, const udx_pair_dpd *p, int nv, const float *
 /* vertex-solvent pairs (r^2 < 1) since the last reset */
 int udx_fsi_pairs(udx_fsi *f, unsigned long long *npairs, int reset);
 
+/* ---- configuration: replaces conf_ini / conf_read / conf_lookup_* /
+ *      conf_set_* (src/conf/imp.h) and the parameter readers pair_set_conf
+ *      (src/pair/conf.cpp:17-57) and rbc_params_set_conf
+ *      (src/rbc/params/conf.cpp:11-34) ----
+ * The libconfig subset of the reference's .cfg files.  Four layers are
+ * searched in order: values set by the program, the argument string, the
+ * optional file, the default file.  Lookups are typed ("2" is an int, "2.0"
+ * a float); a missing key or a wrong type is an error (UDX_EINVAL) with the
+ * reference's message.  Host only: no GPU work. */
+typedef struct udx_conf udx_conf;
+int udx_conf_ini(udx_conf **c);
+int udx_conf_fin(udx_conf *c);
+int udx_conf_set_include_dir(udx_conf *c, const char *dir);   /* for @include */
+/* any argument may be NULL; args is settings text ("a = 1 b = { c = 2.0 }") */
+int udx_conf_read(udx_conf *c, const char *default_file, const char *opt_file, const char *args);
+int udx_conf_has(const udx_conf *c, const char *desc);        /* 1 if found in any layer */
+int udx_conf_lookup_int(const udx_conf *c, const char *desc, int *a);
+int udx_conf_lookup_float(const udx_conf *c, const char *desc, float *a);
+int udx_conf_lookup_bool(const udx_conf *c, const char *desc, int *a);
+/* the string stays valid while the udx_conf lives */
+int udx_conf_lookup_string(const udx_conf *c, const char *desc, const char **a);
+int udx_conf_lookup_vint(const udx_conf *c, const char *desc, int maxn, int *n, int *a);
+int udx_conf_lookup_vfloat(const udx_conf *c, const char *desc, int maxn, int *n, float *a);
+int udx_conf_lookup_vstring(const udx_conf *c, const char *desc, int maxn, int *n, const char **a);
+int udx_conf_lookup_int3(const udx_conf *c, const char *desc, int a[3]);
+int udx_conf_lookup_float3(const udx_conf *c, const char *desc, float a[3]);
+int udx_conf_set_int(udx_conf *c, const char *desc, int a);
+int udx_conf_set_float(udx_conf *c, const char *desc, float a);
+int udx_conf_set_bool(udx_conf *c, const char *desc, int a);
+int udx_conf_set_string(udx_conf *c, const char *desc, const char *a);
+/* <ns>.{dpd, a, g, spow} -> grey DPD parameters with sigma from kBT and dt */
+int udx_pair_set_conf(const udx_conf *c, const char *ns, float kBT, float dt, udx_pair_dpd *p);
+/* <ns>.{gammaC, gammaT, kBT, kb, phi, ks, x0, mpow, ka, kd, kv, totArea, totVolume} */
+int udx_rbc_params_set_conf(const udx_conf *c, const char *ns, udx_rbc_params *p);
+
 #ifdef __cplusplus
 }
 #endif
