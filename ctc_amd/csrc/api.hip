@@ -40,16 +40,7 @@ constexpr int NSEED = 28;
 struct Trunks {
     udx_rnd t[NSEED];
     void ini() {
-        for (int f = 0; f < 26; f++) {
-            int d[3] = {(f + 2) % 3 - 1, (f / 3 + 2) % 3 - 1, (f / 9 + 2) % 3 - 1};
-            int sum = d[0] + d[1] + d[2];
-            int plus = (sum > 0) || (sum == 0 && d[0] > 0) || (d[0] == 0 && d[1] > 0) || (d[1] == 0 && d[2] > 0);
-            int sign = 2 * plus - 1, v[3];
-            for (int c = 0; c < 3; c++) v[c] = 1 + sign * d[c];
-            int seed = v[0] + 3 * (v[1] + 3 * v[2]);
-            udx_rnd_ini(390 + seed, seed + 615, 12309, 23094, &t[f]);
-        }
-        udx_rnd_ini(0, 0, 0, 0, &t[26]);
+        for (int f = 0; f < 27; f++) flu_trunk_ini(f, &t[f]);
         udx_rnd_ini(42, 42, 42, 42, &t[27]);
     }
     void next(float out[NSEED]) {
@@ -63,6 +54,20 @@ static Box make_box(int Lx, int Ly, int Lz) {
     b.L[1] = Ly;
     b.L[2] = Lz;
     return b;
+}
+
+void flu_trunk_ini(int f, udx_rnd *r) {
+    if (f == 26) {
+        udx_rnd_ini(0, 0, 0, 0, r);
+        return;
+    }
+    int d[3] = {(f + 2) % 3 - 1, (f / 3 + 2) % 3 - 1, (f / 9 + 2) % 3 - 1};
+    int sum = d[0] + d[1] + d[2];
+    int plus = (sum > 0) || (sum == 0 && d[0] > 0) || (d[0] == 0 && d[1] > 0) || (d[1] == 0 && d[2] > 0);
+    int sign = 2 * plus - 1, v[3];
+    for (int c = 0; c < 3; c++) v[c] = 1 + sign * d[c];
+    int seed = v[0] + 3 * (v[1] + 3 * v[2]);
+    udx_rnd_ini(390 + seed, seed + 615, 12309, 23094, r);
 }
 
 }  // namespace udx

@@ -232,6 +232,18 @@ static int clist_sort_impl(hipStream_t s, ClistBufs *c, int n, Pos pos) {
                            b.L[1], b.L[2], c->counts, c->cellslot);
         UDX_LAUNCHED();
     }
+    return clist_finish(s, c, n);
+}
+
+int clist_clear(hipStream_t s, ClistBufs *c) {
+    const int nc = c->box.ncells();
+    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c->counts, nc, 0);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+int clist_finish(hipStream_t s, ClistBufs *c, int n) {
+    const int nc = c->box.ncells();
     UDX_TRY(scan_exclusive(s, c->counts, nc, c->starts, c->scan_tmp));
     if (n > 0) {
         hipLaunchKernelGGL(k_clist_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c->cellslot,

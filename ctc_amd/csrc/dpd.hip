@@ -15,6 +15,8 @@
 // to a per-lane queue in LDS.  When a queue fills (and at the end) the wave
 // drains the queues together, so the expensive pair body (RNG + force) runs
 // with nearly all lanes active instead of the ~15% that pass the cutoff.
+#include <vector>
+
 #include "udx_dev.h"
 #include "udx_internal.h"
 
@@ -31,6 +33,7 @@ struct DpdK {
     const int *starts, *fss;
     int foff[27];
     int Lx, Ly, Lz, nmax, periodic;
+    int i0;                   // first particle of the box; noise indices are relative to it
     DpdPar par;
     const float *seeds;
     float *ff;
@@ -77,7 +80,7 @@ __device__ __forceinline__ PairIn pair_in(const DpdK &A, unsigned ent, int i, co
     const float4 pj = A.pos[j], vj = A.vel[j];
     float ax = pi.x, ay = pi.y, az = pi.z;
     // bulk pair: mean0var1ii(seed, max, min) (src/fluforces/bulk/dev/main.h:32-38)
-    int u = i > j ? i : j, v = i > j ? j : i;
+    int u = (i > j ? i : j) - A.i0, v = (i > j ? j : i) - A.i0;
     float seed = seed_bulk;
     if (w != W_NONE) {
         // pair across the periodic boundary: the local particle is moved into
@@ -124,16 +127,19 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * per + (x < rem ? x : rem) + k;
 }
 
-template <int WR, bool EXACT>
-__global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
-    __shared__ unsigned q[QCAP * DPD_THREADS];
-    __shared__ float4 sbuf[DPD_THREADS / 64][SCAP + 4];
-    __shared__ unsigned long long lpairs[DPD_THREADS / 64];
+// The force pass of one block of NT consecutive particles starting at i
+// = first + threadIdx.x; QC queued pairs per lane before a drain.
+template <int WR, bool EXACT, int NT, int QC>
+__device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
+    __shared__ unsigned q[QC * NT];
+    __shared__ float4 sbuf[NT / 64][SCAP + 4];
+    __shared__ unsigned long long lpairs[NT / 64];
+    constexpr int DPD_THREADS = NT, QCAP = QC;
     const int tid = threadIdx.x;
     const int Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
     const int n = A.starts[Lx * Ly * Lz];
-    const int i = xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS + tid;
-    const bool active = i < n && i < A.nmax;
+    const int i = first + tid;
+    const bool active = i < n && i - A.i0 < A.nmax;
 
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
     int cx = 0, cy = 0, cz = 0, istart = 0;
@@ -155,7 +161,7 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     // round-toward-zero window.
     // an idle slot evaluates a discarded pair with an in-bounds particle
     // (i itself, or particle 0 for a lane past the end)
-    const unsigned idle = (unsigned)(active ? i : 0) | (W_NONE << 27);
+    const unsigned idle = (unsigned)(active ? i : A.i0) | (W_NONE << 27);
     auto drain = [&]() {
         for (int e = 0; wave_any(e < fill); e += 2) {
             const bool va = e < fill, vb = e + 1 < fill;
@@ -295,6 +301,27 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
     }
 }
 
+template <int WR, bool EXACT>
+__global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
+    dpd_block<WR, EXACT, DPD_THREADS, QCAP>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+}
+
+// Many boxes in one launch: block b belongs to the box k with
+// bblock[k] <= b < bblock[k + 1]; each box's blocks start at its first
+// particle, so no block spans two boxes.
+template <int WR, bool EXACT, int NT, int QC>
+__global__ __launch_bounds__(NT) void k_dpd_batch(const DpdK *boxes, const int *bblock, int nbox) {
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bblock[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    const DpdK &A = boxes[lo];
+    dpd_block<WR, EXACT, NT, QC>(A, A.i0 + (blk - bblock[lo]) * NT);
+}
+
 // Counts of the cells of the 26 fragments in fragment-cell order (x
 // fastest, src/exch/flu/dev/map.h:22-42), one extra 0 after each fragment.
 // An exclusive scan over this array gives every fragment's packed-list
@@ -317,6 +344,28 @@ __global__ __launch_bounds__(256) void k_frag_counts(const int *starts, int Lx, 
         v = starts[cid + 1] - starts[cid];
     }
     cnt[g] = v;
+}
+
+DpdK make_k(const DpdLaunch &a) {
+    DpdK k;
+    k.pos = a.pos;
+    k.vel = a.vel;
+    k.starts = a.starts;
+    k.fss = a.frag->fss;
+    for (int f = 0; f < 27; f++) k.foff[f] = a.frag->off[f];
+    k.Lx = a.box.L[0];
+    k.Ly = a.box.L[1];
+    k.Lz = a.box.L[2];
+    k.nmax = a.nmax;
+    k.periodic = a.periodic;
+    k.i0 = a.i0;
+    k.par = DpdPar{a.par.a, a.par.g, a.par.s, a.par.spow};
+    k.seeds = a.seeds;
+    k.ff = a.ff;
+    k.ff_stride = a.ff_stride;
+    k.accumulate = a.accumulate;
+    k.pairs = a.pairs;
+    return k;
 }
 
 template <int WR, bool EXACT>
@@ -368,23 +417,7 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
               a.box.L[2]);
     UDX_CHECK(a.nmax < (1 << 27), UDX_ECAP, "dpd: %d particles exceed the 2^27 queue index range", a.nmax);
     if (a.nmax <= 0) return UDX_OK;
-    DpdK k;
-    k.pos = a.pos;
-    k.vel = a.vel;
-    k.starts = a.starts;
-    k.fss = a.frag->fss;
-    for (int f = 0; f < 27; f++) k.foff[f] = a.frag->off[f];
-    k.Lx = a.box.L[0];
-    k.Ly = a.box.L[1];
-    k.Lz = a.box.L[2];
-    k.nmax = a.nmax;
-    k.periodic = a.periodic;
-    k.par = DpdPar{a.par.a, a.par.g, a.par.s, a.par.spow};
-    k.seeds = a.seeds;
-    k.ff = a.ff;
-    k.ff_stride = a.ff_stride;
-    k.accumulate = a.accumulate;
-    k.pairs = a.pairs;
+    const DpdK k = make_k(a);
     const float sp = a.par.spow;
     if (exact) {
         if (sp == 0.25f) return launch_dpd<WR_QUARTER, true>(s, k, a.nmax);
@@ -399,5 +432,88 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
 }
 
 int dpd_forces(hipStream_t s, const DpdLaunch &a) { return dpd_forces_mode(s, a, 0); }
+
+// ---------------------------------------------------------------- batches
+namespace {
+constexpr int VAR_SHAPE[DPD_NVARIANT][2] = {{256, 24}, {128, 16}, {128, 24}, {128, 32}, {256, 16}, {256, 32}};
+
+
+template <int WR, bool EXACT, int NT, int QC>
+int launch_batch(hipStream_t s, const DpdBatch &b) {
+    const int w = NT == 128 ? 0 : 1;
+    if (b.nblocks[w] == 0) return UDX_OK;
+    hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC>), dim3(b.nblocks[w]), dim3(NT), 0, s,
+                       (const DpdK *)b.table, b.bblock[w], b.nbox);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+template <int WR, bool EXACT>
+int launch_batch_wr(hipStream_t s, const DpdBatch &b) {
+    return launch_batch<WR, EXACT, 256, 24>(s, b);
+}
+}  // namespace
+
+void dpd_variant_shape(int v, int *threads, int *qcap) {
+    *threads = VAR_SHAPE[v][0];
+    *qcap = VAR_SHAPE[v][1];
+}
+
+int dpd_batch_build(DpdBatch *b, const DpdLaunch *boxes, int nbox) {
+    UDX_CHECK(nbox > 0, UDX_EINVAL, "dpd: empty batch");
+    std::vector<DpdK> ks(nbox);
+    std::vector<int> bb[2] = {std::vector<int>(nbox + 1), std::vector<int>(nbox + 1)};
+    for (int k = 0; k < nbox; k++) {
+        const DpdLaunch &a = boxes[k];
+        UDX_CHECK(!a.periodic || (a.box.L[0] >= 3 && a.box.L[1] >= 3 && a.box.L[2] >= 3), UDX_EINVAL,
+                  "dpd: box %d: a periodic box needs at least 3 cells per side", k);
+        UDX_CHECK(a.par.spow == boxes[0].par.spow, UDX_EINVAL, "dpd: the boxes of a batch share spow");
+        ks[k] = make_k(a);
+        for (int w = 0; w < 2; w++) bb[w][k + 1] = bb[w][k] + ceil_div(a.nmax, w == 0 ? 128 : 256);
+    }
+    UDX_CHECK((long)boxes[nbox - 1].i0 + boxes[nbox - 1].nmax < (1L << 27), UDX_ECAP,
+              "dpd: the batch exceeds the 2^27 queue index range");
+    dpd_batch_free(b);
+    UDX_HIP(hipMalloc(&b->table, sizeof(DpdK) * nbox));
+    UDX_HIP(hipMemcpy(b->table, ks.data(), sizeof(DpdK) * nbox, hipMemcpyHostToDevice));
+    for (int w = 0; w < 2; w++) {
+        UDX_TRY(dev_alloc(&b->bblock[w], (size_t)nbox + 1));
+        UDX_HIP(hipMemcpy(b->bblock[w], bb[w].data(), sizeof(int) * (nbox + 1), hipMemcpyHostToDevice));
+        b->nblocks[w] = bb[w][nbox];
+    }
+    b->nbox = nbox;
+    b->spow = boxes[0].par.spow;
+    return UDX_OK;
+}
+
+void dpd_batch_free(DpdBatch *b) {
+    if (b->table) (void)hipFree(b->table);
+    b->table = nullptr;
+    dev_free(b->bblock[0]);
+    dev_free(b->bblock[1]);
+}
+
+int dpd_forces_batch(hipStream_t s, const DpdBatch &b, int mode, int variant) {
+    const float sp = b.spow;
+    if (mode == UDX_MODE_EXACT) {
+        if (sp == 0.25f) return launch_batch_wr<WR_QUARTER, true>(s, b);
+        if (sp == 0.5f) return launch_batch_wr<WR_HALF, true>(s, b);
+        if (sp == 1.0f) return launch_batch_wr<WR_ONE, true>(s, b);
+        return launch_batch_wr<WR_POW, true>(s, b);
+    }
+    if (sp == 0.25f) {
+        switch (variant) {
+            case 1: return launch_batch<WR_QUARTER, false, 128, 16>(s, b);
+            case 2: return launch_batch<WR_QUARTER, false, 128, 24>(s, b);
+            case 3: return launch_batch<WR_QUARTER, false, 128, 32>(s, b);
+            case 4: return launch_batch<WR_QUARTER, false, 256, 16>(s, b);
+            case 5: return launch_batch<WR_QUARTER, false, 256, 32>(s, b);
+            default: return launch_batch<WR_QUARTER, false, 256, 24>(s, b);
+        }
+    }
+    if (sp == 0.5f) return launch_batch_wr<WR_HALF, false>(s, b);
+    if (sp == 1.0f) return launch_batch_wr<WR_ONE, false>(s, b);
+    return launch_batch_wr<WR_POW, false>(s, b);
+}
 
 }  // namespace udx

@@ -8,22 +8,6 @@ namespace udx {
 
 namespace {
 
-// periodic wrap into [-L/2, L/2); a result rounded onto +L/2 maps to -L/2
-__device__ __forceinline__ float wrap_coord(float x, float L) {
-    const float h = 0.5f * L;
-    if (x < -h) x = x + L;
-    if (x >= h) x = x - L;
-    return x;
-}
-
-// kick-drift with the constant body force folded in
-// (src/scheme/move/dev/vv.h:1-10, src/scheme/force/dev/main.h:1-4)
-__device__ __forceinline__ void kick_drift(float dt, float mass, float a, float f, float *r, float *v) {
-    const float ft = __builtin_fmaf(mass, a, f);
-    *v = __builtin_fmaf(ft / mass, dt, *v);
-    *r = __builtin_fmaf(*v, dt, *r);
-}
-
 __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
                                                  float dt, float mass, float ax, float ay, float az,
                                                  const float4 *ff, float4 *pos, float4 *vel) {

@@ -157,6 +157,23 @@ __device__ __forceinline__ float3 dpd_pair(const DpdPar &p, float dx, float dy, 
     return make_float3(f0 * ex, f0 * ey, f0 * ez);
 }
 
+// ---------------------------------------------------------------- integrator
+// periodic wrap into [-L/2, L/2); a result rounded onto +L/2 maps to -L/2
+__device__ __forceinline__ float wrap_coord(float x, float L) {
+    const float h = 0.5f * L;
+    if (x < -h) x = x + L;
+    if (x >= h) x = x - L;
+    return x;
+}
+
+// kick-drift with the constant body force folded in
+// (src/scheme/move/dev/vv.h:1-10, src/scheme/force/dev/main.h:1-4)
+__device__ __forceinline__ void kick_drift(float dt, float mass, float a, float f, float *r, float *v) {
+    const float ft = __builtin_fmaf(mass, a, f);
+    *v = __builtin_fmaf(ft / mass, dt, *v);
+    *r = __builtin_fmaf(*v, dt, *r);
+}
+
 // ---------------------------------------------------------------- fragments
 // src/frag/dev.h:19-36
 __device__ __forceinline__ int frag_d2i(int x, int y, int z) {

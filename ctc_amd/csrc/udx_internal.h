@@ -84,6 +84,10 @@ void clist_free(ClistBufs *c);
 // starts[] filled; the number kept is starts[ncells] on the device
 int clist_sort(hipStream_t s, ClistBufs *c, int n, const float4 *pos4);
 int clist_sort_aos(hipStream_t s, ClistBufs *c, int n, const float *pp);
+// the two halves of a sort for callers that fill counts[] and cellslot[]
+// themselves: zero the counts; scan, scatter and the stable in-cell order
+int clist_clear(hipStream_t s, ClistBufs *c);
+int clist_finish(hipStream_t s, ClistBufs *c, int n);
 // out[k] = in[order[k]] for k < starts[ncells]; n bounds the launch
 int gather_f4(hipStream_t s, const ClistBufs *c, int n, const float4 *pin, const float4 *vin,
               float4 *pout, float4 *vout);
@@ -117,10 +121,27 @@ struct DpdLaunch {
     int ff_stride;
     int accumulate;
     unsigned long long *pairs;   // ordered-pair counter or null
+    int i0 = 0;             // first particle of the box in pos/vel (noise indices are relative)
 };
 int dpd_forces(hipStream_t s, const DpdLaunch &a);
 // mode: UDX_MODE_FAST or UDX_MODE_EXACT
 int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode);
+
+// many boxes in one launch (one descriptor per box, all sharing spow)
+struct DpdBatch {
+    void *table = nullptr;     // device: per-box kernel descriptors
+    int *bblock[2] = {nullptr, nullptr};   // device: first block of each box for 128 / 256 threads
+    int nblocks[2] = {0, 0};
+    int nbox = 0;
+    float spow = 0.25f;
+};
+// tile variants of the batched force kernel: threads per block and queued
+// pairs per lane; variant 0 is the default
+enum { DPD_NVARIANT = 6 };
+void dpd_variant_shape(int v, int *threads, int *qcap);
+int dpd_batch_build(DpdBatch *b, const DpdLaunch *boxes, int nbox);
+void dpd_batch_free(DpdBatch *b);
+int dpd_forces_batch(hipStream_t s, const DpdBatch &b, int mode, int variant);
 
 // ------------------------------------------------------------------ scheme
 int aos_to_f4(hipStream_t s, int n, const float *pp, float4 *pos, float4 *vel);
@@ -159,6 +180,11 @@ int wall_repulse_aos(hipStream_t s, const SdfField *q, float lambda, int n, cons
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
                  const float4 *ff, float4 *pos, float4 *vel, const SdfField *q);
 int diag_aos(hipStream_t s, int n, const float *pp, double *partials_dev, double out[5]);
+
+// solvent trunk f of one rank: the halo fragments 0..25
+// (src/fluforces/imp/ini.h:11-58, seed_base = 0) and the bulk trunk at 26
+// (src/fluforces/imp/ini.h:6)
+void flu_trunk_ini(int f, udx_rnd *r);
 
 // ------------------------------------------------------------------ FSI
 struct FsiBufs {

@@ -30,6 +30,11 @@ class FluConf(C.Structure):
                 ("mass", C.c_float), ("accel", C.c_float * 3), ("mode", C.c_int)]
 
 
+class EnsBox(C.Structure):
+    _fields_ = [("L", C.c_int * 3), ("n", C.c_int), ("dpd", PairDpd), ("dt", C.c_float), ("mass", C.c_float),
+                ("accel", C.c_float * 3)]
+
+
 class RbcParams(C.Structure):
     """src/rbc/params/imp.h:3-9"""
     _fields_ = [(k, C.c_float) for k in ("gammaC", "gammaT", "kBT", "kb", "phi", "ks", "x0", "mpow", "ka", "kd",
@@ -129,6 +134,16 @@ _SIGS = {
     "udx_conf_set_string": (_I, [_VP, C.c_char_p, C.c_char_p]),
     "udx_pair_set_conf": (_I, [_VP, C.c_char_p, _F, _F, C.POINTER(PairDpd)]),
     "udx_rbc_params_set_conf": (_I, [_VP, C.c_char_p, C.POINTER(RbcParams)]),
+    "udx_ens_ini": (_I, [_VP, _I, C.POINTER(EnsBox), C.POINTER(_VP)]),
+    "udx_ens_fin": (_I, [_VP]),
+    "udx_ens_set_mode": (_I, [_VP, _I]),
+    "udx_ens_set_pp": (_I, [_VP, _VP]),
+    "udx_ens_get_pp": (_I, [_VP, _VP]),
+    "udx_ens_run": (_I, [_VP, _I, C.POINTER(_F)]),
+    "udx_ens_set_graph": (_I, [_VP, _I]),
+    "udx_ens_autotune": (_I, [_VP, _I, C.POINTER(_I), C.POINTER(_F)]),
+    "udx_ens_set_variant": (_I, [_VP, _I]),
+    "udx_ens_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -692,3 +707,92 @@ class Conf:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- ensemble
+class Ensemble:
+    """Independent periodic boxes advanced together (one launch per kernel
+    per step for all boxes)."""
+
+    def __init__(self, ctx, boxes, exact=False, mode=None):
+        """boxes: list of dicts with L, n, params (PairDpd), dt, mass, accel"""
+        self.ctx = ctx
+        arr = (EnsBox * len(boxes))()
+        for k, b in enumerate(boxes):
+            for c in range(3):
+                arr[k].L[c] = b["L"][c]
+                arr[k].accel[c] = b.get("accel", (0.0, 0.0, 0.0))[c]
+            arr[k].n = b["n"]
+            arr[k].dpd = b["params"]
+            arr[k].dt = b["dt"]
+            arr[k].mass = b.get("mass", 1.0)
+        self.nbox = len(boxes)
+        self.n = sum(b["n"] for b in boxes)
+        self.h = C.c_void_p()
+        _ok(lib().udx_ens_ini(ctx.h, self.nbox, arr, C.byref(self.h)))
+        _ok(lib().udx_ens_set_mode(self.h, _mode(exact, mode)))
+
+    def set_pp(self, pp):
+        _ok(lib().udx_ens_set_pp(self.h, _ptr(pp)))
+
+    def get_pp(self, out):
+        _ok(lib().udx_ens_get_pp(self.h, _ptr(out)))
+
+    def run(self, nsteps, seeds=None):
+        if seeds is None:
+            _ok(lib().udx_ens_run(self.h, nsteps, None))
+        else:
+            import numpy as np
+            s = np.ascontiguousarray(seeds, np.float32)
+            assert s.shape == (nsteps, self.nbox, 27)
+            _ok(lib().udx_ens_run(self.h, nsteps, s.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def graph(self, enable=True):
+        _ok(lib().udx_ens_set_graph(self.h, 1 if enable else 0))
+
+    def autotune(self, steps=3):
+        best = C.c_int()
+        ms = (C.c_float * 16)()
+        _ok(lib().udx_ens_autotune(self.h, steps, C.byref(best), ms))
+        return best.value, list(ms[:6])
+
+    def set_variant(self, v):
+        _ok(lib().udx_ens_set_variant(self.h, v))
+
+    def pairs(self, reset=False):
+        v = C.c_ulonglong()
+        _ok(lib().udx_ens_pairs(self.h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().udx_ens_fin(self.h)
+            self.h = C.c_void_p()
+
+
+def ens_trunk_seeds(nbox, nsteps):
+    """the boxes' default seed stream (nsteps, nbox, 27): box b's trunks are
+    the solvent trunks with the first KISS word offset by 7919 b"""
+    import numpy as np
+    tr = []
+    for b in range(nbox):
+        row = []
+        for f in range(27):
+            if f == 26:
+                x, y, z, c = 0, 0, 0, 0
+            else:
+                d = ((f + 2) % 3 - 1, (f // 3 + 2) % 3 - 1, (f // 9 + 2) % 3 - 1)
+                s = sum(d)
+                plus = (s > 0) or (s == 0 and d[0] > 0) or (d[0] == 0 and d[1] > 0) or (d[1] == 0 and d[2] > 0)
+                sign = 2 * int(plus) - 1
+                v = [1 + sign * dc for dc in d]
+                seed = v[0] + 3 * (v[1] + 3 * v[2])
+                x, y, z, c = 390 + seed, seed + 615, 12309, 23094
+            row.append(RndTrunk((x + 7919 * b) & 0xFFFFFFFF, y, z, c))
+        tr.append(row)
+    out = np.zeros((nsteps, nbox, 27), np.float32)
+    for k in range(nsteps):
+        for b in range(nbox):
+            for f in range(27):
+                out[k, b, f] = tr[b][f].get()
+    return out

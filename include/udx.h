@@ -273,6 +273,38 @@ int udx_pair_set_conf(const udx_conf *c, const char *ns, float kBT, float dt, ud
 /* <ns>.{gammaC, gammaT, kBT, kb, phi, ks, x0, mpow, ka, kd, kv, totArea, totVolume} */
 int udx_rbc_params_set_conf(const udx_conf *c, const char *ns, udx_rbc_params *p);
 
+/* ---- ensemble: many independent periodic boxes advanced together ----
+ * No counterpart in the reference (each box there is a separate run of the
+ * solvent step, src/sim/imp/run.h:8-68).  Every kernel of the step is
+ * launched once for all boxes; per box the arithmetic and its order equal a
+ * single-box udx_flu run with the same seeds, so results are identical.
+ * Box b's default trunks are the solvent trunks with the first KISS word
+ * offset by 7919 b (box 0 = the reference's). */
+typedef struct udx_ens_box {
+    int L[3];
+    int n;                     /* particles */
+    udx_pair_dpd dpd;          /* a, gamma, sigma, spow (spow shared by all boxes) */
+    float dt, mass;
+    float accel[3];
+} udx_ens_box;
+typedef struct udx_ens udx_ens;
+int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **e);
+int udx_ens_fin(udx_ens *e);
+int udx_ens_set_mode(udx_ens *e, int mode);      /* UDX_MODE_FAST or UDX_MODE_EXACT */
+/* pp: AoS particles of all boxes, box after box (sum of n) */
+int udx_ens_set_pp(udx_ens *e, const float *pp);
+int udx_ens_get_pp(udx_ens *e, float *pp);       /* cell order within each box */
+/* seeds: NULL (the boxes' trunks) or nsteps x nbox x 27 floats (26 halo
+ * fragments then the bulk, as udx_fluforces_apply_seeds) */
+int udx_ens_run(udx_ens *e, int nsteps, const float *seeds);
+/* replay each pair of steps from one captured hipGraph */
+int udx_ens_set_graph(udx_ens *e, int enable);
+/* time the force kernel's tile variants (threads per block, queued pairs
+ * per lane) over `steps` steps each and keep the fastest; ms[v] per step */
+int udx_ens_autotune(udx_ens *e, int steps, int *best, float *ms);
+int udx_ens_set_variant(udx_ens *e, int variant);
+int udx_ens_pairs(udx_ens *e, unsigned long long *npairs, int reset);   /* unordered pairs */
+
 #ifdef __cplusplus
 }
 #endif

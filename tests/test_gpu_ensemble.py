@@ -1,0 +1,157 @@
+"""Batched ensemble of periodic boxes: identical to running every box on its
+own (bit for bit, both modes), to the oracle in exact mode, with and without
+the captured graph, for every tile variant of the batched force kernel."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from ctc_amd import udx as U  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return U.Context()
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def make_boxes(nbox, seed, edges=(3, 12), dt=1e-3):
+    """uneven boxes: edges in [3, 12], density in {3, 5, 10}, gamma in
+    [4, 10], kBT in [0.04, 1]"""
+    rng = np.random.default_rng(seed)
+    boxes, pps = [], []
+    for b in range(nbox):
+        L = tuple(int(x) for x in rng.integers(edges[0], edges[1] + 1, 3))
+        rho = int(rng.choice([3, 5, 10]))
+        g = float(rng.uniform(4, 10))
+        kBT = float(rng.uniform(0.04, 1.0))
+        n = rho * L[0] * L[1] * L[2]
+        pp = np.zeros((n, 6), np.float32)
+        pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+        pp[:, 3:] = rng.normal(0, np.sqrt(kBT), (n, 3)).astype(np.float32)
+        par = (25.0, g, O.dpd_sigma(kBT, g, dt), 0.25)
+        boxes.append(dict(L=L, n=n, params=U.PairDpd(*par), dt=dt, mass=1.0, accel=(0.1 * b, 0.0, 0.0),
+                          par=par))
+        pps.append(pp)
+    return boxes, pps
+
+
+def run_single(ctx, box, pp, seeds27, exact):
+    """one box alone through udx_flu with the same seeds"""
+    nsteps = len(seeds27)
+    flu = U.Flu(ctx, box["L"], box["n"], box["params"], box["dt"], 1.0, box["accel"], exact=exact)
+    d = dev(pp)
+    flu.set_pp(d)
+    s28 = np.concatenate([seeds27, np.zeros((nsteps, 1), np.float32)], axis=1)
+    flu.run(nsteps, s28)
+    out = torch.zeros_like(d)
+    flu.get_pp(out)
+    pairs = flu.pairs()
+    flu.close()
+    return host(out), pairs
+
+
+def run_ensemble(ctx, boxes, pps, nsteps, seeds, exact, graph=False, variant=None):
+    e = U.Ensemble(ctx, boxes, exact=exact)
+    if variant is not None:
+        e.set_variant(variant)
+    e.graph(graph)
+    d = dev(np.concatenate(pps))
+    e.set_pp(d)
+    e.run(nsteps, seeds)
+    out = torch.zeros_like(d)
+    e.get_pp(out)
+    pairs = e.pairs()
+    e.close()
+    return host(out), pairs
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_ensemble_equals_single_boxes(ctx, exact):
+    nbox, nsteps = 7, 12
+    boxes, pps = make_boxes(nbox, 3)
+    seeds = U.ens_trunk_seeds(nbox, nsteps)
+    say("ensemble vs single boxes, exact", exact, [b["L"] for b in boxes])
+    got, pairs = run_ensemble(ctx, boxes, pps, nsteps, None, exact)    # the default trunks = seeds
+    off = np.cumsum([0] + [b["n"] for b in boxes])
+    total = 0
+    for b in range(nbox):
+        want, p = run_single(ctx, boxes[b], pps[b], seeds[:, b], exact)
+        total += p
+        assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), want.view(np.uint32)), "box %d" % b
+    assert pairs == total
+
+
+def test_ensemble_exact_equals_oracle(ctx):
+    nbox, nsteps = 4, 6
+    boxes, pps = make_boxes(nbox, 5, edges=(3, 8))
+    seeds = U.ens_trunk_seeds(nbox, nsteps)
+    got, _ = run_ensemble(ctx, boxes, pps, nsteps, seeds, True)
+    off = np.cumsum([0] + [b["n"] for b in boxes])
+    for b in range(nbox):
+        ref = pps[b].copy()
+        for k in range(nsteps):
+            ref, _ = O.flu_step(boxes[b]["L"], ref, boxes[b]["par"], seeds[k, b], boxes[b]["dt"],
+                                accel=boxes[b]["accel"])
+        assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), ref.view(np.uint32)), "box %d" % b
+
+
+def test_graph_replays_match_eager(ctx):
+    """The captured step pair, replayed with eager work and host syncs in
+    between, gives the eager result after every replay."""
+    nbox = 6
+    boxes, pps = make_boxes(nbox, 7)
+    seeds = U.ens_trunk_seeds(nbox, 10)
+    eg = U.Ensemble(ctx, boxes)
+    gr = U.Ensemble(ctx, boxes)
+    gr.graph(True)
+    d = dev(np.concatenate(pps))
+    eg.set_pp(d)
+    gr.set_pp(d)
+    a, b = torch.zeros_like(d), torch.zeros_like(d)
+    for rep in range(5):
+        eg.run(2, seeds[2 * rep:2 * rep + 2])
+        gr.run(2, seeds[2 * rep:2 * rep + 2])
+        eg.get_pp(a)
+        gr.get_pp(b)
+        _ = torch.ones(1000, device="cuda").sum().item()      # eager work between replays
+        assert np.array_equal(host(a).view(np.uint32), host(b).view(np.uint32)), "replay %d" % rep
+    eg.close()
+    gr.close()
+
+
+def test_tile_variants_identical_and_autotune(ctx):
+    nbox, nsteps = 5, 4
+    boxes, pps = make_boxes(nbox, 11)
+    seeds = U.ens_trunk_seeds(nbox, nsteps)
+    ref, pref = run_ensemble(ctx, boxes, pps, nsteps, seeds, False, variant=0)
+    for v in range(1, 6):
+        got, p = run_ensemble(ctx, boxes, pps, nsteps, seeds, False, variant=v)
+        assert p == pref and np.array_equal(got.view(np.uint32), ref.view(np.uint32)), "variant %d" % v
+    e = U.Ensemble(ctx, boxes)
+    e.set_pp(dev(np.concatenate(pps)))
+    best, ms = e.autotune(2)
+    say("autotune", best, ms)
+    assert 0 <= best < 6 and all(t > 0 for t in ms)
+    e.close()
