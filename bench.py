@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20, help="steps timed per kernel phase")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB copied by one plain device copy before the workload (calibrates HBM counters)")
+    ap.add_argument("--ens-boxes", type=int, default=64, help="boxes of the batched-ensemble leg (0: skip)")
+    ap.add_argument("--ens-steps", type=int, default=40, help="timed steps of the ensemble leg")
     return ap.parse_args()
 
 
@@ -220,7 +222,88 @@ def main():
     if wall is not None:
         wall.close()
         sdf.close()
+    del pp
+    torch.cuda.empty_cache()
+    if args.ens_boxes > 0:
+        out["ensemble"] = ensemble_leg(ctx, args.ens_boxes, args.ens_steps)
     print(json.dumps(out))
+
+
+def ensemble_leg(ctx, nbox, nsteps):
+    """Batched ensemble vs launching each box on its own: nbox periodic
+    boxes, edges uniform in 8..48, density in {3, 5, 10}, gamma in [4, 10],
+    kBT in [0.04, 1], a = 25, dt = 1e-3, one seed stream per box.  Both runs
+    start from the same state with the same seeds and must end identical."""
+    import numpy as np
+    import torch
+
+    from ctc_amd import udx as U
+
+    rng = np.random.default_rng(2024)
+    boxes, pps = [], []
+    for b in range(nbox):
+        L = tuple(int(x) for x in rng.integers(8, 49, 3))
+        rho = int(rng.choice([3, 5, 10]))
+        g, kBT = float(rng.uniform(4, 10)), float(rng.uniform(0.04, 1.0))
+        n = rho * L[0] * L[1] * L[2]
+        boxes.append(dict(L=L, n=n, params=U.dpd_params(25.0, g, kBT, 1e-3, 0.25), dt=1e-3, mass=1.0,
+                          accel=(0.0, 0.0, 0.0)))
+        pp = torch.empty((n, 6), dtype=torch.float32, device="cuda")
+        pp[:, :3] = (torch.rand((n, 3), device="cuda") - 0.5) * torch.tensor(L, device="cuda", dtype=torch.float32)
+        for c in range(3):
+            pp[:, c].clamp_(-0.5 * L[c], 0.5 * L[c] - 1e-4)
+        pp[:, 3:] = torch.randn((n, 3), device="cuda") * float(np.sqrt(kBT))
+        pps.append(pp)
+    ntot = sum(b["n"] for b in boxes)
+    seeds = U.ens_trunk_seeds(nbox, 2 * nsteps)   # warm-up run, then the timed run
+    allpp = torch.cat(pps)
+
+    # batched: autotuned tile, graph-captured step pairs
+    ens = U.Ensemble(ctx, boxes)
+    ens.set_pp(allpp)
+    best, ms = ens.autotune(2)
+    ens.set_pp(allpp)
+    ens.graph(True)
+    ens.run(nsteps, seeds[:nsteps])             # warm-up (captures the graph)
+    ens.pairs(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run(nsteps, seeds[nsteps:])
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    pairs = ens.pairs()
+    got = torch.empty_like(allpp)
+    ens.get_pp(got)
+    ens.close()
+
+    # each box on its own: its full run of warm-up + timed steps, one box after another
+    off = np.cumsum([0] + [b["n"] for b in boxes])
+    t_s = 0.0
+    same = True
+    for b in range(nbox):
+        x = boxes[b]
+        flu = U.Flu(ctx, x["L"], x["n"], x["params"], x["dt"], 1.0, x["accel"])
+        flu.set_pp(pps[b])
+        s28 = np.concatenate([seeds[:, b], np.zeros((2 * nsteps, 1), np.float32)], axis=1)
+        flu.run(nsteps, s28[:nsteps])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        flu.run(nsteps, s28[nsteps:])
+        torch.cuda.synchronize()
+        t_s += time.perf_counter() - t0
+        mine = torch.empty_like(pps[b])
+        flu.get_pp(mine)
+        torch.cuda.synchronize()
+        same = same and bool(torch.equal(mine.view(torch.int32), got[off[b]:off[b + 1]].view(torch.int32)))
+        flu.close()
+    return {
+        "boxes": nbox, "particles": ntot, "steps": nsteps,
+        "batched_steps_per_s": nsteps / t_b, "per_box_steps_per_s": nsteps / t_s,
+        "speedup": t_s / t_b, "batched_pairs_per_s": pairs / t_b,
+        "tile_variant": best, "tile_ms": ms, "identical_to_per_box": same,
+        "workload": "64 periodic boxes, edges U{8..48}, density {3,5,10}, gamma U[4,10], kBT U[0.04,1], "
+                    "a = 25, dt = 1e-3, graph-captured, autotuned" if nbox == 64 else "%d boxes" % nbox,
+    }
 
 
 def host(t):
