@@ -166,6 +166,14 @@ double orc_rbc_energy(const orc_rbc_par *p, int md, int nv, int nc, int nt, cons
                       int terms);
 void   orc_rbc_stretch(int nc, int nv, const float *f, float *ff);     /* src/rbc/stretch/dev/main.h */
 
+/* membrane adjoint (udx_oracle_adj.c): for a cotangent g (3 per vertex),
+ * gx = (dF/dx)^T g = (dF/dx) g (F = -grad E, symmetric Jacobian) of the
+ * conservative membrane force, terms[k] the unit-coefficient term forces
+ * (k: ka, kd, kv, ks, kb; NULL to skip) and gp[k] = g . terms[k] = dL/dk */
+void   orc_rbc_adjoint(const orc_rbc_par *p, int md, int nv, int nc, int nt, const int *tt, const int *adj0,
+                       const int *adj1, const float *pp, const float *ll, const float *aa, float l0, float a0,
+                       const double *g, double *gx, double *terms, double gp[5]);
+
 /* ---- solvent-membrane interactions (udx_oracle_fsi.c) ------------------- */
 /* spp: cell-ordered solvent with starts (ncells + 1); opp: no membrane
  * vertices (any position); seeds[0] bulk, seeds[1] halo.  Forces ADDED to fo

@@ -45,10 +45,8 @@ def test_area_volume_golden():
     assert abs(A - want[0]) < 1e-3 and abs(V / want[1] - 1) < 1e-5
 
 
-def test_area_volume_closed_form():
-    """Area and volume of an icosahedron of circumradius 1, translated away
-    from the origin (the volume formula is origin independent for a closed
-    oriented surface)."""
+def icosahedron():
+    """unit-circumradius icosahedron, outward-oriented faces"""
     t = (1 + 5 ** 0.5) / 2
     v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
                   [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
@@ -56,6 +54,14 @@ def test_area_volume_closed_form():
     f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
                   [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
                   [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int32)
+    return v, f
+
+
+def test_area_volume_closed_form():
+    """Area and volume of an icosahedron of circumradius 1, translated away
+    from the origin (the volume formula is origin independent for a closed
+    oriented surface)."""
+    v, f = icosahedron()
     m = R.Mesh(v.astype(np.float32), f)
     pp = np.zeros((12, 6), np.float32)
     pp[:, :3] = (v + [3.0, -2.0, 1.5]).astype(np.float32)
@@ -304,3 +310,73 @@ def test_fsi_seeds_trunk():
     import oracle as O
     s = R.fsi_seeds(3)
     assert np.array_equal(s.ravel(), O.kiss_fill(1908, 1409, 290, 12968, 6))
+
+
+# ------------------------------------------------------------------ adjoint
+COEF = ("ka", "kd", "kv", "ks", "kb")
+
+
+def adj_setup(nc=2, seed=1, stress_free=False, ico=False):
+    """noisy 642-vertex cells (sigma 0.05) off their targets; ico: a scaled
+    icosahedron instead (all dihedrals far from flat)"""
+    if ico:
+        v, f = icosahedron()
+        m = R.Mesh((2.0 * v).astype(np.float32), f)
+    else:
+        m = mesh(642)
+    rng = np.random.default_rng(seed)
+    T = [np.eye(4) for _ in range(nc)]
+    for k, A in enumerate(T):
+        A[:3, 3] = [3.0 * k, 0.0, 0.0]
+    pp = R.place(m.rr, T, (0, 0, 0))
+    pp[:, :3] += rng.normal(0, 0.05, (len(pp), 3)).astype(np.float32)
+    tmpl = np.zeros((m.nv, 6), np.float32)
+    tmpl[:, :3] = m.rr
+    A0, V0 = R.area_volume(m, tmpl, 1)
+    kw = dict(R.RBC_TEST, gammaC=0.0, gammaT=0.0, totArea=np.float32(A0 * 0.97), totVolume=np.float32(V0 * 1.03))
+    sf = m.shape() if stress_free else None
+    l0, a0 = m.stressful(kw["totArea"])
+    g = rng.normal(0, 1, (len(pp), 3))
+    return m, pp, kw, sf, l0, a0, g
+
+
+@pytest.mark.parametrize("stress_free", [False, True])
+def test_adjoint_terms_sum_to_the_force(stress_free):
+    m, pp, kw, sf, l0, a0, g = adj_setup(stress_free=stress_free)
+    p = R.par(**kw)
+    gx, terms, gp = R.adjoint(p, m, pp, 2, g, sf, l0, a0)
+    F = R.forces(p, m, pp, 2, R.area_volume(m, pp, 2), stress_free=sf, l0=l0, a0=a0)
+    tot = sum(kw[c] * terms[k] for k, c in enumerate(COEF))
+    # the force rounds the spring length and powf to float, the adjoint does
+    # not: 2e-6 of the largest (spring-dominated) force at most
+    assert np.abs(tot - F).max() <= 1e-5 * np.abs(F).max()
+    assert np.allclose(gp, [(g * terms[k]).sum() for k in range(5)], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("term", range(5))
+def test_adjoint_is_the_force_jacobian(term):
+    """gx = (dF/dx) g for each term alone: central differences of the force
+    along g (h = 1e-3 on float positions: 2e-3 of the largest component),
+    and g . gx = -d^2 E / d eps^2 along g from the FP64 energy (1e-5).  The
+    bending term is checked on an icosahedron: the RBC meshes have nearly
+    flat dihedrals where the reference's 1/sin(theta) (clamped at 1e-3 and
+    signed) is not differentiable."""
+    m, pp, kw, sf, l0, a0, g = adj_setup(seed=3 + term, ico=term == 4)
+    base = dict(kw, ka=0.0, kd=0.0, kv=0.0, ks=0.0, kb=0.0)
+    base[COEF[term]] = kw[COEF[term]]
+    p = R.par(**base)
+    gx, _, _ = R.adjoint(p, m, pp, 2, g, sf, l0, a0)
+    h = 1e-3
+
+    def force(x):
+        q = pp.copy()
+        q[:, :3] = x.astype(np.float32)
+        return R.forces(p, m, q, 2, R.area_volume(m, q, 2), stress_free=sf, l0=l0, a0=a0)
+    x0 = pp[:, :3].astype(np.float64)
+    fd = (force(x0 + h * g) - force(x0 - h * g)) / (2 * h)
+    assert np.abs(fd - gx).max() <= 2e-3 * np.abs(gx).max()
+    bit = (1, 4, 2, 8, 16)[term]          # energy term bits of orc_rbc_energy
+    e = lambda x: R.energy(p, m, x, 2, bit, sf, l0, a0)
+    hh = 1e-4
+    d2 = (e(x0 + hh * g) - 2 * e(x0) + e(x0 - hh * g)) / (hh * hh)
+    assert abs((g * gx).sum() + d2) <= 1e-5 * abs(d2) + 1e-8 * np.abs(g * gx).sum()
