@@ -198,6 +198,186 @@ __global__ void k_rbc_stretch(int n, int nv, const float *f, float *ff) {
     o[2] += f[3 * k + 2];
 }
 
+// ---------------------------------------------------------------- adjoint
+// F = -grad E with E = ka E_A + kd E_a + kv E_V + ks E_s + kb E_b, so dF/dx
+// is symmetric and the vector-Jacobian product with a cotangent g is the
+// derivative of F along g.  One thread per vertex re-evaluates its ring's
+// triangle and dihedral terms (nothing is stored from the forward pass) in
+// forward-mode dual numbers with tangent g, with the cell's area and volume
+// tangents from k_rbc_av_dual; the unit-coefficient term values give
+// dL/dk = g . F_k.  The viscous term is excluded (zero relative velocities)
+// and the spring length is the double root.
+struct Dd { double v, d; };
+__device__ __forceinline__ Dd dc(double v) { return Dd{v, 0.0}; }
+__device__ __forceinline__ Dd operator+(Dd a, Dd b) { return Dd{a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dd operator-(Dd a, Dd b) { return Dd{a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dd operator*(Dd a, Dd b) { return Dd{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dd operator/(Dd a, Dd b) { return Dd{a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+__device__ __forceinline__ Dd dsqrt(Dd a) {
+    const double s = sqrt(a.v);
+    return Dd{s, a.d / (2 * s)};
+}
+__device__ __forceinline__ Dd dpowc(Dd a, double c) {
+    const double p = pow(a.v, c);
+    return Dd{p, c * p / a.v * a.d};
+}
+struct V3 { Dd x, y, z; };
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator*(Dd s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ Dd vdot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ V3 vcross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 vzero() { return V3{dc(0), dc(0), dc(0)}; }
+__device__ __forceinline__ V3 dpos(const float *pp, const float *g, int i) {
+    const float *p = pp + 6 * (long)i, *t = g + 3 * (long)i;
+    return V3{Dd{p[0], t[0]}, Dd{p[1], t[1]}, Dd{p[2], t[2]}};
+}
+
+// per cell [A, dA, V, dV]
+__global__ __launch_bounds__(AV_THREADS) void k_rbc_av_dual(int nt, int nv, const int *tt, const float *pp,
+                                                            const float *g, double *avd) {
+    __shared__ double s[4][AV_THREADS / 64];
+    const int cell = blockIdx.x, off = cell * nv;
+    Dd a = dc(0), vol = dc(0);
+    for (int t = threadIdx.x; t < nt; t += AV_THREADS) {
+        const V3 r0 = dpos(pp, g, tt[3 * t] + off), r1 = dpos(pp, g, tt[3 * t + 1] + off),
+                 r2 = dpos(pp, g, tt[3 * t + 2] + off);
+        const V3 n = vcross(r1 - r0, r2 - r0);
+        a = a + dc(0.5) * dsqrt(vdot(n, n));
+        vol = vol + vdot(r0, vcross(r1, r2)) / dc(6.0);
+    }
+    double q[4] = {a.v, a.d, vol.v, vol.d};
+    for (int k = 0; k < 4; k++)
+        for (int d = 32; d >= 1; d >>= 1) q[k] += __shfl_xor(q[k], d, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 4; k++) s[k][threadIdx.x >> 6] = q[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0;
+        for (int w = 0; w < AV_THREADS / 64; w++) t += s[threadIdx.x][w];
+        avd[4 * cell + threadIdx.x] = t;
+    }
+}
+
+struct AdjK {
+    int md, nv, nc;
+    const int *adj0, *adj1;
+    const float *ll, *aa;
+    float l0, a0;
+    double coef[5];           // ka, kd, kv, ks, kb
+    double totArea, totVolume, x0, mpow, sphi, cphi;
+};
+
+__device__ __forceinline__ int adj_map_a(const AdjK &K, int v, int j, int *i1, int *i2, int *i3, int *i4) {
+    const int md = K.md, k = (v % K.nv) * md;
+    *i1 = K.adj0[k + j];
+    if (*i1 == -1) return 0;
+    int b = K.adj0[k + (j + 1) % md], c;
+    if (b == -1) {
+        b = K.adj0[k];
+        c = K.adj0[k + 1];
+    } else {
+        c = K.adj0[k + (j + 2) % md];
+        if (c == -1) c = K.adj0[k];
+    }
+    *i2 = b;
+    *i3 = c;
+    *i4 = K.adj1[k + j];
+    return 1;
+}
+
+__device__ void tri_terms(const AdjK &K, V3 r1, V3 r2, V3 r3, double l0, double a0, Dd A, Dd V, V3 out[4]) {
+    const V3 x21 = r2 - r1, x32 = r3 - r2, x31 = r3 - r1;
+    const V3 nn = vcross(x21, x31), f = vcross(nn, x32);
+    const Dd a = dc(0.5) * dsqrt(vdot(nn, nn));
+    const Dd fA = (dc(-1.0) * (A - dc(K.totArea))) / (dc(4 * K.totArea) * a);
+    const Dd fa = (dc(-1.0) * (a - dc(a0))) / (dc(4 * a0) * a);
+    out[0] = fA * f;
+    out[1] = fa * f;
+    out[2] = ((V - dc(K.totVolume)) / dc(6 * K.totVolume)) * vcross(r3, r2);
+    const double lmax = l0 / K.x0, m = K.mpow;
+    Dd r = dsqrt(vdot(x21, x21));
+    if (r.v >= lmax) r = dc(0.99 * lmax);
+    const Dd xr = r / dc(lmax);
+    const Dd w = ((dc(4) * (xr * xr) - dc(9) * xr) + dc(6)) / (dc(4) * ((xr - dc(1)) * (xr - dc(1))));
+    const Dd fwlc = dc(1.0 / lmax) * w;
+    const double xl = l0 / lmax, wl = (4 * xl * xl - 9 * xl + 6) / (4 * (xl - 1) * (xl - 1)) / lmax;
+    const Dd fpow = dc(-wl * pow(l0, m + 1)) / dpowc(r, m + 1);
+    out[3] = (fwlc + fpow) * x21;
+}
+
+template <int WHICH>
+__device__ V3 dih_dual(const AdjK &K, V3 a, V3 b, V3 c, V3 d) {
+    const V3 ab = a - b, ac = a - c, cd = c - d, bd = b - d, bc = b - c, da = d - a;
+    const V3 k = vcross(ab, ac), n = vcross(bd, bc);
+    const Dd over_k = dc(1) / dsqrt(vdot(k, k)), over_n = dc(1) / dsqrt(vdot(n, n));
+    const Dd cost = (vdot(k, n) * over_k) * over_n;
+    const Dd s2 = dc(1) - cost * cost;
+    Dd over_sin = dc(1) / dsqrt(s2.v > 1e-6 ? s2 : dc(1e-6));
+    if (vdot(k - n, da).v < 0) over_sin = Dd{-over_sin.v, -over_sin.d};
+    const Dd beta = dc(K.cphi) - (cost * dc(K.sphi)) * over_sin;
+    const Dd b11 = ((dc(-1) * beta) * cost) * (over_k * over_k), b12 = beta * (over_k * over_n);
+    if (WHICH == 1) {
+        const V3 r32 = c - b;
+        return b11 * vcross(k, r32) + b12 * vcross(n, r32);
+    }
+    const Dd b22 = ((dc(-1) * beta) * cost) * (over_n * over_n);
+    V3 f = b11 * vcross(k, ac);
+    f = f + b12 * (vcross(k, cd) + vcross(n, ac));
+    return f + b22 * vcross(n, cd);
+}
+
+__global__ __launch_bounds__(RBC_THREADS) void k_rbc_adjoint(AdjK K, const float *pp, const float *g,
+                                                             const double *avd, float *gx, double *gpv) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= K.nc * K.nv) return;
+    const int cell = v / K.nv, off = cell * K.nv;
+    const Dd A{avd[4 * cell], avd[4 * cell + 1]}, Vol{avd[4 * cell + 2], avd[4 * cell + 3]};
+    const V3 r0 = dpos(pp, g, v);
+    V3 t[5] = {vzero(), vzero(), vzero(), vzero(), vzero()};
+    for (int j = 0; j < K.md; j++) {
+        int i1, i2, i3, i4;
+        if (!adj_map_a(K, v, j, &i1, &i2, &i3, &i4)) continue;
+        const int s = (v % K.nv) * K.md + j;
+        const double l0 = K.ll ? (double)K.ll[s] : (double)K.l0, a0 = K.aa ? (double)K.aa[s] : (double)K.a0;
+        const V3 r1 = dpos(pp, g, i1 + off), r2 = dpos(pp, g, i2 + off);
+        V3 f4[4];
+        tri_terms(K, r0, r1, r2, l0, a0, A, Vol, f4);
+        for (int k = 0; k < 4; k++) t[k] = t[k] + f4[k];
+        const V3 r3 = dpos(pp, g, i3 + off), r4 = dpos(pp, g, i4 + off);
+        t[4] = t[4] + dih_dual<1>(K, r0, r2, r1, r4);
+        t[4] = t[4] + dih_dual<2>(K, r1, r0, r2, r3);
+    }
+    const float *gv = g + 3 * (long)v;
+    double dx = 0, dy = 0, dz = 0;
+    for (int k = 0; k < 5; k++) {
+        dx += K.coef[k] * t[k].x.d;
+        dy += K.coef[k] * t[k].y.d;
+        dz += K.coef[k] * t[k].z.d;
+        gpv[5 * (long)v + k] = ((double)gv[0] * t[k].x.v + (double)gv[1] * t[k].y.v) + (double)gv[2] * t[k].z.v;
+    }
+    float *o = gx + 3 * (long)v;
+    o[0] = (float)dx;
+    o[1] = (float)dy;
+    o[2] = (float)dz;
+}
+
+// gp[k] = sum over vertices of gpv[5 v + k]: one block, fixed order
+__global__ __launch_bounds__(256) void k_rbc_gp(int n, const double *gpv, double *gp) {
+    __shared__ double s[5][4];
+    double q[5] = {0, 0, 0, 0, 0};
+    for (int v = threadIdx.x; v < n; v += 256)
+        for (int k = 0; k < 5; k++) q[k] += gpv[5 * (long)v + k];
+    for (int k = 0; k < 5; k++)
+        for (int d = 32; d >= 1; d >>= 1) q[k] += __shfl_xor(q[k], d, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; k++) s[k][threadIdx.x >> 6] = q[k];
+    __syncthreads();
+    if (threadIdx.x < 5) gp[threadIdx.x] = s[threadIdx.x][0] + s[threadIdx.x][1] + s[threadIdx.x][2] + s[threadIdx.x][3];
+}
+
 // ---------------------------------------------------------------- host
 // ring adjacency of every vertex (src/rbc/adj/imp/ini.h:1-45): the
 // neighbours in face order starting from the smallest, and the vertex across
@@ -260,6 +440,7 @@ struct udx_rbc {
     float l0, a0;
     int stress_free, stressful_set, bending;
     double *av;                  // device, 2 * max_cells
+    double *avd, *gpv, *gp;      // adjoint scratch: 4 * max_cells, 5 * max_cells * nv, 5
 };
 
 extern "C" {
@@ -272,6 +453,9 @@ int udx_rbc_fin(udx_rbc *r) {
     dev_free(r->ll);
     dev_free(r->aa);
     dev_free(r->av);
+    dev_free(r->avd);
+    dev_free(r->gpv);
+    dev_free(r->gp);
     delete r;
     return UDX_OK;
 }
@@ -429,6 +613,56 @@ int udx_rbc_force_apply(udx_rbc *r, const udx_rbc_params *p, float dt, int nc, c
     hipLaunchKernelGGL(k_rbc_force, dim3(ceil_div(n, RBC_THREADS)), dim3(RBC_THREADS), 0, r->ctx->stream, K, pp,
                        r->av, ff);
     UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const float *pp, const float *g,
+                          float *gx, double gp[5]) {
+    UDX_CHECK(r && p && pp && g && gx && gp, UDX_EINVAL, "udx_rbc_force_adjoint: null argument");
+    UDX_CHECK(r->stress_free || r->stressful_set, UDX_ESTATE,
+              "udx_rbc_force_adjoint: call udx_rbc_set_stressful or udx_rbc_set_stressfree first");
+    UDX_CHECK(nc >= 0 && nc <= r->max_cells, UDX_ECAP, "udx_rbc_force_adjoint: nc = %d above max_cells = %d", nc,
+              r->max_cells);
+    for (int k = 0; k < 5; k++) gp[k] = 0;
+    if (nc == 0) return UDX_OK;
+    hipStream_t s = r->ctx->stream;
+    if (!r->avd) {
+        UDX_TRY(dev_alloc(&r->avd, 4 * (size_t)r->max_cells));
+        UDX_TRY(dev_alloc(&r->gpv, 5 * (size_t)r->max_cells * r->nv));
+        UDX_TRY(dev_alloc(&r->gp, 5));
+    }
+    hipLaunchKernelGGL(k_rbc_av_dual, dim3(nc), dim3(AV_THREADS), 0, s, r->nt, r->nv, r->tt, pp, g, r->avd);
+    UDX_LAUNCHED();
+    AdjK K;
+    K.md = r->md;
+    K.nv = r->nv;
+    K.nc = nc;
+    K.adj0 = r->adj0;
+    K.adj1 = r->adj1;
+    K.ll = r->stress_free ? r->ll : nullptr;
+    K.aa = r->stress_free ? r->aa : nullptr;
+    K.l0 = r->l0;
+    K.a0 = r->a0;
+    K.coef[0] = p->ka;
+    K.coef[1] = p->kd;
+    K.coef[2] = p->kv;
+    K.coef[3] = p->ks;
+    K.coef[4] = r->bending ? p->kb : 0.0;
+    K.totArea = p->totArea;
+    K.totVolume = p->totVolume;
+    K.x0 = p->x0;
+    K.mpow = p->mpow;
+    const double phi = (double)p->phi / 180.0 * 3.14159265358979323846;
+    K.sphi = std::sin(phi);
+    K.cphi = std::cos(phi);
+    const int n = nc * r->nv;
+    hipLaunchKernelGGL(k_rbc_adjoint, dim3(ceil_div(n, RBC_THREADS)), dim3(RBC_THREADS), 0, s, K, pp, g, r->avd,
+                       gx, r->gpv);
+    UDX_LAUNCHED();
+    hipLaunchKernelGGL(k_rbc_gp, dim3(1), dim3(256), 0, s, n, r->gpv, r->gp);
+    UDX_LAUNCHED();
+    UDX_HIP(hipMemcpyAsync(gp, r->gp, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+    UDX_HIP(hipStreamSynchronize(s));
     return UDX_OK;
 }
 

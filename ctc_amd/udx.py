@@ -107,6 +107,7 @@ _SIGS = {
     "udx_rbc_av": (_VP, [_VP]),
     "udx_rbc_force_apply": (_I, [_VP, C.POINTER(RbcParams), _F, _I, _VP, _VP]),
     "udx_rbc_stretch_apply": (_I, [_VP, _I, _I, _VP, _VP]),
+    "udx_rbc_force_adjoint": (_I, [_VP, C.POINTER(RbcParams), _I, _VP, _VP, _VP, C.POINTER(C.c_double)]),
     "udx_fsi_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
     "udx_fsi_fin": (_I, [_VP]),
     "udx_fsi_set_mode": (_I, [_VP, _I]),
@@ -572,6 +573,12 @@ class Rbc:
 
     def force(self, params, dt, nc, pp, ff):
         _ok(lib().udx_rbc_force_apply(self.h, C.byref(params), float(dt), nc, _ptr(pp), _ptr(ff)))
+
+    def adjoint(self, params, nc, pp, g, gx):
+        """gx = (dF/dx)^T g; returns [dL/dka, dL/dkd, dL/dkv, dL/dks, dL/dkb]"""
+        gp = (C.c_double * 5)()
+        _ok(lib().udx_rbc_force_adjoint(self.h, C.byref(params), nc, _ptr(pp), _ptr(g), _ptr(gx), gp))
+        return list(gp)
 
     def close(self):
         if self.h:

@@ -212,6 +212,15 @@ const double *udx_rbc_av(const udx_rbc *r);                /* device, 2 * max_ce
 /* area/volume, bending and membrane forces ADDED to ff (nc * nv Forces);
  * the membrane's thermal noise (rnd = true) is not supported */
 int udx_rbc_force_apply(udx_rbc *r, const udx_rbc_params *p, float dt, int nc, const float *pp, float *ff);
+/* Membrane adjoint.  No counterpart in the reference; for calibrating the
+ * membrane coefficients.  For a cotangent g (device, 3 floats per vertex) of
+ * the conservative membrane force F (area, volume, springs, bending; the
+ * viscous term is left out, i.e. zero relative velocities), writes
+ * gx = (dF/dx)^T g (device, 3 floats per vertex) and gp = g . dF/dk for
+ * k = ka, kd, kv, ks, kb (host, summed over all cells).  Nothing is kept
+ * from the forward call: the geometry is recomputed. */
+int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const float *pp, const float *g,
+                          float *gx, double gp[5]);
 /* ff[i] += f[i % nv] for nc * nv vertices (f: device, 3 * nv) */
 int udx_rbc_stretch_apply(udx_ctx *ctx, int nc, int nv, const float *f, float *ff);
 

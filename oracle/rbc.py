@@ -40,7 +40,7 @@ def _lib():
             "orc_rbc_stretch": (None, [C.c_int, C.c_int, F32P, F32P]),
             "orc_fsi_forces": (C.c_long, [I32P, C.c_int, F32P, I32P, C.c_int, F32P, Dpd, F32P, F32P, F32P]),
             "orc_fsi_seeds": (None, [C.c_int, F32P]),
-            "orc_rbc_adjoint": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, I32P, F32P, F32P, F32P,
+            "orc_rbc_adjoint": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, I32P, F64P, F32P, F32P,
                                        C.c_float, C.c_float, F64P, F64P, F64P, F64P]),
         }
         for name, (res, args) in sig.items():
@@ -197,8 +197,10 @@ def fsi_seeds(nsteps):
 
 def adjoint(p, mesh, pp, nc, g, stress_free=None, l0=0.0, a0=0.0):
     """(gx, terms (5, n, 3), gp (5,)): the membrane force's vector-Jacobian
-    product with g, the unit-coefficient term forces and g . term"""
-    pp = _f32(pp).reshape(-1, 6)
+    product with g, the unit-coefficient term forces and g . term; pp is
+    (n, 6) particles or (n, 3) positions, taken in double"""
+    pp = np.asarray(pp)
+    x = _f64(pp.reshape(len(pp), -1)[:, :3])
     g = _f64(g).reshape(-1, 3)
     n = nc * mesh.nv
     gx = np.zeros((n, 3), np.float64)
@@ -208,7 +210,7 @@ def adjoint(p, mesh, pp, nc, g, stress_free=None, l0=0.0, a0=0.0):
     if stress_free is not None:
         ll, aa = (_f32(x) for x in stress_free)
     _lib().orc_rbc_adjoint(C.byref(p), mesh.md, mesh.nv, nc, mesh.nt, _ptr(mesh.tt, I32P), _ptr(mesh.adj0, I32P),
-                           _ptr(mesh.adj1, I32P), _ptr(pp, F32P), _ptr(ll, F32P) if ll is not None else None,
+                           _ptr(mesh.adj1, I32P), _ptr(x, F64P), _ptr(ll, F32P) if ll is not None else None,
                            _ptr(aa, F32P) if aa is not None else None, np.float32(l0), np.float32(a0),
                            _ptr(g, F64P), _ptr(gx, F64P), _ptr(terms, F64P), _ptr(gp, F64P))
     return gx, terms, gp

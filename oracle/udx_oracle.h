@@ -169,9 +169,10 @@ void   orc_rbc_stretch(int nc, int nv, const float *f, float *ff);     /* src/rb
 /* membrane adjoint (udx_oracle_adj.c): for a cotangent g (3 per vertex),
  * gx = (dF/dx)^T g = (dF/dx) g (F = -grad E, symmetric Jacobian) of the
  * conservative membrane force, terms[k] the unit-coefficient term forces
- * (k: ka, kd, kv, ks, kb; NULL to skip) and gp[k] = g . terms[k] = dL/dk */
+ * (k: ka, kd, kv, ks, kb; NULL to skip) and gp[k] = g . terms[k] = dL/dk,
+ * at double positions x (3 per vertex) */
 void   orc_rbc_adjoint(const orc_rbc_par *p, int md, int nv, int nc, int nt, const int *tt, const int *adj0,
-                       const int *adj1, const float *pp, const float *ll, const float *aa, float l0, float a0,
+                       const int *adj1, const double *x, const float *ll, const float *aa, float l0, float a0,
                        const double *g, double *gx, double *terms, double gp[5]);
 
 /* ---- solvent-membrane interactions (udx_oracle_fsi.c) ------------------- */

@@ -13,7 +13,8 @@
  * unit-coefficient term forces F_k whose dot products with g are
  * dL/dk = g . F_k.  The viscous term is not part of it (zero relative
  * velocities); the spring length is the double root (the production force
- * rounds it to float).
+ * rounds it to float).  Positions are double (3 per vertex), so with g = 0
+ * the term forces are also an FP64 evaluation of the membrane force.
  */
 #include "udx_oracle.h"
 
@@ -44,12 +45,12 @@ static v3 vcross(v3 a, v3 b) {
 }
 static v3 vzero(void) { return V(C(0), C(0), C(0)); }
 
-static v3 dpos(const float *pp, const double *g, int i) {
-    return V(D(pp[6 * i], g[3 * i]), D(pp[6 * i + 1], g[3 * i + 1]), D(pp[6 * i + 2], g[3 * i + 2]));
+static v3 dpos(const double *x, const double *g, int i) {
+    return V(D(x[3 * i], g[3 * i]), D(x[3 * i + 1], g[3 * i + 1]), D(x[3 * i + 2], g[3 * i + 2]));
 }
 
 /* per-cell area and volume with their tangents */
-static void av_dual(int nt, int nv, int c, const int *tt, const float *pp, const double *g, dd *A, dd *Vol) {
+static void av_dual(int nt, int nv, int c, const int *tt, const double *pp, const double *g, dd *A, dd *Vol) {
     dd a = C(0), vol = C(0);
     for (int t = 0; t < nt; t++) {
         v3 r0 = dpos(pp, g, tt[3 * t] + c * nv), r1 = dpos(pp, g, tt[3 * t + 1] + c * nv),
@@ -109,7 +110,7 @@ static v3 dih_dual(int which, double sphi, double cphi, v3 a, v3 b, v3 c, v3 d) 
 }
 
 void orc_rbc_adjoint(const orc_rbc_par *p, int md, int nv, int nc, int nt, const int *tt, const int *adj0,
-                     const int *adj1, const float *pp, const float *ll, const float *aa, float l0, float a0,
+                     const int *adj1, const double *pp, const float *ll, const float *aa, float l0, float a0,
                      const double *g, double *gx, double *terms, double gp[5]) {
     int m[6];
     const double coef[5] = {p->ka, p->kd, p->kv, p->ks, p->kb};

@@ -16,12 +16,7 @@ import numpy as np
 import pytest
 
 from oracle import rbc as R
-
-GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
-
-
-def mesh(nv):
-    return R.Mesh(*R.read_off(os.path.join(GOLD, "cells", "%d.off" % nv)))
+from membranes import C4, COEF, GOLD, adj_setup, icosahedron, mesh
 
 
 def two_cells(m, L=(16, 16, 16)):
@@ -43,18 +38,6 @@ def test_area_volume_golden():
     want = [float(x) for x in open(os.path.join(GOLD, "av.ref.rbc.u.area_volume.dev.0.txt")).read().split()]
     assert "%g" % A == "%g" % want[0] and abs(V - want[1]) < 0.05
     assert abs(A - want[0]) < 1e-3 and abs(V / want[1] - 1) < 1e-5
-
-
-def icosahedron():
-    """unit-circumradius icosahedron, outward-oriented faces"""
-    t = (1 + 5 ** 0.5) / 2
-    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
-                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
-    v /= np.linalg.norm(v[0])
-    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
-                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
-                  [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int32)
-    return v, f
 
 
 def test_area_volume_closed_form():
@@ -313,33 +296,6 @@ def test_fsi_seeds_trunk():
 
 
 # ------------------------------------------------------------------ adjoint
-COEF = ("ka", "kd", "kv", "ks", "kb")
-
-
-def adj_setup(nc=2, seed=1, stress_free=False, ico=False):
-    """noisy 642-vertex cells (sigma 0.05) off their targets; ico: a scaled
-    icosahedron instead (all dihedrals far from flat)"""
-    if ico:
-        v, f = icosahedron()
-        m = R.Mesh((2.0 * v).astype(np.float32), f)
-    else:
-        m = mesh(642)
-    rng = np.random.default_rng(seed)
-    T = [np.eye(4) for _ in range(nc)]
-    for k, A in enumerate(T):
-        A[:3, 3] = [3.0 * k, 0.0, 0.0]
-    pp = R.place(m.rr, T, (0, 0, 0))
-    pp[:, :3] += rng.normal(0, 0.05, (len(pp), 3)).astype(np.float32)
-    tmpl = np.zeros((m.nv, 6), np.float32)
-    tmpl[:, :3] = m.rr
-    A0, V0 = R.area_volume(m, tmpl, 1)
-    kw = dict(R.RBC_TEST, gammaC=0.0, gammaT=0.0, totArea=np.float32(A0 * 0.97), totVolume=np.float32(V0 * 1.03))
-    sf = m.shape() if stress_free else None
-    l0, a0 = m.stressful(kw["totArea"])
-    g = rng.normal(0, 1, (len(pp), 3))
-    return m, pp, kw, sf, l0, a0, g
-
-
 @pytest.mark.parametrize("stress_free", [False, True])
 def test_adjoint_terms_sum_to_the_force(stress_free):
     m, pp, kw, sf, l0, a0, g = adj_setup(stress_free=stress_free)
@@ -380,3 +336,27 @@ def test_adjoint_is_the_force_jacobian(term):
     hh = 1e-4
     d2 = (e(x0 + hh * g) - 2 * e(x0) + e(x0 - hh * g)) / (hh * hh)
     assert abs((g * gx).sum() + d2) <= 1e-5 * abs(d2) + 1e-8 * np.abs(g * gx).sum()
+
+
+def test_c4_fp64_gradient_is_the_central_difference():
+    """The FP64 CPU gradient of the membrane calibration loss (4 cells, 4
+    relaxation steps) through the oracle adjoint against central differences
+    with h = 1e-3: along a random direction of the initial positions (spontaneous
+    angle 0, where the bending energy is smooth: with the reference's 6.97
+    degrees its 1/sin(theta) flips sign at flat dihedrals and differences
+    across one are meaningless) within 1e-5, and for each coefficient
+    (h = 1e-3 of it, the reference's angle) within 1e-5."""
+    c = C4(nc=4, seed=2, phi=0.0)
+    _, gx, _ = c.grad64()
+    h = 1e-3
+    fd = (c.loss64(x0=c.x0 + h * c.u)[0] - c.loss64(x0=c.x0 - h * c.u)[0]) / (2 * h)
+    assert abs(fd - (gx * c.u).sum()) <= 1e-5 * abs(fd)
+    c = C4(nc=4, seed=2)
+    _, _, gk = c.grad64()
+    k = c.coef()
+    for j in range(5):
+        kp, km = k.copy(), k.copy()
+        kp[j] += h * k[j]
+        km[j] -= h * k[j]
+        fdk = (c.loss64(coef=kp)[0] - c.loss64(coef=km)[0]) / (2 * h * k[j])
+        assert abs(fdk - gk[j]) <= 1e-5 * abs(gk[j]), COEF[j]
