@@ -8,7 +8,9 @@ forces.ref.rbc.u.{simple,stree.free} are its src/test_data files for
 src/test/rbc/u/{area_volume/dev,simple,stress.free}, run with
 conf/rbc.test.cfg (tests/golden/ref/conf/rbc.test.cfg), the initial condition
 "two" (identity and a shift by (1, 2, 3)) in a 16^3 box, positions and forces
-printed %g after scaling by 0.1 (simple) or 0.025 (stress free).
+printed %g after scaling by 0.1 (simple) or 0.025 (stress free);
+eng.ref.mesh.eng_kantor is its src/test/mesh/eng_kantor output on
+src/data/cells/rbc/sc/2562.off (tests/golden/ref/cells/sc2562.off).
 """
 import os
 
@@ -293,6 +295,36 @@ def test_fsi_seeds_trunk():
     import oracle as O
     s = R.fsi_seeds(3)
     assert np.array_equal(s.ravel(), O.kiss_fill(1908, 1409, 290, 12968, 6))
+
+
+
+def test_kantor_energy_golden():
+    """eng.ref.mesh.eng_kantor: the reference's Kantor energy utility
+    (src/u/mesh/eng_kantor/main.cpp, src/mesh/eng_kantor/imp/main.h:30-39)
+    on its 2562-vertex stress-free RBC mesh with kb = 1 and angle 0 prints,
+    per vertex, the cylindrical coordinates and the mean of the energies
+    2 kb (1 - cos theta) of the vertex's edges (src/mesh/scatter/imp/main.h).
+    Sum over vertices of mean times degree = twice the total edge energy.  The
+    oracle's bending energy kb (1 - cos theta) per edge, the potential whose
+    negative gradient the Kantor force is (checked by central differences),
+    is half of the utility's per-edge value: the totals differ by exactly 2
+    (to the 6 printed digits)."""
+    rr, tt = R.read_off(os.path.join(GOLD, "cells", "sc2562.off"))
+    m = R.Mesh(rr, tt)
+    ref = np.loadtxt(os.path.join(GOLD, "eng.ref.mesh.eng_kantor.txt"))
+    assert ref.shape == (m.nv, 4)
+    assert np.abs(np.hypot(rr[:, 0], rr[:, 1]) - ref[:, 0]).max() < 1e-4      # same vertex order
+    assert np.abs(rr[:, 2] - ref[:, 2]).max() < 1e-5
+    edges = {(min(u, v), max(u, v)) for a, b, c in tt for u, v in ((a, b), (b, c), (c, a))}
+    assert len(edges) == 3 * m.nv - 6                                        # closed genus-0 mesh
+    deg = np.zeros(m.nv)
+    for u, v in edges:
+        deg[u] += 1
+        deg[v] += 1
+    utility = (ref[:, 3] * deg).sum() / 2
+    p = R.par(kb=1.0, phi=0.0, ka=0.0, kd=0.0, kv=0.0, ks=0.0, x0=0.5, mpow=2.0, totArea=1.0, totVolume=1.0)
+    E = R.energy(p, m, rr.astype(np.float64), 1, 16)
+    assert abs(utility / (2 * E) - 1) < 1e-6
 
 
 # ------------------------------------------------------------------ adjoint
