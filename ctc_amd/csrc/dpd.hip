@@ -27,6 +27,7 @@ namespace {
 constexpr int DPD_THREADS = 256;
 constexpr int QCAP = 24;   // queued pairs per lane before a drain
 constexpr int SCAP = 160;  // staged candidates per wave and row
+constexpr int DRAIN_STEPS = 2;  // pair steps per lane when a queue is full
 
 struct DpdK {
     const float4 *pos, *vel;
@@ -153,40 +154,57 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     }
     const float seed_bulk = A.seeds[26];
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    int npair = 0, fill = 0;
+    int npair = 0, fill = 0, head = 0;
     unsigned *qlane = q + tid;
 
-    // Drain the per-lane queues in queue order (= candidate order), two
-    // pairs per lane at a time so two noise chains share each
-    // round-toward-zero window.
-    // an idle slot evaluates a discarded pair with an in-bounds particle
-    // (i itself, or particle 0 for a lane past the end)
+    // Each lane's queue is a ring of QCAP entries (head, fill) consumed in
+    // queue order (= candidate order), two pairs per lane per step so two
+    // noise chains share each round-toward-zero window.  When a lane runs
+    // out of room the wave takes DRAIN_STEPS steps only: lanes fill their
+    // queues at very different rates (a lane near the -z face of its cell
+    // hits most in the dz = -1 rows), and emptying every queue at that point
+    // would run the expensive pair body with most lanes idle.
+    // An idle slot evaluates a discarded pair with an in-bounds particle
+    // (i itself, or the box's first particle for a lane past the end).
     const unsigned idle = (unsigned)(active ? i : A.i0) | (W_NONE << 27);
+    auto ring = [&](int k) {
+        const int s = head + k;
+        return s >= QCAP ? s - QCAP : s;
+    };
+    auto drain_step = [&]() {
+        const bool va = fill > 0, vb = fill > 1;
+        const unsigned ea = va ? qlane[ring(0) * DPD_THREADS] : idle;
+        const unsigned eb = vb ? qlane[ring(1) * DPD_THREADS] : idle;
+        PairIn a = pair_in(A, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+        PairIn b = pair_in(A, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+        float xa = a.x, xb = b.x;
+        logistic18x2(&xa, &xb);
+        const float3 f1 = dpd_pair<WR, EXACT>(A.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+        const float3 f2 = dpd_pair<WR, EXACT>(A.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+        fx += va ? f1.x : 0.f;
+        fy += va ? f1.y : 0.f;
+        fz += va ? f1.z : 0.f;
+        fx += vb ? f2.x : 0.f;
+        fy += vb ? f2.y : 0.f;
+        fz += vb ? f2.z : 0.f;
+        const int took = fill < 2 ? fill : 2;
+        head = ring(took);
+        fill -= took;
+        npair += took;
+    };
+    // room for `need` more entries in every lane (need <= 2 DRAIN_STEPS)
+    auto make_room = [&](int need) {
+        if (wave_any(fill + need > QCAP))
+#pragma unroll 1
+            for (int k = 0; k < DRAIN_STEPS; k++) drain_step();
+    };
     auto drain = [&]() {
-        for (int e = 0; wave_any(e < fill); e += 2) {
-            const bool va = e < fill, vb = e + 1 < fill;
-            const unsigned ea = va ? qlane[e * DPD_THREADS] : idle;
-            const unsigned eb = vb ? qlane[(e + 1) * DPD_THREADS] : idle;
-            PairIn a = pair_in(A, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
-            PairIn b = pair_in(A, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
-            float xa = a.x, xb = b.x;
-            logistic18x2(&xa, &xb);
-            const float3 f1 = dpd_pair<WR, EXACT>(A.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
-            const float3 f2 = dpd_pair<WR, EXACT>(A.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
-            fx += va ? f1.x : 0.f;
-            fy += va ? f1.y : 0.f;
-            fz += va ? f1.z : 0.f;
-            fx += vb ? f2.x : 0.f;
-            fy += vb ? f2.y : 0.f;
-            fz += vb ? f2.z : 0.f;
-        }
-        npair += fill;
-        fill = 0;
+        while (wave_any(fill > 0)) drain_step();
     };
 
     auto push = [&](bool hit, int j, unsigned w) {
         if (hit) {
-            qlane[fill * DPD_THREADS] = (unsigned)j | (w << 27);
+            qlane[ring(fill) * DPD_THREADS] = (unsigned)j | (w << 27);
             fill++;
         }
     };
@@ -232,7 +250,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             __builtin_amdgcn_wave_barrier();
             const float4 *sp = sb + (rowok ? j0 - lo : 0);
             for (int t = 0; wave_any(t < n0); t += 4) {
-                if (wave_any(fill + 4 > QCAP)) drain();
+                make_room(4);
                 const float4 c0 = sp[t], c1 = sp[t + 1], c2 = sp[t + 2], c3 = sp[t + 3];
 #define UDX_TEST(C, U)                                                                             \
     {                                                                                              \
@@ -254,7 +272,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             const unsigned w0 = wcode(wx0, wy, wz), w1 = wcode(wx1, wy, wz);
             const int len = n0 + n1;
             for (int t = 0; wave_any(t < len); t += 2) {
-                if (wave_any(fill + 2 > QCAP)) drain();
+                make_room(2);
                 const int ta = t, tb = t + 1;
                 const bool oka = ta < len, okb = tb < len;
                 const int ja = ta >= n0 ? j1 + (ta - n0) : j0 + ta;
