@@ -44,6 +44,8 @@ def parse():
                     help="MiB copied by one plain device copy before the workload (calibrates HBM counters)")
     ap.add_argument("--ens-boxes", type=int, default=64, help="boxes of the batched-ensemble leg (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=40, help="timed steps of the ensemble leg")
+    ap.add_argument("--adj-cells", type=int, default=256, help="cells of the membrane adjoint leg (0: skip)")
+    ap.add_argument("--adj-reps", type=int, default=20, help="timed forward + adjoint batches")
     return ap.parse_args()
 
 
@@ -226,6 +228,8 @@ def main():
     torch.cuda.empty_cache()
     if args.ens_boxes > 0:
         out["ensemble"] = ensemble_leg(ctx, args.ens_boxes, args.ens_steps)
+    if args.adj_cells > 0:
+        out["adjoint"] = adjoint_leg(ctx, args.adj_cells, args.adj_reps, args.cpu_steps > 0)
     print(json.dumps(out))
 
 
@@ -304,6 +308,64 @@ def ensemble_leg(ctx, nbox, nsteps):
         "workload": "64 periodic boxes, edges U{8..48}, density {3,5,10}, gamma U[4,10], kBT U[0.04,1], "
                     "a = 25, dt = 1e-3, graph-captured, autotuned" if nbox == 64 else "%d boxes" % nbox,
     }
+
+
+def adjoint_leg(ctx, nc, reps, cpu):
+    """Membrane forward + adjoint per batch: nc copies of the 642-vertex
+    biconcave cell with Gaussian vertex noise (sigma 0.05), stress-free
+    springs, Kantor bending, the reference's RBC test coefficients; a random
+    cotangent.  The CPU leg is the FP64 oracle gradient of the same batch."""
+    import numpy as np
+    import torch
+
+    from ctc_amd import udx as U
+
+    rr, tt = U.read_off(os.path.join(ROOT, "tests", "golden", "ref", "cells", "642.off"))
+    nv = len(rr)
+    rng = np.random.default_rng(7)
+    off = np.stack([12.0 * (np.arange(nc) % 16), 12.0 * (np.arange(nc) // 16 % 16), 12.0 * (np.arange(nc) // 256)], 1)
+    x = (rr[None].astype(np.float64) + off[:, None]).reshape(-1, 3) + rng.normal(0, 0.05, (nc * nv, 3))
+    pp_h = np.zeros((nc * nv, 6), np.float32)
+    pp_h[:, :3] = x
+    g_h = rng.normal(0, 1, (nc * nv, 3)).astype(np.float32)
+    kw = dict(gammaC=0.0, gammaT=0.0, kBT=0.0444, kb=17.78, phi=6.97, ks=14.8, x0=0.5, mpow=2.0, ka=2450.0,
+              kd=100.0, kv=3750.0, totArea=80.0594, totVolume=43.3586)
+    r = U.Rbc(ctx, tt, nv, nc)
+    r.set_stressfree(rr)
+    r.set_bending(True)
+    par = U.rbc_params(**kw)
+    pp = torch.as_tensor(pp_h).cuda()
+    g = torch.as_tensor(g_h).cuda()
+    ff = torch.zeros((nc * nv, 3), dtype=torch.float32, device="cuda")
+    gx = torch.empty_like(ff)
+    for _ in range(3):
+        r.force(par, 0.0, nc, pp, ff)
+        r.adjoint(par, nc, pp, g, gx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ff.zero_()
+        r.force(par, 0.0, nc, pp, ff)
+        gp = r.adjoint(par, nc, pp, g, gx)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    r.close()
+    out = {"cells": nc, "vertices": nc * nv, "faces": nc * len(tt), "ms_forward_adjoint": ms,
+           "dL_dk": gp,
+           "workload": "%d x 642-vertex membranes, sigma 0.05 vertex noise, stress-free springs, Kantor "
+                       "bending; forward force + adjoint (vertex and coefficient gradients) per batch" % nc}
+    if cpu:
+        from oracle import rbc as R
+        m = R.Mesh(rr, tt)
+        p = R.par(**kw)
+        sf = m.shape()
+        t0 = time.perf_counter()
+        R.adjoint(p, m, pp_h, nc, g_h.astype(np.float64), sf)
+        sec = time.perf_counter() - t0
+        out["cpu_fp64_ms"] = sec * 1e3
+        out["cpu_cores"] = 1
+        out["gpu_over_cpu"] = sec * 1e3 / ms
+    return out
 
 
 def host(t):

@@ -329,10 +329,9 @@ __device__ V3 dih_dual(const AdjK &K, V3 a, V3 b, V3 c, V3 d) {
     return f + b22 * vcross(n, cd);
 }
 
-__global__ __launch_bounds__(RBC_THREADS) void k_rbc_adjoint(AdjK K, const float *pp, const float *g,
-                                                             const double *avd, float *gx, double *gpv) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= K.nc * K.nv) return;
+// one vertex: gx and q[k] = g . F_k
+__device__ __forceinline__ void adjoint_vertex(const AdjK &K, int v, const float *pp, const float *g,
+                                               const double *avd, float *gx, double q[5]) {
     const int cell = v / K.nv, off = cell * K.nv;
     const Dd A{avd[4 * cell], avd[4 * cell + 1]}, Vol{avd[4 * cell + 2], avd[4 * cell + 3]};
     const V3 r0 = dpos(pp, g, v);
@@ -356,7 +355,7 @@ __global__ __launch_bounds__(RBC_THREADS) void k_rbc_adjoint(AdjK K, const float
         dx += K.coef[k] * t[k].x.d;
         dy += K.coef[k] * t[k].y.d;
         dz += K.coef[k] * t[k].z.d;
-        gpv[5 * (long)v + k] = ((double)gv[0] * t[k].x.v + (double)gv[1] * t[k].y.v) + (double)gv[2] * t[k].z.v;
+        q[k] = ((double)gv[0] * t[k].x.v + (double)gv[1] * t[k].y.v) + (double)gv[2] * t[k].z.v;
     }
     float *o = gx + 3 * (long)v;
     o[0] = (float)dx;
@@ -364,12 +363,31 @@ __global__ __launch_bounds__(RBC_THREADS) void k_rbc_adjoint(AdjK K, const float
     o[2] = (float)dz;
 }
 
-// gp[k] = sum over vertices of gpv[5 v + k]: one block, fixed order
-__global__ __launch_bounds__(256) void k_rbc_gp(int n, const double *gpv, double *gp) {
+// gpb[5 b + k]: block b's sum of g . F_k over its vertices (wave shuffles,
+// then the block's waves in order)
+__global__ __launch_bounds__(RBC_THREADS) void k_rbc_adjoint(AdjK K, const float *pp, const float *g,
+                                                             const double *avd, float *gx, double *gpb) {
+    __shared__ double sg[5][RBC_THREADS / 64];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double q[5] = {0, 0, 0, 0, 0};
+    if (v < K.nc * K.nv) adjoint_vertex(K, v, pp, g, avd, gx, q);
+    for (int k = 0; k < 5; k++)
+        for (int d = 32; d >= 1; d >>= 1) q[k] += __shfl_xor(q[k], d, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; k++) sg[k][threadIdx.x >> 6] = q[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double t = 0;
+        for (int w = 0; w < RBC_THREADS / 64; w++) t += sg[threadIdx.x][w];
+        gpb[5 * (long)blockIdx.x + threadIdx.x] = t;
+    }
+}
+// gp[k] = sum over blocks of gpb[5 b + k]: one block, fixed order
+__global__ __launch_bounds__(256) void k_rbc_gp(int nb, const double *gpb, double *gp) {
     __shared__ double s[5][4];
     double q[5] = {0, 0, 0, 0, 0};
-    for (int v = threadIdx.x; v < n; v += 256)
-        for (int k = 0; k < 5; k++) q[k] += gpv[5 * (long)v + k];
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int k = 0; k < 5; k++) q[k] += gpb[5 * (long)b + k];
     for (int k = 0; k < 5; k++)
         for (int d = 32; d >= 1; d >>= 1) q[k] += __shfl_xor(q[k], d, 64);
     if ((threadIdx.x & 63) == 0)
@@ -440,7 +458,7 @@ struct udx_rbc {
     float l0, a0;
     int stress_free, stressful_set, bending;
     double *av;                  // device, 2 * max_cells
-    double *avd, *gpv, *gp;      // adjoint scratch: 4 * max_cells, 5 * max_cells * nv, 5
+    double *avd, *gpb, *gp;      // adjoint scratch: 4 * max_cells, 5 per vertex block, 5
 };
 
 extern "C" {
@@ -454,7 +472,7 @@ int udx_rbc_fin(udx_rbc *r) {
     dev_free(r->aa);
     dev_free(r->av);
     dev_free(r->avd);
-    dev_free(r->gpv);
+    dev_free(r->gpb);
     dev_free(r->gp);
     delete r;
     return UDX_OK;
@@ -628,7 +646,7 @@ int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const flo
     hipStream_t s = r->ctx->stream;
     if (!r->avd) {
         UDX_TRY(dev_alloc(&r->avd, 4 * (size_t)r->max_cells));
-        UDX_TRY(dev_alloc(&r->gpv, 5 * (size_t)r->max_cells * r->nv));
+        UDX_TRY(dev_alloc(&r->gpb, 5 * (size_t)ceil_div(r->max_cells * r->nv, RBC_THREADS)));
         UDX_TRY(dev_alloc(&r->gp, 5));
     }
     hipLaunchKernelGGL(k_rbc_av_dual, dim3(nc), dim3(AV_THREADS), 0, s, r->nt, r->nv, r->tt, pp, g, r->avd);
@@ -655,11 +673,10 @@ int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const flo
     const double phi = (double)p->phi / 180.0 * 3.14159265358979323846;
     K.sphi = std::sin(phi);
     K.cphi = std::cos(phi);
-    const int n = nc * r->nv;
-    hipLaunchKernelGGL(k_rbc_adjoint, dim3(ceil_div(n, RBC_THREADS)), dim3(RBC_THREADS), 0, s, K, pp, g, r->avd,
-                       gx, r->gpv);
+    const int nb = ceil_div(nc * r->nv, RBC_THREADS);
+    hipLaunchKernelGGL(k_rbc_adjoint, dim3(nb), dim3(RBC_THREADS), 0, s, K, pp, g, r->avd, gx, r->gpb);
     UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_rbc_gp, dim3(1), dim3(256), 0, s, n, r->gpv, r->gp);
+    hipLaunchKernelGGL(k_rbc_gp, dim3(1), dim3(256), 0, s, nb, r->gpb, r->gp);
     UDX_LAUNCHED();
     UDX_HIP(hipMemcpyAsync(gp, r->gp, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
     UDX_HIP(hipStreamSynchronize(s));
