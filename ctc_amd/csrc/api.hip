@@ -128,7 +128,31 @@ struct udx_flu {
     int timing;
     hipEvent_t ev[5];
     float tms[5];
+    // membranes in the step (udx_flu_set_rbc)
+    struct Membranes {
+        udx_rbc *r = nullptr;
+        udx_fsi *fsi = nullptr;
+        udx_rbc_params rp;
+        udx_pair_dpd fp;
+        float mass = 0;
+        int substeps = 0, nc = 0, nv = 0, cap = 0;
+        float *pp = nullptr;                               // the caller's vertices (AoS), advanced in place
+        float *fo = nullptr, *ffast = nullptr, *tmp = nullptr;   // FSI force, sub-step force, redistribution staging
+        int *cfid = nullptr, *cnew = nullptr;
+    } mb;
 };
+
+// FSI seeds of step k: after the NSEED x seed_cap solvent table
+static const float *flu_fsi_seeds(const udx_flu *f, int k) { return f->seeds_dev + NSEED * f->seed_cap + 2 * k; }
+
+static void flu_free_membranes(udx_flu *f) {
+    dev_free(f->mb.fo);
+    dev_free(f->mb.ffast);
+    dev_free(f->mb.tmp);
+    dev_free(f->mb.cfid);
+    dev_free(f->mb.cnew);
+    f->mb = udx_flu::Membranes{};
+}
 
 extern "C" {
 
@@ -393,6 +417,7 @@ int udx_flu_fin(udx_flu *f) {
     if (f->seeds_host) (void)hipHostFree(f->seeds_host);
     for (int e = 0; e < 5; e++)
         if (f->ev[e]) (void)hipEventDestroy(f->ev[e]);
+    flu_free_membranes(f);
     delete f;
     return UDX_OK;
 }
@@ -425,8 +450,8 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
         if (f->seeds_host) (void)hipHostFree(f->seeds_host);
         f->seeds_host = nullptr;
         f->seed_cap = 0;
-        UDX_TRY(dev_alloc(&f->seeds_dev, NSEED * (size_t)nsteps));
-        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, NSEED * sizeof(float) * (size_t)nsteps));
+        UDX_TRY(dev_alloc(&f->seeds_dev, (NSEED + 2) * (size_t)nsteps));
+        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, (NSEED + 2) * sizeof(float) * (size_t)nsteps));
         f->seed_cap = nsteps;
     } else {
         // the staging buffer may still feed an earlier copy
@@ -438,21 +463,33 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
         else
             f->trunks.next(f->seeds_host + NSEED * k);
     }
+    // the FSI trunk draws twice per step, bulk then halo (src/fsi/imp/bulk.h:8, src/fsi/imp/halo.h:7)
+    float *fsi = f->seeds_host + NSEED * f->seed_cap;
+    for (int k = 0; k < nsteps; k++) {
+        fsi[2 * k] = f->mb.r ? fsi_draw(f->mb.fsi) : 0.f;
+        fsi[2 * k + 1] = f->mb.r ? fsi_draw(f->mb.fsi) : 0.f;
+    }
     UDX_HIP(hipMemcpyAsync(f->seeds_dev, f->seeds_host, NSEED * sizeof(float) * (size_t)nsteps,
+                           hipMemcpyHostToDevice, s));
+    UDX_HIP(hipMemcpyAsync(f->seeds_dev + NSEED * f->seed_cap, fsi, 2 * sizeof(float) * (size_t)nsteps,
                            hipMemcpyHostToDevice, s));
     return UDX_OK;
 }
 
-// one step of src/sim/imp/run.h:8-68 for the solvent: cell list + reorder,
-// DPD forces, body force + kick-drift + periodic wrap
-static int flu_step(udx_flu *f, const float *seeds_dev) {
+// one step of src/sim/imp/run.h:8-68: solvent cell list + reorder, membrane
+// redistribution, DPD forces, walls, FSI, body force + solvent kick-drift +
+// bounce + periodic wrap, membrane sub-steps (FSI + internal forces, kick-drift)
+static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) {
     hipStream_t s = f->ctx->stream;
     const int nc = f->box.ncells();
     const int a = f->cur, b = 1 - f->cur;
+    udx_flu::Membranes &m = f->mb;
+    const int nvtx = m.nc * m.nv;
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[0], s));
     UDX_TRY(clist_sort(s, &f->cl, f->n, f->pos[a]));
     UDX_TRY(gather_f4(s, &f->cl, f->n, f->pos[a], f->vel[a], f->pos[b], f->vel[b]));
     UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts));
+    if (m.r) UDX_TRY(rbc_redistribute(s, f->box, m.nc, m.nv, m.pp, m.tmp, m.cfid, m.cnew));
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[1], s));
     DpdLaunch d;
     d.pos = f->pos[b];
@@ -477,9 +514,26 @@ static int flu_step(udx_flu *f, const float *seeds_dev) {
                             f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1,
                             f->conf.mode == UDX_MODE_EXACT));
     }
+    if (m.r && nvtx > 0) {
+        // FSI: +f on the vertices, -f on the solvent (src/sim/objinter/imp/main.h:42-91)
+        UDX_HIP(hipMemsetAsync(m.fo, 0, 3 * sizeof(float) * (size_t)nvtx, s));
+        UDX_TRY(fsi_forces(s, fsi_bufs(m.fsi), f->n, f->pos[b], f->vel[b], f->cl.starts, nvtx, m.pp, m.fp, fsi_seeds,
+                           m.fo, (float *)f->ff, 4, fsi_pairs_dev(m.fsi), f->conf.mode == UDX_MODE_EXACT));
+    }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
                          f->pos[b], f->vel[b], sdf));
+    if (m.r && nvtx > 0) {
+        // objects_advance (src/sim/objects/imp/main.h:40-61): per sub-step the
+        // fast force starts as the FSI force, the membrane force is added and
+        // the vertices kick-drift with their own mass
+        const float sub_dt = f->conf.dt / (float)m.substeps;
+        for (int k = 0; k < m.substeps; k++) {
+            UDX_HIP(hipMemcpyAsync(m.ffast, m.fo, 3 * sizeof(float) * (size_t)nvtx, hipMemcpyDeviceToDevice, s));
+            UDX_TRY(udx_rbc_force_apply(m.r, &m.rp, sub_dt, m.nc, m.pp, m.ffast));
+            UDX_TRY(move_aos(s, nvtx, sub_dt, m.mass, m.ffast, m.pp));
+        }
+    }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[4], s));
     f->cur = b;
     return UDX_OK;
@@ -491,7 +545,7 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
     UDX_TRY(flu_seed_table(f, nsteps, seeds_host));
     float acc[4] = {0, 0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
-        UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k));
+        UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k, flu_fsi_seeds(f, k)));
         if (f->timing) {
             UDX_HIP(hipEventSynchronize(f->ev[4]));
             for (int c = 0; c < 4; c++) {
@@ -698,7 +752,53 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
     UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_wall: null handle");
     UDX_CHECK(!w || (w->sdf->f.L[0] == f->box.L[0] && w->sdf->f.L[1] == f->box.L[1] && w->sdf->f.L[2] == f->box.L[2]),
               UDX_EINVAL, "udx_flu_set_wall: wall and solvent boxes differ");
+    UDX_CHECK(!w || !f->mb.r, UDX_EINVAL, "udx_flu_set_wall: membranes together with walls are not supported");
     f->wall = w;
+    return UDX_OK;
+}
+
+int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *rp, const udx_pair_dpd *fp,
+                    float mass, int substeps, int nc, float *pp) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_rbc: null handle");
+    if (!r) {
+        UDX_HIP(hipStreamSynchronize(f->ctx->stream));
+        flu_free_membranes(f);
+        return UDX_OK;
+    }
+    UDX_CHECK(fsi && rp && fp && (nc == 0 || pp), UDX_EINVAL, "udx_flu_set_rbc: null argument");
+    UDX_CHECK(rbc_ctx(r) == f->ctx && fsi_ctx(fsi) == f->ctx, UDX_EINVAL,
+              "udx_flu_set_rbc: membranes, FSI and solvent must share one context");
+    UDX_CHECK(!f->wall, UDX_EINVAL, "udx_flu_set_rbc: membranes together with walls are not supported");
+    UDX_CHECK(mass > 0 && substeps >= 1, UDX_EINVAL, "udx_flu_set_rbc: bad mass or substeps");
+    const int nv = rbc_nv(r);
+    UDX_CHECK(nc >= 0 && nc <= rbc_max_cells(r), UDX_ECAP, "udx_flu_set_rbc: %d cells above the membrane capacity %d",
+              nc, rbc_max_cells(r));
+    UDX_CHECK((long)nc * nv <= fsi_bufs(fsi)->max_obj, UDX_ECAP,
+              "udx_flu_set_rbc: %d vertices above the FSI capacity %d", nc * nv, fsi_bufs(fsi)->max_obj);
+    const FsiBufs *fb = fsi_bufs(fsi);
+    UDX_CHECK(fb->box.L[0] == f->box.L[0] && fb->box.L[1] == f->box.L[1] && fb->box.L[2] == f->box.L[2], UDX_EINVAL,
+              "udx_flu_set_rbc: FSI and solvent boxes differ");
+    udx_flu::Membranes &m = f->mb;
+    const int n = nc * nv;
+    if (n > m.cap) {
+        UDX_HIP(hipStreamSynchronize(f->ctx->stream));
+        flu_free_membranes(f);
+        UDX_TRY(dev_alloc(&m.fo, 3 * (size_t)n));
+        UDX_TRY(dev_alloc(&m.ffast, 3 * (size_t)n));
+        UDX_TRY(dev_alloc(&m.tmp, 6 * (size_t)n));
+        UDX_TRY(dev_alloc(&m.cfid, (size_t)nc));
+        UDX_TRY(dev_alloc(&m.cnew, (size_t)nc));
+        m.cap = n;
+    }
+    m.r = r;
+    m.fsi = fsi;
+    m.rp = *rp;
+    m.fp = *fp;
+    m.mass = mass;
+    m.substeps = substeps;
+    m.nc = nc;
+    m.nv = nv;
+    m.pp = pp;
     return UDX_OK;
 }
 

@@ -322,6 +322,13 @@ struct udx_fsi {
     unsigned long long *pairs_dev;
 };
 
+namespace udx {
+FsiBufs *fsi_bufs(udx_fsi *q) { return &q->f; }
+udx_ctx *fsi_ctx(const udx_fsi *q) { return q->ctx; }
+float fsi_draw(udx_fsi *q) { return udx_rnd_get(&q->trunk); }
+unsigned long long *fsi_pairs_dev(udx_fsi *q) { return q->pairs_dev; }
+}  // namespace udx
+
 extern "C" {
 
 int udx_fsi_fin(udx_fsi *q) {

@@ -444,7 +444,105 @@ int build_adjacency(int md, int nt, int nv, const int *tt, std::vector<int> &adj
     return UDX_OK;
 }
 
+// ---------------------------------------------------------------- redistribution
+// Membranes move as whole cells: a cell whose bounding-box centre
+// 0.5 (min + max) has left [-L/2, L/2) on an axis goes to the fragment of
+// that direction (src/distr/rbc/dev.h:3-20, src/distr/map/dev.h:17-26) and,
+// on one periodic domain, comes back shifted by -L d (src/distr/common/dev.h:
+// 20-32, src/distr/rbc/imp/unpack.h:30-51).  The reference appends the moved
+// cells after the kept ones, kept cells first, then fragments 0..25; within
+// each group the order here is ascending (the reference's comes from
+// atomics).
+constexpr int RD_THREADS = 256;
+__global__ __launch_bounds__(RD_THREADS) void k_rbc_cell_fid(int nv, int Lx, int Ly, int Lz, const float *pp,
+                                                            int *cfid) {
+    __shared__ float s[6][RD_THREADS / 64];
+    const int cell = blockIdx.x;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int k = threadIdx.x; k < nv; k += RD_THREADS) {
+        const float *p = pp + 6 * ((long)cell * nv + k);
+        for (int c = 0; c < 3; c++) {
+            lo[c] = fminf(lo[c], p[c]);
+            hi[c] = fmaxf(hi[c], p[c]);
+        }
+    }
+    for (int c = 0; c < 3; c++)
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo[c] = fminf(lo[c], __shfl_xor(lo[c], d, 64));
+            hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], d, 64));
+        }
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 3; c++) {
+            s[c][threadIdx.x >> 6] = lo[c];
+            s[3 + c][threadIdx.x >> 6] = hi[c];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int L[3] = {Lx, Ly, Lz};
+        int d[3];
+        for (int c = 0; c < 3; c++) {
+            float a = s[c][0], b = s[3 + c][0];
+            for (int w = 1; w < RD_THREADS / 64; w++) {
+                a = fminf(a, s[c][w]);
+                b = fmaxf(b, s[3 + c][w]);
+            }
+            const float r = 0.5f * (a + b);
+            d[c] = -1 + (r >= (float)(-L[c] / 2)) + (r >= (float)(L[c] / 2));
+        }
+        // fragment id of direction d (src/frag/dev.h:19-36)
+        cfid[cell] = (d[0] + 2) % 3 + 3 * ((d[1] + 2) % 3) + 9 * ((d[2] + 2) % 3);
+    }
+}
+
+// new position of every cell: kept cells (fragment 26) first, then
+// fragments 0..25, ascending within each; one thread, nc is small
+__global__ void k_rbc_cell_order(int nc, const int *cfid, int *cnew) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int cnt[27];
+    for (int f = 0; f < 27; f++) cnt[f] = 0;
+    for (int c = 0; c < nc; c++) cnt[cfid[c]]++;
+    int start[27];
+    start[26] = 0;
+    int acc = cnt[26];
+    for (int f = 0; f < 26; f++) {
+        start[f] = acc;
+        acc += cnt[f];
+    }
+    for (int c = 0; c < nc; c++) cnew[c] = start[cfid[c]]++;
+}
+
+__global__ __launch_bounds__(256) void k_rbc_permute(int n, int nv, int Lx, int Ly, int Lz, const int *cfid,
+                                                     const int *cnew, const float *src, float *dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = i / nv, k = i - c * nv, f = cfid[c];
+    const int dx = (f + 2) % 3 - 1, dy = (f / 3 + 2) % 3 - 1, dz = (f / 9 + 2) % 3 - 1;
+    const float *p = src + 6 * (long)i;
+    float *q = dst + 6 * ((long)cnew[c] * nv + k);
+    q[0] = p[0] + (float)(-dx * Lx);
+    q[1] = p[1] + (float)(-dy * Ly);
+    q[2] = p[2] + (float)(-dz * Lz);
+    q[3] = p[3];
+    q[4] = p[4];
+    q[5] = p[5];
+}
+
 }  // namespace
+
+int rbc_redistribute(hipStream_t s, const Box &b, int nc, int nv, float *pp, float *tmp, int *cfid, int *cnew) {
+    if (nc == 0) return UDX_OK;
+    hipLaunchKernelGGL(k_rbc_cell_fid, dim3(nc), dim3(RD_THREADS), 0, s, nv, b.L[0], b.L[1], b.L[2], pp, cfid);
+    UDX_LAUNCHED();
+    hipLaunchKernelGGL(k_rbc_cell_order, dim3(1), dim3(64), 0, s, nc, cfid, cnew);
+    UDX_LAUNCHED();
+    const int n = nc * nv;
+    hipLaunchKernelGGL(k_rbc_permute, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, nv, b.L[0], b.L[1], b.L[2], cfid,
+                       cnew, pp, tmp);
+    UDX_LAUNCHED();
+    UDX_HIP(hipMemcpyAsync(pp, tmp, 6 * sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    return UDX_OK;
+}
+
 }  // namespace udx
 
 using namespace udx;
@@ -460,6 +558,12 @@ struct udx_rbc {
     double *av;                  // device, 2 * max_cells
     double *avd, *gpb, *gp;      // adjoint scratch: 4 * max_cells, 5 per vertex block, 5
 };
+
+namespace udx {
+udx_ctx *rbc_ctx(const udx_rbc *r) { return r->ctx; }
+int rbc_nv(const udx_rbc *r) { return r->nv; }
+int rbc_max_cells(const udx_rbc *r) { return r->max_cells; }
+}  // namespace udx
 
 extern "C" {
 

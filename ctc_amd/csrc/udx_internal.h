@@ -202,6 +202,16 @@ void fsi_free(FsiBufs *f);
 int fsi_forces(hipStream_t s, FsiBufs *f, int ns, const float4 *spos, const float4 *svel, const int *starts,
                int no, const float *opp, const udx_pair_dpd &p, const float *seeds_dev, float *fo, float *fs,
                int fs_stride, unsigned long long *pairs, int exact);
+FsiBufs *fsi_bufs(udx_fsi *q);
+udx_ctx *fsi_ctx(const udx_fsi *q);
+float fsi_draw(udx_fsi *q);               // next seed of the FSI trunk
+unsigned long long *fsi_pairs_dev(udx_fsi *q);
+udx_ctx *rbc_ctx(const udx_rbc *r);
+int rbc_nv(const udx_rbc *r);
+int rbc_max_cells(const udx_rbc *r);
+// whole-cell periodic redistribution of nc cells of nv vertices (AoS pp,
+// in place; tmp: 6 nc nv floats, cfid and cnew: nc ints)
+int rbc_redistribute(hipStream_t s, const Box &b, int nc, int nv, float *pp, float *tmp, int *cfid, int *cnew);
 
 }  // namespace udx
 

@@ -108,6 +108,7 @@ _SIGS = {
     "udx_rbc_force_apply": (_I, [_VP, C.POINTER(RbcParams), _F, _I, _VP, _VP]),
     "udx_rbc_stretch_apply": (_I, [_VP, _I, _I, _VP, _VP]),
     "udx_rbc_force_adjoint": (_I, [_VP, C.POINTER(RbcParams), _I, _VP, _VP, _VP, C.POINTER(C.c_double)]),
+    "udx_flu_set_rbc": (_I, [_VP, _VP, _VP, C.POINTER(RbcParams), C.POINTER(PairDpd), _F, _I, _I, _VP]),
     "udx_fsi_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
     "udx_fsi_fin": (_I, [_VP]),
     "udx_fsi_set_mode": (_I, [_VP, _I]),
@@ -388,6 +389,17 @@ class Flu:
     def set_wall(self, wall):
         self.wall = wall
         _ok(lib().udx_flu_set_wall(self.h, wall.h if wall is not None else None))
+
+    def set_rbc(self, rbc, fsi=None, rbc_params=None, fsi_params=None, mass=0.5, substeps=1, nc=0, pp=None):
+        """membranes in the step (udx_flu_set_rbc); pp: torch (nc * nv, 6)
+        float32 on the device, advanced in place; rbc=None detaches"""
+        if rbc is None:
+            _ok(lib().udx_flu_set_rbc(self.h, None, None, None, None, 0.0, 0, 0, None))
+            self._rbc = None
+            return
+        self._rbc = (rbc, fsi, pp)
+        _ok(lib().udx_flu_set_rbc(self.h, rbc.h, fsi.h, C.byref(rbc_params), C.byref(fsi_params), float(mass),
+                                  int(substeps), int(nc), _ptr(pp) if pp is not None else None))
 
     def wall_pairs(self):
         v = C.c_ulonglong()

@@ -246,6 +246,22 @@ int udx_fsi_apply_seeds(udx_fsi *f, const udx_pair_dpd *p, int nv, const float *
                         const float seeds[2]);
 /* vertex-solvent pairs (r^2 < 1) since the last reset */
 int udx_fsi_pairs(udx_fsi *f, unsigned long long *npairs, int reset);
+/* Membranes in the solvent step (src/sim/imp/run.h:8-68): nc cells of r's
+ * mesh whose vertices pp (device AoS, nc * nv particles, owned by the caller)
+ * the step advances in place.  Per step, after the solvent cell list: whole
+ * cells are redistributed (a cell whose bounding-box centre left the box is
+ * shifted by a box length and moved after the kept cells, kept cells first,
+ * then fragments 0..25, ascending within each; src/distr/rbc/dev.h,
+ * src/distr/map/dev.h:17-26); after the solvent and wall forces, FSI with the
+ * pair fp (fsi: buffers and the seed trunk, two draws per step) adds +f to the
+ * vertices and -f to the solvent; after the solvent kick-drift, each of the
+ * `substeps` membrane sub-steps starts the vertex force from the FSI force,
+ * adds the membrane force rp (udx_rbc_force_apply) and kick-drifts the
+ * vertices with `mass` and dt / substeps (src/sim/objects/imp/main.h:40-61).
+ * Replaces objects_distribute, forces_objects and objects_advance of the
+ * reference's step.  r = NULL detaches the membranes.  Not combined with walls. */
+int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *rp, const udx_pair_dpd *fp,
+                    float mass, int substeps, int nc, float *pp);
 
 /* ---- configuration: replaces conf_ini / conf_read / conf_lookup_* /
  *      conf_set_* (src/conf/imp.h) and the parameter readers pair_set_conf

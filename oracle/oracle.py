@@ -75,6 +75,7 @@ def _declare(L):
         "orc_dpd_forces": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
                                       C.c_int, F32P]),
         "orc_integrate": (None, [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P]),
+        "orc_move": (None, [C.c_int, C.c_float, C.c_float, F32P, F32P]),
         "orc_diag": (None, [C.c_int, F32P, F64P]),
         "orc_flu_step": (C.c_long, [I32P, C.c_int, F32P, F32P, I32P, I32P, F32P, Dpd, F32P,
                                     C.c_float, C.c_float, F32P, C.c_int]),
@@ -221,6 +222,13 @@ def integrate(L, pp, ff, dt, mass=1.0, accel=(0.0, 0.0, 0.0)):
     acc = _f32(accel)
     lib().orc_integrate(_ptr(L, I32P), pp.shape[0], np.float32(dt), np.float32(mass),
                         _ptr(acc, F32P), _ptr(ff, F32P), _ptr(pp, F32P))
+    return pp
+
+
+def move(pp, ff, dt, mass):
+    """kick-drift without the periodic wrap (membrane vertices)"""
+    pp = _f32(pp).reshape(-1, 6).copy()
+    lib().orc_move(pp.shape[0], np.float32(dt), np.float32(mass), _ptr(_f32(ff), F32P), _ptr(pp, F32P))
     return pp
 
 

@@ -214,3 +214,42 @@ def adjoint(p, mesh, pp, nc, g, stress_free=None, l0=0.0, a0=0.0):
                            _ptr(aa, F32P) if aa is not None else None, np.float32(l0), np.float32(a0),
                            _ptr(g, F64P), _ptr(gx, F64P), _ptr(terms, F64P), _ptr(gp, F64P))
     return gx, terms, gp
+
+
+def redistribute(L, opp, nc, nv):
+    """Whole-cell periodic redistribution: a cell whose bounding-box centre
+    0.5 (min + max) left [-L/2, L/2) on an axis is shifted by -L d
+    (src/distr/rbc/dev.h:3-20, src/distr/map/dev.h:17-26,
+    src/distr/common/dev.h:20-32); the kept cells come first, then fragments
+    0..25 (src/distr/rbc/imp/unpack.h:30-51), ascending within each group.
+    Returns (pp, fragment id of each input cell)."""
+    x = _f32(opp).reshape(nc, nv, 6).copy()
+    Li = np.array(L, np.int64)
+    r = np.float32(0.5) * (x[:, :, :3].min(axis=1) + x[:, :, :3].max(axis=1))
+    d = -1 + (r >= (-(Li // 2)).astype(np.float32)).astype(np.int64) + (r >= (Li // 2).astype(np.float32))
+    fid = (d[:, 0] + 2) % 3 + 3 * ((d[:, 1] + 2) % 3) + 9 * ((d[:, 2] + 2) % 3)
+    x[:, :, :3] = x[:, :, :3] + (-d * Li).astype(np.float32)[:, None, :]
+    order = [c for c in range(nc) if fid[c] == 26] + [c for f in range(26) for c in range(nc) if fid[c] == f]
+    return x[order].reshape(-1, 6), fid
+
+
+def suspension_step(L, spp, opp, mesh, nc, dpd, seeds28, fsi_par, fsi_seed2, p, dt, mass=1.0, rbc_mass=0.5,
+                    substeps=1, stress_free=None, l0=0.0, a0=0.0, nthreads=1):
+    """One step of solvent + membranes on one periodic domain in the order of
+    src/sim/imp/run.h:8-68: solvent cell list, membrane redistribution,
+    solvent DPD, FSI (+f vertices, -f solvent), solvent kick-drift + wrap,
+    then per sub-step the vertex force = FSI force + float(membrane force)
+    and the vertex kick-drift (src/sim/objects/imp/main.h:40-61).  Returns
+    (cell-ordered solvent, vertices, solvent pairs, FSI pairs)."""
+    from . import oracle as O
+    starts, counts, srt, _ = O.clist_build(L, spp)
+    opp, _ = redistribute(L, opp, nc, mesh.nv)
+    ff, npair = O.dpd_forces(L, srt, starts, counts, dpd, seeds28[:27], nthreads=nthreads)
+    fo, ff, nfsi = fsi_forces(L, srt, starts, opp, fsi_par, fsi_seed2, fs=ff)
+    sout = O.integrate(L, srt, ff, dt, mass)
+    sub_dt = np.float32(np.float32(dt) / np.float32(substeps))
+    for _ in range(substeps):
+        F = forces(p, mesh, opp, nc, area_volume(mesh, opp, nc), stress_free=stress_free, l0=l0, a0=a0)
+        ffast = fo + F.astype(np.float32)
+        opp = O.move(opp, ffast, sub_dt, rbc_mass)
+    return sout, opp, npair // 2, nfsi

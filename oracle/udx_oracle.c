@@ -472,6 +472,18 @@ void orc_integrate(const int L[3], int n, float dt, float mass, const float acce
     }
 }
 
+/* kick-drift without wrap, for membrane vertices (src/scheme/move/dev/vv.h:1-10) */
+void orc_move(int n, float dt, float mass, const float *ff, float *pp) {
+    for (int i = 0; i < n; i++) {
+        float *r = pp + 6 * i, *v = r + 3;
+        const float *f = ff + 3 * i;
+        for (int k = 0; k < 3; k++) {
+            v[k] = fmaf(f[k] / mass, dt, v[k]);
+            r[k] = fmaf(v[k], dt, r[k]);
+        }
+    }
+}
+
 /* ======================================================================== */
 /* Diagnostics: src/io/diag/part/imp/main.h:37-63 in double                 */
 /* ======================================================================== */

@@ -1,0 +1,145 @@
+"""Solvent, membranes and FSI in one step (udx_flu_set_rbc) against the
+oracle's suspension step (oracle/rbc.py suspension_step) on the same seeds.
+
+Two 498-vertex cells scaled to a diameter of 7.8 in a 16^3 box at density 10:
+one whose bounding-box centre starts outside the box (the first step's
+redistribution shifts it by a box length and moves it after the other cell)
+and one that crosses the y boundary (FSI through periodic images).  Exact
+mode: the solvent DPD and FSI kernels are bit-exact with the oracle, the
+membrane force is within the float rounding of its double sum (device and
+host pow/sqrt may differ by an ulp), so after one step the solvent state is
+identical bit for bit and the vertices within 1e-6; after three steps both
+agree within 1e-5 of the box scale (positions) and of the largest velocity."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import rbc as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from ctc_amd import udx as U  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return U.Context()
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a, np.float32)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def case(L=(16, 16, 16), rho=10, seed=11):
+    rr, tt = R.read_off(os.path.join(GOLD, "cells", "498.off"))
+    m = R.Mesh((rr * 1.5).astype(np.float32), tt)
+    rng = np.random.default_rng(seed)
+    T = []
+    for centre in ([0.5 * L[0] + 0.3, -2.0, 1.0], [-3.0, 0.5 * L[1] - 1.5, -2.5]):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        A = np.eye(4)
+        A[:3, :3] = q * np.sign(np.linalg.det(q))
+        A[:3, 3] = centre
+        T.append(A)
+    opp = R.place(m.rr, T, (0, 0, 0))
+    opp[:, 3:] = rng.normal(0, 0.05, (len(opp), 3)).astype(np.float32)
+    n = int(rho * np.prod(L))
+    spp = np.zeros((n, 6), np.float32)
+    spp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    spp[:, 3:] = rng.normal(0, 0.21, (n, 3)).astype(np.float32)
+    tmpl = np.zeros((m.nv, 6), np.float32)
+    tmpl[:, :3] = m.rr
+    A0, V0 = R.area_volume(m, tmpl, 1)
+    kw = dict(R.RBC_TEST, kb=40.0, totArea=np.float32(A0), totVolume=np.float32(V0))
+    l0, a0 = m.stressful(kw["totArea"])
+    return L, m, spp, opp, kw, l0, a0
+
+
+KBT, DT = 0.0444, 5e-4
+
+
+def run_gpu(ctx, L, m, spp, opp, kw, nsteps, seeds, dpd, fsi_par, substeps, exact=True):
+    nc = len(opp) // m.nv
+    flu = U.Flu(ctx, L, len(spp), U.PairDpd(*dpd), DT, 1.0, exact=exact)
+    flu.set_pp(dev(spp))
+    r = U.Rbc(ctx, m.tt, m.nv, nc)
+    r.set_stressful(kw["totArea"])
+    r.set_bending(True)
+    fsi = U.Fsi(ctx, L, nc * m.nv)
+    vert = dev(opp)
+    flu.set_rbc(r, fsi, U.rbc_params(**kw), U.PairDpd(*fsi_par), 0.5, substeps, nc, vert)
+    flu.run(nsteps, seeds)
+    sol = torch.zeros((len(spp), 6), dtype=torch.float32, device="cuda")
+    flu.get_pp(sol)
+    out = host(sol), host(vert), flu.pairs(), fsi.pairs()
+    flu.set_rbc(None)
+    flu.close()
+    r.close()
+    return out
+
+
+def run_oracle(L, m, spp, opp, kw, l0, a0, nsteps, seeds, dpd, fsi_par, substeps):
+    nc = len(opp) // m.nv
+    fseeds = R.fsi_seeds(nsteps)
+    p = R.par(**kw)
+    npair = nfsi = 0
+    for k in range(nsteps):
+        spp, opp, a, b = R.suspension_step(L, spp, opp, m, nc, dpd, seeds[k], fsi_par, fseeds[k], p, DT,
+                                           substeps=substeps, l0=l0, a0=a0, nthreads=8)
+        npair += a
+        nfsi += b
+    return spp, opp, npair, nfsi
+
+
+@pytest.mark.parametrize("nsteps,substeps", [(1, 1), (3, 2)])
+def test_suspension_step(ctx, nsteps, substeps):
+    L, m, spp, opp, kw, l0, a0 = case()
+    say("suspension: %d steps, %d membrane sub-steps" % (nsteps, substeps))
+    dpd = (2.5, 8.0, O.dpd_sigma(KBT, 8.0, DT), 0.25)
+    fsi_par = (2.6666667, 8.0, O.dpd_sigma(KBT, 8.0, DT), 0.25)
+    seeds = O.seeds_table(nsteps)
+    gs, gv, gpairs, gfsi = run_gpu(ctx, L, m, spp, opp, kw, nsteps, seeds, dpd, fsi_par, substeps)
+    rs, rv, rpairs, rfsi = run_oracle(L, m, spp, opp, kw, l0, a0, nsteps, seeds, dpd, fsi_par, substeps)
+    assert gpairs == rpairs and gfsi == rfsi
+    # the first cell's centre started outside: it was shifted by -L x and now comes last
+    assert np.abs(gv[m.nv:, 0] - (opp[:m.nv, 0] - L[0])).max() < 0.1
+    vmax = np.abs(rv[:, 3:]).max()
+    if nsteps == 1:
+        assert np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
+        assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-6
+        assert np.abs(gv[:, 3:] - rv[:, 3:]).max() <= 1e-6 * vmax
+    else:
+        assert np.abs(gs[:, :3] - rs[:, :3]).max() <= 1e-5 * L[0]
+        assert np.abs(gs[:, 3:] - rs[:, 3:]).max() <= 1e-5 * np.abs(rs[:, 3:]).max()
+        assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-5 * L[0]
+        assert np.abs(gv[:, 3:] - rv[:, 3:]).max() <= 1e-5 * vmax
+
+
+def test_suspension_rejects_bad_calls(ctx):
+    L, m, spp, opp, kw, l0, a0 = case()
+    flu = U.Flu(ctx, L, len(spp), U.PairDpd(2.5, 8.0, 1.0, 0.25), DT, 1.0)
+    r = U.Rbc(ctx, m.tt, m.nv, 1)
+    fsi = U.Fsi(ctx, L, m.nv)
+    with pytest.raises(U.UdxError, match="capacity"):
+        flu.set_rbc(r, fsi, U.rbc_params(**kw), U.PairDpd(2.6, 8.0, 1.0, 0.25), 0.5, 1, 2, dev(opp))
+    flu.close()
+    r.close()

@@ -392,3 +392,37 @@ def test_c4_fp64_gradient_is_the_central_difference():
         km[j] -= h * k[j]
         fdk = (c.loss64(coef=kp)[0] - c.loss64(coef=km)[0]) / (2 * h * k[j])
         assert abs(fdk - gk[j]) <= 1e-5 * abs(gk[j]), COEF[j]
+
+
+def test_redistribute_whole_cells():
+    """Cells move as a whole by their bounding-box centre: kept cells first,
+    moved cells after them in fragment order, each shifted by -L d, every
+    centre back inside the box; a cell whose centre sits on +L/2 moves
+    (the interval is [-L/2, L/2)), and its shifted centre may round to just
+    below -L/2, as in the reference (float positions, float shift)."""
+    L = (16, 12, 20)
+    m = mesh(498)
+    rng = np.random.default_rng(4)
+    centres = [[0.0, 0.0, 0.0], [8.5, 0.0, 0.0], [-3.0, -7.0, 11.0], [2.0, 3.0, -1.0], [0.0, 6.0, 0.0]]
+    T = []
+    for c in centres:
+        A = np.eye(4)
+        A[:3, 3] = c
+        T.append(A)
+    pp = R.place(m.rr, T, (0, 0, 0))
+    # the fifth cell: bounding-box centre exactly on +Ly/2
+    x = pp[4 * m.nv:, 1]
+    x += np.float32(6.0) - np.float32(0.5) * (x.min() + x.max())
+    out, fid = R.redistribute(L, pp, len(centres), m.nv)
+    d = np.array([[0, 0, 0], [1, 0, 0], [0, -1, 1], [0, 0, 0], [0, 1, 0]])
+    want_fid = (d[:, 0] + 2) % 3 + 3 * ((d[:, 1] + 2) % 3) + 9 * ((d[:, 2] + 2) % 3)
+    assert np.array_equal(fid, want_fid)
+    order = [0, 3] + sorted([1, 2, 4], key=lambda c: want_fid[c])
+    src = pp.reshape(5, m.nv, 6)
+    got = out.reshape(5, m.nv, 6)
+    for k, c in enumerate(order):
+        assert np.array_equal(got[k, :, 3:], src[c, :, 3:])
+        assert np.array_equal(got[k, :, :3], src[c, :, :3] + (-d[c] * np.array(L)).astype(np.float32))
+        lo, hi = got[k, :, :3].min(0), got[k, :, :3].max(0)
+        ctr = np.float32(0.5) * (lo + hi)
+        assert np.all(ctr >= -np.array(L) / 2 - 1e-5) and np.all(ctr < np.array(L) / 2)
