@@ -11,8 +11,9 @@
 //
 // Here both sides gather, so no atomics are needed and the sums have a
 // fixed order:
-//  - k_fsi_vertex: one thread per vertex, its three z-plane partial sums and
-//    then its images in halo-list order;
+//  - k_fsi_instance: one thread per instance (vertex or image) in cell
+//    order, one sum over its block; k_fsi_vertex_sum adds to each vertex
+//    its own sum, then its images' in halo-list order;
 //  - k_fsi_solvent: one thread per solvent particle over the vertex/image
 //    instances of the 27 cells around it, from a cell list of the instances
 //    on the (L+2)^3 grid of cells -1..L (the only cells whose clipped block
@@ -54,15 +55,18 @@ __global__ __launch_bounds__(256) void k_fsi_flags(int no, const float *opp, int
 // instance AoS: vertices [0, no) as they are, images at no + aid shifted by
 // +d_i L; meta = (noise index, 0 vertex / 1 image).  The unused slots up to
 // 8 no (at most 7 images per vertex) get a position outside every cell so
-// the instance cell list drops them.
+// the instance cell list drops them.  The force sum of every real instance
+// starts at zero (instances the cell list drops have no solvent in reach).
 __global__ __launch_bounds__(256) void k_fsi_instances(int no, const float *opp, int Lx, int Ly, int Lz,
-                                                       const int *flag, const int *aid, float *inst, int2 *meta) {
+                                                       const int *flag, const int *aid, float *inst, int2 *meta,
+                                                       float *sinst) {
     const long g = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (g < no) {
         const float *p = opp + 6 * g;
         float *o = inst + 6 * g;
         for (int c = 0; c < 6; c++) o[c] = p[c];
         meta[g] = make_int2((int)g, 0);
+        sinst[3 * g] = sinst[3 * g + 1] = sinst[3 * g + 2] = 0.f;
     } else if (g < 8L * no && g >= no + aid[26L * no]) {
         float *o = inst + 6 * g;
         o[0] = o[1] = o[2] = 1e30f;
@@ -82,6 +86,7 @@ __global__ __launch_bounds__(256) void k_fsi_instances(int no, const float *opp,
     o[4] = p[4];
     o[5] = p[5];
     meta[k] = make_int2(aid[g], 1);
+    sinst[3 * k] = sinst[3 * k + 1] = sinst[3 * k + 2] = 0.f;
 }
 
 struct FsiK {
@@ -92,77 +97,122 @@ struct FsiK {
     const float *seeds;      // device: [bulk, halo]
 };
 
-// sum over one z plane of the clipped block around r (src/fsi/dev/map/bulk.h)
+// Both sides queue the candidates that pass the cutoff in a per-lane ring
+// in LDS and drain the rings together, two pairs per lane per step, so the
+// pair body (noise + force) runs with most lanes active: tested inline, about
+// one candidate in six passes and every lane of the wave would wait for the
+// body at nearly every candidate.  The queues keep candidate order, so the
+// sums are the same as a sequential walk.
+constexpr int FQ_V = 24, FQ_S = 16;   // ring entries per lane: instance side, solvent side
+constexpr int FSI_DRAIN_STEPS = 2;
+constexpr int FSI_SCAP = 160;         // staged instances per wave and row (solvent side)
+struct FsiIn {
+    float dx, dy, dz, r2, dvx, dvy, dvz, x;
+    bool pair;
+};
+
+// Vertex side, one thread per instance (the vertex or one of its images)
+// in cell order of the instance grid, so the lanes of a wave read the same
+// solvent cells: the instance's sum over its clipped block, z planes 0..2,
+// rows y - 1..y + 1, solvent in cell order (src/fsi/dev/map/bulk.h), written
+// to sinst at the instance's own index.  k_fsi_vertex_sum then adds, per
+// vertex, its own sum and its images' sums in halo-list order.
 template <int WR, bool EXACT>
-__device__ int plane_sum(const FsiK &K, float rx, float ry, float rz, const float3 &v, int zplane, unsigned lid,
-                         float seed, float *fx, float *fy, float *fz) {
+__global__ __launch_bounds__(FSI_THREADS) void k_fsi_instance(FsiK K, const int *n_dev, const float *inst,
+                                                              const int2 *meta, const int *order, float *sinst,
+                                                              unsigned long long *pairs) {
+    __shared__ unsigned q[FQ_V * FSI_THREADS];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < *n_dev;
+    const float *p = inst + 6 * (long)(active ? t : 0);
+    const float rx = p[0], ry = p[1], rz = p[2], ux = p[3], uy = p[4], uz = p[5];
+    const int2 m = active ? meta[t] : make_int2(0, 0);
+    const unsigned lid = (unsigned)m.x;
+    const float seed = m.y ? K.seeds[1] : K.seeds[0];
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    int fill = 0, head = 0, np = 0;
+    unsigned *ql = q + threadIdx.x;
+    auto ring = [&](int k) {
+        const int r = head + k;
+        return r >= FQ_V ? r - FQ_V : r;
+    };
+    auto in_of = [&](unsigned b) {
+        FsiIn o;
+        const float4 pb = K.spos[b], vb = K.svel[b];
+        o.dx = rx - pb.x;
+        o.dy = ry - pb.y;
+        o.dz = rz - pb.z;
+        o.r2 = __builtin_fmaf(o.dz, o.dz, __builtin_fmaf(o.dx, o.dx, o.dy * o.dy));
+        o.dvx = ux - vb.x;
+        o.dvy = uy - vb.y;
+        o.dvz = uz - vb.z;
+        // mean0var1uu(seed, lid, b) before the logistic map (src/math/rnd/dev.h:65-72)
+        o.x = seed - low_discrepancy((float)(lid & 0x3FFu), (float)lid, (float)(b & 0x3FFu), (float)b);
+        o.pair = true;
+        return o;
+    };
+    auto drain_step = [&]() {
+        const bool va = fill > 0, vb = fill > 1;
+        const unsigned ea = va ? ql[ring(0) * FSI_THREADS] : 0u;     // solvent 0 exists when anything is queued
+        const unsigned eb = vb ? ql[ring(1) * FSI_THREADS] : ea;
+        const FsiIn a = in_of(ea), b = in_of(eb);
+        float xa = a.x, xb = b.x;
+        logistic18x2(&xa, &xb);
+        const float3 fa = dpd_pair<WR, EXACT>(K.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+        const float3 fb = dpd_pair<WR, EXACT>(K.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+        if (va) {
+            sx += fa.x;
+            sy += fa.y;
+            sz += fa.z;
+        }
+        if (vb) {
+            sx += fb.x;
+            sy += fb.y;
+            sz += fb.z;
+        }
+        const int took = fill < 2 ? fill : 2;
+        np += took;
+        head = ring(took);
+        fill -= took;
+    };
     const int xc = K.Lx / 2 + (int)__builtin_floorf(rx);
     const int yc = K.Ly / 2 + (int)__builtin_floorf(ry);
     const int zc = K.Lz / 2 + (int)__builtin_floorf(rz);
-    if (xc - 1 >= K.Lx || xc + 2 <= 0) return 0;
+    const bool xok = active && !(xc - 1 >= K.Lx || xc + 2 <= 0);
     const int xs = xc - 1 > 0 ? xc - 1 : 0, xe = xc + 2 < K.Lx ? xc + 2 : K.Lx;
-    const int z = zc - 1 + zplane;
-    if (z < 0 || z >= K.Lz) return 0;
-    int np = 0;
-    for (int k = 0; k < 3; k++) {
-        const int y = yc - 1 + k;
-        if (y < 0 || y >= K.Ly) continue;
-        const int cid = xs + K.Lx * (y + K.Ly * z);
-        const int b0 = K.starts[cid], b1 = K.starts[cid + (xe - xs)];
-        for (int b = b0; b < b1; b++) {
-            const float4 pb = K.spos[b], vb = K.svel[b];
-            const float dx = rx - pb.x, dy = ry - pb.y, dz = rz - pb.z;
-            const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
-            if (r2 < 1.0f) {
-                const float rnd = mean0var1uu(seed, lid, (unsigned)b);
-                const float3 f = dpd_pair<WR, EXACT>(K.par, dx, dy, dz, r2, v.x - vb.x, v.y - vb.y, v.z - vb.z, rnd);
-                *fx += f.x;
-                *fy += f.y;
-                *fz += f.z;
-                np++;
+    for (int row = 0; row < 9; row++) {
+        const int z = zc - 1 + row / 3, y = yc - 1 + row % 3;
+        int b0 = 0, len = 0;
+        if (xok && z >= 0 && z < K.Lz && y >= 0 && y < K.Ly) {
+            const int cid = xs + K.Lx * (y + K.Ly * z);
+            b0 = K.starts[cid];
+            len = K.starts[cid + (xe - xs)] - b0;
+        }
+        // four candidates per iteration: four loads in flight per lane
+        for (int c = 0; wave_any(c < len); c += 4) {
+            if (wave_any(fill + 4 > FQ_V))
+#pragma unroll 1
+                for (int k = 0; k < FSI_DRAIN_STEPS; k++) drain_step();
+            float4 pb[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) pb[u] = c + u < len ? K.spos[b0 + c + u] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float dx = rx - pb[u].x, dy = ry - pb[u].y, dz = rz - pb[u].z;
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
+                if (c + u < len && r2 < 1.0f) {
+                    ql[ring(fill) * FSI_THREADS] = (unsigned)(b0 + c + u);
+                    fill++;
+                }
             }
         }
     }
-    return np;
-}
-
-template <int WR, bool EXACT>
-__global__ __launch_bounds__(FSI_THREADS) void k_fsi_vertex(FsiK K, int no, const float *opp, const int *flag,
-                                                            const int *aid, float *fo,
-                                                            unsigned long long *pairs) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    int np = 0;
-    if (v < no) {
-        const float *p = opp + 6 * (long)v;
-        const float3 u = make_float3(p[3], p[4], p[5]);
-        const float sb = K.seeds[0], sh = K.seeds[1];
-        float *o = fo + 3 * (long)v;
-        float ox = o[0], oy = o[1], oz = o[2];
-        for (int zp = 0; zp < 3; zp++) {
-            float sx = 0.f, sy = 0.f, sz = 0.f;
-            np += plane_sum<WR, EXACT>(K, p[0], p[1], p[2], u, zp, (unsigned)v, sb, &sx, &sy, &sz);
-            ox += sx;
-            oy += sy;
-            oz += sz;
-        }
-        const int ex = emap_axis(p[0], K.Lx), ey = emap_axis(p[1], K.Ly), ez = emap_axis(p[2], K.Lz);
-        if (ex | ey | ez)
-            for (int i = 0; i < 26; i++) {
-                if (!flag[(long)i * no + v]) continue;
-                const int dx = (i + 2) % 3 - 1, dy = (i / 3 + 2) % 3 - 1, dz = (i / 9 + 2) % 3 - 1;
-                const float rx = p[0] + (float)(dx * K.Lx), ry = p[1] + (float)(dy * K.Ly),
-                            rz = p[2] + (float)(dz * K.Lz);
-                const unsigned lid = (unsigned)aid[(long)i * no + v];
-                float sx = 0.f, sy = 0.f, sz = 0.f;
-                for (int zp = 0; zp < 3; zp++)
-                    np += plane_sum<WR, EXACT>(K, rx, ry, rz, u, zp, lid, sh, &sx, &sy, &sz);
-                ox += sx;
-                oy += sy;
-                oz += sz;
-            }
-        o[0] = ox;
-        o[1] = oy;
-        o[2] = oz;
+    while (wave_any(fill > 0)) drain_step();
+    if (active) {
+        float *o = sinst + 3 * (long)order[t];
+        o[0] = sx;
+        o[1] = sy;
+        o[2] = sz;
     }
     if (pairs) {
         unsigned long long c = (unsigned long long)np;
@@ -171,44 +221,166 @@ __global__ __launch_bounds__(FSI_THREADS) void k_fsi_vertex(FsiK K, int no, cons
     }
 }
 
+// vertex force += its own instance sum, then its images' in halo-list order
+__global__ __launch_bounds__(256) void k_fsi_vertex_sum(int no, const int *flag, const int *aid, const float *sinst,
+                                                        float *fo) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= no) return;
+    float *o = fo + 3 * (long)v;
+    float ox = o[0] + sinst[3 * (long)v], oy = o[1] + sinst[3 * (long)v + 1], oz = o[2] + sinst[3 * (long)v + 2];
+    for (int i = 0; i < 26; i++) {
+        if (!flag[(long)i * no + v]) continue;
+        const float *s = sinst + 3 * ((long)no + aid[(long)i * no + v]);
+        ox += s[0];
+        oy += s[1];
+        oz += s[2];
+    }
+    o[0] = ox;
+    o[1] = oy;
+    o[2] = oz;
+}
+
 // solvent side: -f of every instance within one cell, instances in cell
 // order on the (L+2)^3 grid, stable within a cell
 template <int WR, bool EXACT>
 __global__ __launch_bounds__(FSI_THREADS) void k_fsi_solvent(FsiK K, int ns, const float *inst, const int2 *meta,
                                                              const int *istarts, float *fs, int fs_stride) {
+    __shared__ unsigned q[FQ_S * FSI_THREADS];
+    __shared__ float4 stage[FSI_THREADS / 64][FSI_SCAP];
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= ns) return;
-    const float4 pb = K.spos[b], vb = K.svel[b];
+    const bool active = b < ns;
+    float4 *sw = stage[threadIdx.x >> 6];
+    float4 pb = make_float4(0.f, 0.f, 0.f, 0.f), vb = pb;
     int cx = 0, cy = 0, cz = 0;
-    cell_coord(pb.x, K.Lx, &cx);
-    cell_coord(pb.y, K.Ly, &cy);
-    cell_coord(pb.z, K.Lz, &cz);
+    if (active) {
+        pb = K.spos[b];
+        vb = K.svel[b];
+        cell_coord(pb.x, K.Lx, &cx);
+        cell_coord(pb.y, K.Ly, &cy);
+        cell_coord(pb.z, K.Lz, &cz);
+    }
     const int Wx = K.Lx + 2, Wy = K.Ly + 2;
     const float sb = K.seeds[0], sh = K.seeds[1];
     float ax = 0.f, ay = 0.f, az = 0.f;
+    int fill = 0, head = 0;
+    unsigned *ql = q + threadIdx.x;
+    auto ring = [&](int k) {
+        const int r = head + k;
+        return r >= FQ_S ? r - FQ_S : r;
+    };
+    auto in_of = [&](unsigned t) {
+        FsiIn o;
+        const float *qq = inst + 6 * (long)t;
+        const int2 m = meta[t];
+        o.dx = qq[0] - pb.x;
+        o.dy = qq[1] - pb.y;
+        o.dz = qq[2] - pb.z;
+        o.r2 = __builtin_fmaf(o.dz, o.dz, __builtin_fmaf(o.dx, o.dx, o.dy * o.dy));
+        o.dvx = qq[3] - vb.x;
+        o.dvy = qq[4] - vb.y;
+        o.dvz = qq[5] - vb.z;
+        const unsigned lid = (unsigned)m.x, bu = (unsigned)b;
+        o.x = (m.y ? sh : sb) - low_discrepancy((float)(lid & 0x3FFu), (float)lid, (float)(bu & 0x3FFu), (float)bu);
+        o.pair = true;
+        return o;
+    };
+    auto drain_step = [&]() {
+        const bool va = fill > 0, vb2 = fill > 1;
+        const unsigned ea = va ? ql[ring(0) * FSI_THREADS] : 0u;   // instance 0 exists (no > 0)
+        const unsigned eb = vb2 ? ql[ring(1) * FSI_THREADS] : 0u;
+        const FsiIn a = in_of(ea), c = in_of(eb);
+        float xa = a.x, xc = c.x;
+        logistic18x2(&xa, &xc);
+        const float3 fa = dpd_pair<WR, EXACT>(K.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+        const float3 fc = dpd_pair<WR, EXACT>(K.par, c.dx, c.dy, c.dz, c.r2, c.dvx, c.dvy, c.dvz, xc * R_SQRT2);
+        if (va) {
+            ax -= fa.x;
+            ay -= fa.y;
+            az -= fa.z;
+        }
+        if (vb2) {
+            ax -= fc.x;
+            ay -= fc.y;
+            az -= fc.z;
+        }
+        const int took = fill < 2 ? fill : 2;
+        head = ring(took);
+        fill -= took;
+    };
+    auto make_room = [&](int need) {
+        if (wave_any(fill + need > FQ_S))
+#pragma unroll 1
+            for (int k = 0; k < FSI_DRAIN_STEPS; k++) drain_step();
+    };
     for (int dz = 0; dz < 3; dz++)
         for (int dy = 0; dy < 3; dy++) {
-            const int w0 = cx + Wx * ((cy + dy) + Wy * (cz + dz));
-            const int t0 = istarts[w0], t1 = istarts[w0 + 3];
-            for (int t = t0; t < t1; t++) {
-                const float *q = inst + 6 * (long)t;
-                const float dx = q[0] - pb.x, ddy = q[1] - pb.y, ddz = q[2] - pb.z;
-                const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(dx, dx, ddy * ddy));
-                if (r2 < 1.0f) {
-                    const int2 m = meta[t];
-                    const float rnd = mean0var1uu(m.y ? sh : sb, (unsigned)m.x, (unsigned)b);
-                    const float3 f = dpd_pair<WR, EXACT>(K.par, dx, ddy, ddz, r2, q[3] - vb.x, q[4] - vb.y,
-                                                         q[5] - vb.z, rnd);
-                    ax -= f.x;
-                    ay -= f.y;
-                    az -= f.z;
+            int t0 = 0, len = 0;
+            if (active) {
+                const int w0 = cx + Wx * ((cy + dy) + Wy * (cz + dz));
+                t0 = istarts[w0];
+                len = istarts[w0 + 3] - t0;
+            }
+            // the lanes are consecutive cell-ordered solvent particles: the
+            // union of their instance runs usually fits one small range,
+            // staged in LDS with coalesced loads
+            int lo = active ? t0 : 0x7fffffff, hi = active ? t0 + len : 0;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) {
+                lo = min(lo, __shfl_xor(lo, d, 64));
+                hi = max(hi, __shfl_xor(hi, d, 64));
+            }
+            if (hi > lo && hi - lo <= FSI_SCAP) {
+                __builtin_amdgcn_wave_barrier();
+                for (int k = threadIdx.x & 63; k < hi - lo; k += 64) {
+                    const float *qq = inst + 6 * (long)(lo + k);
+                    sw[k] = make_float4(qq[0], qq[1], qq[2], 0.f);
+                }
+                __builtin_amdgcn_wave_barrier();
+                const int o = active ? t0 - lo : 0;
+                for (int t = 0; wave_any(t < len); t += 4) {
+                    make_room(4);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const float4 c = sw[o + t + u < FSI_SCAP ? o + t + u : 0];
+                        const float dx = c.x - pb.x, ddy = c.y - pb.y, ddz = c.z - pb.z;
+                        const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(dx, dx, ddy * ddy));
+                        if (t + u < len && r2 < 1.0f) {
+                            ql[ring(fill) * FSI_THREADS] = (unsigned)(t0 + t + u);
+                            fill++;
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                continue;
+            }
+            for (int t = 0; wave_any(t < len); t += 2) {
+                if (wave_any(fill + 2 > FQ_S))
+#pragma unroll 1
+                    for (int k = 0; k < FSI_DRAIN_STEPS; k++) drain_step();
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const bool in = t + u < len;
+                    const int tt = t0 + t + u;
+                    float r2 = 2.0f;
+                    if (in) {
+                        const float *qq = inst + 6 * (long)tt;
+                        const float dx = qq[0] - pb.x, ddy = qq[1] - pb.y, ddz = qq[2] - pb.z;
+                        r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(dx, dx, ddy * ddy));
+                    }
+                    if (in && r2 < 1.0f) {
+                        ql[ring(fill) * FSI_THREADS] = (unsigned)tt;
+                        fill++;
+                    }
                 }
             }
         }
-    float *o = fs + (long)fs_stride * b;
-    o[0] += ax;
-    o[1] += ay;
-    o[2] += az;
+    while (wave_any(fill > 0)) drain_step();
+    if (active) {
+        float *o = fs + (long)fs_stride * b;
+        o[0] += ax;
+        o[1] += ay;
+        o[2] += az;
+    }
 }
 
 __global__ void k_gather_int2(int n, const int *n_dev, const int *order, const int2 *in, int2 *out) {
@@ -231,6 +403,7 @@ int fsi_alloc(FsiBufs *f, const Box &b, int max_obj) {
     UDX_TRY(dev_alloc(&f->inst_sorted, 6 * ni));
     UDX_TRY(dev_alloc(&f->meta, ni));
     UDX_TRY(dev_alloc(&f->meta_sorted, ni));
+    UDX_TRY(dev_alloc(&f->sinst, 3 * ni));
     Box m{{b.L[0] + 2, b.L[1] + 2, b.L[2] + 2}};
     UDX_TRY(clist_alloc(&f->cl, m, (int)ni));
     return UDX_OK;
@@ -244,6 +417,7 @@ void fsi_free(FsiBufs *f) {
     dev_free(f->inst_sorted);
     dev_free(f->meta);
     dev_free(f->meta_sorted);
+    dev_free(f->sinst);
     clist_free(&f->cl);
 }
 
@@ -260,7 +434,7 @@ int fsi_forces(hipStream_t s, FsiBufs *f, int ns, const float4 *spos, const floa
     UDX_LAUNCHED();
     UDX_TRY(scan_exclusive(s, f->flag, (int)g, f->aid, f->scan_tmp));
     hipLaunchKernelGGL(k_fsi_instances, dim3(ceil_div(g, 256)), dim3(256), 0, s, no, opp, b.L[0], b.L[1], b.L[2],
-                       f->flag, f->aid, f->inst, f->meta);
+                       f->flag, f->aid, f->inst, f->meta, f->sinst);
     UDX_LAUNCHED();
     // at most 7 images per vertex (faces, edges, corner of its outside axes)
     const int nmax = 8 * no;
@@ -281,8 +455,9 @@ int fsi_forces(hipStream_t s, FsiBufs *f, int ns, const float4 *spos, const floa
     K.seeds = seeds_dev;
 #define UDX_FSI_LAUNCH(WR, EX)                                                                                  \
     do {                                                                                                        \
-        hipLaunchKernelGGL((k_fsi_vertex<WR, EX>), dim3(ceil_div(no, FSI_THREADS)), dim3(FSI_THREADS), 0, s, K, \
-                           no, opp, f->flag, f->aid, fo, pairs);                                                \
+        hipLaunchKernelGGL((k_fsi_instance<WR, EX>), dim3(ceil_div(nmax, FSI_THREADS)), dim3(FSI_THREADS), 0,  \
+                           s, K, f->cl.starts + nc, f->inst_sorted, f->meta_sorted, f->cl.order, f->sinst,      \
+                           pairs);                                                                              \
         if (ns > 0)                                                                                             \
             hipLaunchKernelGGL((k_fsi_solvent<WR, EX>), dim3(ceil_div(ns, FSI_THREADS)), dim3(FSI_THREADS), 0,  \
                                s, K, ns, f->inst_sorted, f->meta_sorted, f->cl.starts, fs, fs_stride);          \
@@ -300,6 +475,8 @@ int fsi_forces(hipStream_t s, FsiBufs *f, int ns, const float4 *spos, const floa
         else UDX_FSI_LAUNCH(WR_POW, false);
     }
 #undef UDX_FSI_LAUNCH
+    UDX_LAUNCHED();
+    hipLaunchKernelGGL(k_fsi_vertex_sum, dim3(ceil_div(no, 256)), dim3(256), 0, s, no, f->flag, f->aid, f->sinst, fo);
     UDX_LAUNCHED();
     return UDX_OK;
 }

@@ -495,20 +495,51 @@ __global__ __launch_bounds__(RD_THREADS) void k_rbc_cell_fid(int nv, int Lx, int
 }
 
 // new position of every cell: kept cells (fragment 26) first, then
-// fragments 0..25, ascending within each; one thread, nc is small
-__global__ void k_rbc_cell_order(int nc, const int *cfid, int *cnew) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int cnt[27];
-    for (int f = 0; f < 27; f++) cnt[f] = 0;
-    for (int c = 0; c < nc; c++) cnt[cfid[c]]++;
-    int start[27];
-    start[26] = 0;
-    int acc = cnt[26];
-    for (int f = 0; f < 26; f++) {
-        start[f] = acc;
-        acc += cnt[f];
+// fragments 0..25, ascending within each: one block; counts per fragment,
+// then ranks chunk by chunk from wave ballots
+constexpr int CO_THREADS = 1024;
+__global__ __launch_bounds__(CO_THREADS) void k_rbc_cell_order(int nc, const int *cfid, int *cnew) {
+    __shared__ int start[27];
+    __shared__ int wcnt[CO_THREADS / 64][27];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid < 27) start[tid] = 0;
+    __syncthreads();
+    for (int c = tid; c < nc; c += CO_THREADS) atomicAdd(&start[cfid[c]], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = start[26];
+        start[26] = 0;
+        for (int f = 0; f < 26; f++) {
+            const int k = start[f];
+            start[f] = acc;
+            acc += k;
+        }
     }
-    for (int c = 0; c < nc; c++) cnew[c] = start[cfid[c]]++;
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int base = 0; base < nc; base += CO_THREADS) {
+        const int c = base + tid;
+        const int f = c < nc ? cfid[c] : -1;
+        int rank = 0;
+        for (int k = 0; k < 27; k++) {
+            const unsigned long long m = __ballot(f == k);
+            if (lane == 0) wcnt[w][k] = __popcll(m);
+            if (f == k) rank = __popcll(m & below);
+        }
+        __syncthreads();
+        if (f >= 0) {
+            int before = 0;
+            for (int ww = 0; ww < w; ww++) before += wcnt[ww][f];
+            cnew[c] = start[f] + before + rank;
+        }
+        __syncthreads();
+        if (tid < 27) {
+            int tot = 0;
+            for (int ww = 0; ww < CO_THREADS / 64; ww++) tot += wcnt[ww][tid];
+            start[tid] += tot;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(256) void k_rbc_permute(int n, int nv, int Lx, int Ly, int Lz, const int *cfid,
@@ -533,7 +564,7 @@ int rbc_redistribute(hipStream_t s, const Box &b, int nc, int nv, float *pp, flo
     if (nc == 0) return UDX_OK;
     hipLaunchKernelGGL(k_rbc_cell_fid, dim3(nc), dim3(RD_THREADS), 0, s, nv, b.L[0], b.L[1], b.L[2], pp, cfid);
     UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_rbc_cell_order, dim3(1), dim3(64), 0, s, nc, cfid, cnew);
+    hipLaunchKernelGGL(k_rbc_cell_order, dim3(1), dim3(CO_THREADS), 0, s, nc, cfid, cnew);
     UDX_LAUNCHED();
     const int n = nc * nv;
     hipLaunchKernelGGL(k_rbc_permute, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, nv, b.L[0], b.L[1], b.L[2], cfid,
@@ -557,6 +588,8 @@ struct udx_rbc {
     int stress_free, stressful_set, bending;
     double *av;                  // device, 2 * max_cells
     double *avd, *gpb, *gp;      // adjoint scratch: 4 * max_cells, 5 per vertex block, 5
+    float *rd_tmp;               // redistribution scratch: 6 floats per vertex, 2 ints per cell
+    int *rd_fid, *rd_new;
 };
 
 namespace udx {
@@ -578,6 +611,9 @@ int udx_rbc_fin(udx_rbc *r) {
     dev_free(r->avd);
     dev_free(r->gpb);
     dev_free(r->gp);
+    dev_free(r->rd_tmp);
+    dev_free(r->rd_fid);
+    dev_free(r->rd_new);
     delete r;
     return UDX_OK;
 }
@@ -785,6 +821,21 @@ int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const flo
     UDX_HIP(hipMemcpyAsync(gp, r->gp, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
     UDX_HIP(hipStreamSynchronize(s));
     return UDX_OK;
+}
+
+int udx_rbc_distribute(udx_rbc *r, int Lx, int Ly, int Lz, int nc, float *pp) {
+    UDX_CHECK(r && (nc == 0 || pp), UDX_EINVAL, "udx_rbc_distribute: null argument");
+    UDX_CHECK(Lx > 0 && Ly > 0 && Lz > 0, UDX_EINVAL, "udx_rbc_distribute: bad box");
+    UDX_CHECK(nc >= 0 && nc <= r->max_cells, UDX_ECAP, "udx_rbc_distribute: nc = %d above max_cells = %d", nc,
+              r->max_cells);
+    if (nc == 0) return UDX_OK;
+    if (!r->rd_tmp) {
+        UDX_TRY(dev_alloc(&r->rd_tmp, 6 * (size_t)r->max_cells * r->nv));
+        UDX_TRY(dev_alloc(&r->rd_fid, (size_t)r->max_cells));
+        UDX_TRY(dev_alloc(&r->rd_new, (size_t)r->max_cells));
+    }
+    Box b{{Lx, Ly, Lz}};
+    return rbc_redistribute(r->ctx->stream, b, nc, r->nv, pp, r->rd_tmp, r->rd_fid, r->rd_new);
 }
 
 int udx_rbc_stretch_apply(udx_ctx *ctx, int nc, int nv, const float *f, float *ff) {

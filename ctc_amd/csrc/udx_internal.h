@@ -193,6 +193,7 @@ struct FsiBufs {
     int *flag, *aid, *scan_tmp;   // 26 * max_obj halo flags and their exclusive scan
     float *inst, *inst_sorted;    // vertex and image instances (AoS), 8 * max_obj
     int2 *meta, *meta_sorted;     // (noise index, image?)
+    float *sinst;                 // per-instance force sums, 3 * 8 * max_obj
     ClistBufs cl;                 // instances on the (L+2)^3 grid of cells -1..L
 };
 int fsi_alloc(FsiBufs *f, const Box &b, int max_obj);

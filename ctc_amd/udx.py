@@ -109,6 +109,7 @@ _SIGS = {
     "udx_rbc_stretch_apply": (_I, [_VP, _I, _I, _VP, _VP]),
     "udx_rbc_force_adjoint": (_I, [_VP, C.POINTER(RbcParams), _I, _VP, _VP, _VP, C.POINTER(C.c_double)]),
     "udx_flu_set_rbc": (_I, [_VP, _VP, _VP, C.POINTER(RbcParams), C.POINTER(PairDpd), _F, _I, _I, _VP]),
+    "udx_rbc_distribute": (_I, [_VP, _I, _I, _I, _I, _VP]),
     "udx_fsi_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
     "udx_fsi_fin": (_I, [_VP]),
     "udx_fsi_set_mode": (_I, [_VP, _I]),
@@ -585,6 +586,10 @@ class Rbc:
 
     def force(self, params, dt, nc, pp, ff):
         _ok(lib().udx_rbc_force_apply(self.h, C.byref(params), float(dt), nc, _ptr(pp), _ptr(ff)))
+
+    def distribute(self, L, nc, pp):
+        """whole-cell periodic redistribution of pp (device, in place)"""
+        _ok(lib().udx_rbc_distribute(self.h, L[0], L[1], L[2], nc, _ptr(pp)))
 
     def adjoint(self, params, nc, pp, g, gx):
         """gx = (dF/dx)^T g; returns [dL/dka, dL/dkd, dL/dkv, dL/dks, dL/dkb]"""

@@ -221,6 +221,14 @@ int udx_rbc_force_apply(udx_rbc *r, const udx_rbc_params *p, float dt, int nc, c
  * from the forward call: the geometry is recomputed. */
 int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const float *pp, const float *g,
                           float *gx, double gp[5]);
+/* whole-cell periodic redistribution on one domain (drbc_build_map / pack /
+ * unpack, src/distr/rbc/imp/map.h, src/distr/rbc/dev.h:3-20,
+ * src/distr/map/dev.h:17-26, src/distr/rbc/imp/unpack.h:30-51): a cell whose
+ * bounding-box centre left [-L/2, L/2) is shifted by -L d and moved after
+ * the kept cells; kept cells first, then fragments 0..25, ascending within
+ * each (the reference's order within a fragment comes from atomics).
+ * pp: device AoS, in place. */
+int udx_rbc_distribute(udx_rbc *r, int Lx, int Ly, int Lz, int nc, float *pp);
 /* ff[i] += f[i % nv] for nc * nv vertices (f: device, 3 * nv) */
 int udx_rbc_stretch_apply(udx_ctx *ctx, int nc, int nv, const float *f, float *ff);
 

@@ -16,8 +16,8 @@
  * in atomic order; the order here is defined (and the HIP path follows it):
  *  - halo list: receive fragment i = 0..25 (the vertices outside the domain
  *    toward -d_i, shifted by +d_i L), vertices ascending within a fragment;
- *  - vertex force: += zplane 0, 1, 2 partial sums of the bulk pass, then
- *    += each image's sum in halo order;
+ *  - vertex force: += the vertex's own sum (z planes 0, 1, 2, rows y - 1..
+ *    y + 1, solvent in cell order), then += each image's sum in halo order;
  *  - solvent force: += one sum over the interacting instances (vertices,
  *    then images) in the order of their cells on the (L+2)^3 grid of cells
  *    [-1, L] (dz, dy, dx around the solvent's cell), ascending instance
@@ -124,18 +124,11 @@ long orc_fsi_forces(const int L[3], int ns, const float *spp, const int *starts,
     int ni = build_instances(L, no, opp, &inst);
     long npairs = 0;
     (void)ns;
-    /* vertex side */
-    for (int v = 0; v < no; v++) {
-        float *o = fo + 3 * v;
-        for (int zp = 0; zp < 3; zp++) {
-            float s[3] = {0, 0, 0};
-            npairs += inst_sum(L, spp, starts, &inst[v], p, seeds[0], zp, s);
-            o[0] += s[0]; o[1] += s[1]; o[2] += s[2];
-        }
-    }
-    for (int k = no; k < ni; k++) {
+    /* vertex side: one sum per instance over its three z planes; the vertex
+     * itself first, then its images (instances no.. are in halo order) */
+    for (int k = 0; k < ni; k++) {
         float s[3] = {0, 0, 0};
-        for (int zp = 0; zp < 3; zp++) npairs += inst_sum(L, spp, starts, &inst[k], p, seeds[1], zp, s);
+        for (int zp = 0; zp < 3; zp++) npairs += inst_sum(L, spp, starts, &inst[k], p, seeds[k < no ? 0 : 1], zp, s);
         float *o = fo + 3 * inst[k].vertex;
         o[0] += s[0]; o[1] += s[1]; o[2] += s[2];
     }

@@ -143,3 +143,29 @@ def test_suspension_rejects_bad_calls(ctx):
         flu.set_rbc(r, fsi, U.rbc_params(**kw), U.PairDpd(2.6, 8.0, 1.0, 0.25), 0.5, 1, 2, dev(opp))
     flu.close()
     r.close()
+
+
+@pytest.mark.parametrize("nc", [7, 1500])
+def test_distribute_matches_oracle(ctx, nc):
+    """udx_rbc_distribute against the oracle's redistribution: random cell
+    centres in and around the box (every direction), more cells than one
+    1024-thread chunk of the ordering kernel; bit-identical output."""
+    L = (16, 12, 20)
+    rr, tt = R.read_off(os.path.join(GOLD, "cells", "498.off"))
+    m = R.Mesh(rr, tt)
+    rng = np.random.default_rng(nc)
+    T = []
+    for _ in range(nc):
+        A = np.eye(4)
+        A[:3, 3] = rng.uniform(-0.75, 0.75, 3) * np.array(L)
+        T.append(A)
+    pp = R.place(m.rr, T, (0, 0, 0))
+    pp[:, 3:] = rng.normal(0, 1, (len(pp), 3)).astype(np.float32)
+    want, fid = R.redistribute(L, pp, nc, m.nv)
+    assert (fid != 26).sum() > 0 and (fid == 26).sum() > 0
+    r = U.Rbc(ctx, m.tt, m.nv, nc)
+    d = dev(pp)
+    r.distribute(L, nc, d)
+    got = host(d)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    r.close()
