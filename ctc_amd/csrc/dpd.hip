@@ -27,7 +27,9 @@ namespace {
 // 128 threads: measured 5% faster per step than 256 on the 96^3, density-10 box
 constexpr int DPD_THREADS = 128;
 constexpr int QCAP = 24;   // queued pairs per lane before a drain
-constexpr int SCAP = 160;  // staged candidates per wave and row
+// staged candidates per wave and row (320 for batches of small boxes measured
+// 4% slower than 160: more staging loads, fewer blocks per CU)
+constexpr int SCAP = 160, SCAP_BATCH = 160;
 constexpr int DRAIN_STEPS = 2;  // pair steps per lane when a queue is full
 
 struct DpdK {
@@ -131,10 +133,10 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 // The force pass of one block of NT consecutive particles starting at i
 // = first + threadIdx.x; QC queued pairs per lane before a drain.
-template <int WR, bool EXACT, int NT, int QC>
+template <int WR, bool EXACT, int NT, int QC, int SC>
 __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     __shared__ unsigned q[QC * NT];
-    __shared__ float4 sbuf[NT / 64][SCAP + 4];
+    __shared__ float4 sbuf[NT / 64][SC + 4];
     __shared__ unsigned long long lpairs[NT / 64];
     constexpr int DPD_THREADS = NT, QCAP = QC;
     const int tid = threadIdx.x;
@@ -215,7 +217,8 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     // is the same for every lane.  The union of the wave's runs of a row is
     // one contiguous index range when no lane's row wraps (the lanes are
     // consecutive cell-ordered particles): it is staged in LDS with coalesced
-    // loads and tested four candidates per iteration; otherwise the wave reads
+    // loads and tested four candidates per iteration; a wave with wrapped runs
+    // is staged the same way when their union fits; otherwise the wave reads
     // its candidates from global memory.
     float4 *sb = sbuf[tid >> 6];
     for (int row = 0; row < 9; row++) {
@@ -239,12 +242,17 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         }
         const bool simple = !rowok || ((wy | wz | wx0) == 0 && n1 == 0);
         int lo = rowok ? j0 : 0x7fffffff, hi = rowok ? j0 + n0 : 0;
+        if (rowok && n1 > 0) {
+            lo = min(lo, j1);
+            hi = max(hi, j1 + n1);
+        }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
             lo = min(lo, __shfl_xor(lo, d, 64));
             hi = max(hi, __shfl_xor(hi, d, 64));
         }
-        const bool lds_row = !wave_any(!simple) && hi > lo && hi - lo <= SCAP;
+        const bool fits = hi > lo && hi - lo <= SC;
+        const bool lds_row = fits && !wave_any(!simple);
         if (lds_row) {
             __builtin_amdgcn_wave_barrier();
             for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = A.pos[lo + k];
@@ -265,6 +273,35 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
                 UDX_TEST(c2, 2)
                 UDX_TEST(c3, 3)
 #undef UDX_TEST
+            }
+            __builtin_amdgcn_wave_barrier();
+        } else if (fits) {
+            // some lane's row wraps, but the wave's runs (a wrapped x run lies
+            // at the other end of the same x row; rows wrapped in y or z are
+            // shared by the lanes of one y row) still fit the staging buffer:
+            // staged as above, each candidate taking its run's shift
+            __builtin_amdgcn_wave_barrier();
+            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = A.pos[lo + k];
+            __builtin_amdgcn_wave_barrier();
+            const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
+            const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
+            const unsigned w0 = wcode(wx0, wy, wz), w1 = wcode(wx1, wy, wz);
+            const int len = n0 + n1, o0 = j0 - lo, o1 = j1 - lo;
+            for (int t = 0; wave_any(t < len); t += 4) {
+                make_room(4);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int tt = t + u;
+                    const bool s1 = tt >= n0;
+                    const int idx = s1 ? o1 + (tt - n0) : o0 + tt;
+                    const bool ok = tt < len;
+                    const float4 c = sb[ok ? idx : 0];
+                    const float ax = s1 ? ax1 : ax0;
+                    const float ddx = ax - c.x, ddy = ay - c.y, ddz = az - c.z;
+                    const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));
+                    const int j = lo + idx;
+                    push(ok && j != i && r2 < 1.0f, j, s1 ? w1 : w0);
+                }
             }
             __builtin_amdgcn_wave_barrier();
         } else {
@@ -321,15 +358,18 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
 }
 
 template <int WR, bool EXACT>
-__global__ __launch_bounds__(DPD_THREADS) void k_dpd(DpdK A) {
-    dpd_block<WR, EXACT, DPD_THREADS, QCAP>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+// at most 128 VGPRs: four waves per SIMD
+__global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_dpd(DpdK A) {
+    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
 }
 
 // Many boxes in one launch: block b belongs to the box k with
 // bblock[k] <= b < bblock[k + 1]; each box's blocks start at its first
 // particle, so no block spans two boxes.
 template <int WR, bool EXACT, int NT, int QC>
-__global__ __launch_bounds__(NT) void k_dpd_batch(const DpdK *boxes, const int *bblock, int nbox) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_dpd_batch(const DpdK *boxes,
+                                                                                           const int *bblock,
+                                                                                           int nbox) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     int lo = 0, hi = nbox;
     while (hi - lo > 1) {
@@ -338,7 +378,7 @@ __global__ __launch_bounds__(NT) void k_dpd_batch(const DpdK *boxes, const int *
         else hi = mid;
     }
     const DpdK &A = boxes[lo];
-    dpd_block<WR, EXACT, NT, QC>(A, A.i0 + (blk - bblock[lo]) * NT);
+    dpd_block<WR, EXACT, NT, QC, SCAP_BATCH>(A, A.i0 + (blk - bblock[lo]) * NT);
 }
 
 // Counts of the cells of the 26 fragments in fragment-cell order (x
