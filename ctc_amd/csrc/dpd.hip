@@ -241,18 +241,33 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             }
         }
         const bool simple = !rowok || ((wy | wz | wx0) == 0 && n1 == 0);
-        int lo = rowok ? j0 : 0x7fffffff, hi = rowok ? j0 + n0 : 0;
-        if (rowok && n1 > 0) {
-            lo = min(lo, j1);
-            hi = max(hi, j1 + n1);
-        }
+        const bool all_simple = !wave_any(!simple);
+        int lo = 0x7fffffff, hi = 0;
+        if (all_simple) {
+            // unwrapped runs of consecutive cell-ordered particles grow with
+            // the lane: the first and the last lane with a row bound them
+            const unsigned long long act = __ballot(rowok);
+            if (act) {
+                lo = __shfl(j0, __ffsll(act) - 1, 64);
+                hi = __shfl(j0 + n0, 63 - __clzll(act), 64);
+            }
+        } else {
+            if (rowok) {
+                lo = j0;
+                hi = j0 + n0;
+                if (n1 > 0) {
+                    lo = min(lo, j1);
+                    hi = max(hi, j1 + n1);
+                }
+            }
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            lo = min(lo, __shfl_xor(lo, d, 64));
-            hi = max(hi, __shfl_xor(hi, d, 64));
+            for (int d = 32; d >= 1; d >>= 1) {
+                lo = min(lo, __shfl_xor(lo, d, 64));
+                hi = max(hi, __shfl_xor(hi, d, 64));
+            }
         }
         const bool fits = hi > lo && hi - lo <= SC;
-        const bool lds_row = fits && !wave_any(!simple);
+        const bool lds_row = fits && all_simple;
         if (lds_row) {
             __builtin_amdgcn_wave_barrier();
             for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = A.pos[lo + k];
