@@ -182,55 +182,115 @@ __global__ __launch_bounds__(256) void k_image_scatter(int nw, const float *wall
 // ------------------------------------------------------------ wall forces
 // solvent x frozen wall particles (src/wall/force/dev/main.h:1-50), one
 // thread per cell-ordered solvent particle, 3x3x3 wall cells around it on
-// the margin box; wall velocity 0; noise mean0var1ii(seed, solvent, wall)
+// the margin box; wall velocity 0; noise mean0var1ii(seed, solvent, wall).
+// Hits are queued per lane in an LDS ring and drained two pairs per lane per
+// step (in candidate order, so the sum is that of a sequential walk): tested
+// inline, about one candidate in six passes and the whole wave would wait
+// for the pair body at nearly every candidate.
+constexpr int WALL_THREADS = 128, WQ = 16;
 template <int WR, bool EXACT>
-__global__ __launch_bounds__(256) void k_wall_force(SdfView v, int nmax, const int *n_dev, const float4 *pos,
-                                                    const float4 *vel, const float4 *wpos, const int *wstarts,
-                                                    int Wx, int Wy, int Wz, DpdPar par, const float *seedp,
-                                                    float *ff, int ff_stride, unsigned long long *pairs) {
+__global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, int nmax, const int *n_dev, const float4 *pos,
+                                                             const float4 *vel, const float4 *wpos,
+                                                             const int *wstarts, int Wx, int Wy, int Wz, DpdPar par,
+                                                             const float *seedp, float *ff, int ff_stride,
+                                                             unsigned long long *pairs) {
+    __shared__ unsigned q[WQ * WALL_THREADS];
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     int npair = 0;
+    bool near = false;
+    float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), va = pa;
     if (a < nmax && a < *n_dev) {
-        const float4 pa = pos[a];
-        if (!sdf_far(v, pa.x, pa.y, pa.z)) {
-            const float4 va = vel[a];
-            const float seed = *seedp;
-            int b0[3];
-            const float pc[3] = {pa.x, pa.y, pa.z};
-            const int W[3] = {Wx, Wy, Wz};
-            for (int c = 0; c < 3; c++) {
-                const int q = (int)__builtin_floor((double)pc[c] + 0.5 * (double)W[c]);
-                b0[c] = q < 1 ? 1 : (q > W[c] - 2 ? W[c] - 2 : q);
+        pa = pos[a];
+        near = !sdf_far(v, pa.x, pa.y, pa.z);
+    }
+    if (!wave_any(near)) return;
+    if (near) va = vel[a];
+    const float seed = *seedp;
+    int b0[3] = {1, 1, 1};
+    if (near) {
+        const float pc[3] = {pa.x, pa.y, pa.z};
+        const int W[3] = {Wx, Wy, Wz};
+        for (int c = 0; c < 3; c++) {
+            const int qq = (int)__builtin_floor((double)pc[c] + 0.5 * (double)W[c]);
+            b0[c] = qq < 1 ? 1 : (qq > W[c] - 2 ? W[c] - 2 : qq);
+        }
+    }
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    int fill = 0, head = 0;
+    unsigned *ql = q + threadIdx.x;
+    auto ring = [&](int k) {
+        const int r = head + k;
+        return r >= WQ ? r - WQ : r;
+    };
+    const float fa_lo = (float)(a & 0x3FF), fa_hi = (float)a;
+    auto drain_step = [&]() {
+        const bool ok0 = fill > 0, ok1 = fill > 1;
+        const unsigned e0 = ok0 ? ql[ring(0) * WALL_THREADS] : 0u;   // wall particle 0 exists
+        const unsigned e1 = ok1 ? ql[ring(1) * WALL_THREADS] : e0;
+        const float4 w0 = wpos[e0], w1 = wpos[e1];
+        const float dx0 = pa.x - w0.x, dy0 = pa.y - w0.y, dz0 = pa.z - w0.z;
+        const float dx1 = pa.x - w1.x, dy1 = pa.y - w1.y, dz1 = pa.z - w1.z;
+        const float r20 = __builtin_fmaf(dz0, dz0, __builtin_fmaf(dx0, dx0, dy0 * dy0));
+        const float r21 = __builtin_fmaf(dz1, dz1, __builtin_fmaf(dx1, dx1, dy1 * dy1));
+        // mean0var1ii(seed, a, b) (src/math/rnd/dev.h:57-63), two chains together
+        float x0 = seed - low_discrepancy(fa_lo, fa_hi, (float)((int)e0 & 0x3FF), (float)(int)e0);
+        float x1 = seed - low_discrepancy(fa_lo, fa_hi, (float)((int)e1 & 0x3FF), (float)(int)e1);
+        logistic18x2(&x0, &x1);
+        const float3 f0 = dpd_pair<WR, EXACT>(par, dx0, dy0, dz0, r20, va.x, va.y, va.z, x0 * R_SQRT2);
+        const float3 f1 = dpd_pair<WR, EXACT>(par, dx1, dy1, dz1, r21, va.x, va.y, va.z, x1 * R_SQRT2);
+        if (ok0) {
+            fx += f0.x;
+            fy += f0.y;
+            fz += f0.z;
+        }
+        if (ok1) {
+            fx += f1.x;
+            fy += f1.y;
+            fz += f1.z;
+        }
+        const int took = fill < 2 ? fill : 2;
+        npair += took;
+        head = ring(took);
+        fill -= took;
+    };
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            int c0 = 0, len = 0;
+            if (near) {
+                c0 = (b0[0] - 1) + Wx * ((b0[1] + dy) + Wy * (b0[2] + dz));
+                c0 = wstarts[c0];
+                len = wstarts[(b0[0] - 1) + Wx * ((b0[1] + dy) + Wy * (b0[2] + dz)) + 3] - c0;
             }
-            float fx = 0.f, fy = 0.f, fz = 0.f;
-            for (int dz = -1; dz <= 1; dz++)
-                for (int dy = -1; dy <= 1; dy++) {
-                    const int c0 = (b0[0] - 1) + Wx * ((b0[1] + dy) + Wy * (b0[2] + dz));
-                    const int e = wstarts[c0 + 3];
-                    for (int b = wstarts[c0]; b < e; b++) {
-                        const float4 pw = wpos[b];
-                        const float dx = pa.x - pw.x, dyy = pa.y - pw.y, dzz = pa.z - pw.z;
-                        const float r2 = __builtin_fmaf(dzz, dzz, __builtin_fmaf(dx, dx, dyy * dyy));
-                        if (r2 < 1.0f) {
-                            const float rnd = mean0var1ii(seed, a, b);
-                            const float3 f = dpd_pair<WR, EXACT>(par, dx, dyy, dzz, r2, va.x, va.y, va.z, rnd);
-                            fx += f.x;
-                            fy += f.y;
-                            fz += f.z;
-                            npair++;
-                        }
+            for (int t = 0; wave_any(t < len); t += 4) {
+                if (wave_any(fill + 4 > WQ))
+#pragma unroll 1
+                    for (int k = 0; k < 2; k++) drain_step();
+                float4 pw[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    pw[u] = t + u < len ? wpos[c0 + t + u] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float dx = pa.x - pw[u].x, dyy = pa.y - pw[u].y, dzz = pa.z - pw[u].z;
+                    const float r2 = __builtin_fmaf(dzz, dzz, __builtin_fmaf(dx, dx, dyy * dyy));
+                    if (t + u < len && r2 < 1.0f) {
+                        ql[ring(fill) * WALL_THREADS] = (unsigned)(c0 + t + u);
+                        fill++;
                     }
                 }
-            if (ff_stride == 4) {
-                float4 *f4 = (float4 *)ff + a;
-                const float4 o = *f4;
-                *f4 = make_float4(o.x + fx, o.y + fy, o.z + fz, o.w);
-            } else {
-                float *f3 = ff + 3 * (long)a;
-                f3[0] += fx;
-                f3[1] += fy;
-                f3[2] += fz;
             }
+        }
+    while (wave_any(fill > 0)) drain_step();
+    if (near) {
+        if (ff_stride == 4) {
+            float4 *f4 = (float4 *)ff + a;
+            const float4 o = *f4;
+            *f4 = make_float4(o.x + fx, o.y + fy, o.z + fz, o.w);
+        } else {
+            float *f3 = ff + 3 * (long)a;
+            f3[0] += fx;
+            f3[1] += fy;
+            f3[2] += fz;
         }
     }
     if (pairs) {
@@ -453,7 +513,7 @@ int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, co
     if (nmax <= 0 || w->n <= 0) return UDX_OK;
     const Box &b = w->cl.box;
     DpdPar par{p.a, p.g, p.s, p.spow};
-    const dim3 grid(ceil_div(nmax, 256)), blk(256);
+    const dim3 grid(ceil_div(nmax, WALL_THREADS)), blk(WALL_THREADS);
 #define UDX_WALL_LAUNCH(WR, EX)                                                                              \
     hipLaunchKernelGGL((k_wall_force<WR, EX>), grid, blk, 0, s, q->view, nmax, n_dev, pos, vel, w->pos,       \
                        w->cl.starts, b.L[0], b.L[1], b.L[2], par, seed_dev, ff, ff_stride, pairs)
