@@ -145,10 +145,36 @@ __global__ __launch_bounds__(256) void k_clist_scatter(int n, const int2 *cellsl
 }
 
 // sort each cell's input indices ascending: the stable order
+// A cell of at most 16 particles is sorted in registers by a bitonic network
+// (one load and one store per entry); larger cells by insertion in memory.
 __global__ __launch_bounds__(256) void k_clist_cellsort(int ncells, const int *starts, int *order) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     int s = starts[c], e = starts[c + 1];
+    if (e - s <= 1) return;
+    if (e - s <= 16) {
+        int v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = s + k < e ? order[s + k] : 0x7fffffff;
+#pragma unroll
+        for (int size = 2; size <= 16; size <<= 1)
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+                for (int k = 0; k < 16; k++) {
+                    const int j = k ^ stride;
+                    if (j > k) {
+                        const int lo = min(v[k], v[j]), hi = max(v[k], v[j]);
+                        const bool up = (k & size) == 0;
+                        v[k] = up ? lo : hi;
+                        v[j] = up ? hi : lo;
+                    }
+                }
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (s + k < e) order[s + k] = v[k];
+        return;
+    }
     for (int k = s + 1; k < e; k++) {
         int v = order[k], m = k - 1;
         while (m >= s && order[m] > v) {
