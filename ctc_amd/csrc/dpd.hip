@@ -205,9 +205,15 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         while (wave_any(fill > 0)) drain_step();
     };
 
+    // the write slot head + fill (mod QCAP) is kept as an LDS pointer: a
+    // drain moves head and fill together and leaves it in place
+    unsigned *wq = qlane;
+    unsigned *const qend = qlane + QCAP * DPD_THREADS;
     auto push = [&](bool hit, int j, unsigned w) {
         if (hit) {
-            qlane[ring(fill) * DPD_THREADS] = (unsigned)j | (w << 27);
+            *wq = (unsigned)j | (w << 27);
+            wq += DPD_THREADS;
+            if (wq == qend) wq = qlane;
             fill++;
         }
     };
