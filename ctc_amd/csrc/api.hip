@@ -105,6 +105,8 @@ struct udx_wall {
     float *seed_dev;
     float4 *pos, *vel;          // AoS -> float4 staging for udx_wall_force
     unsigned long long *pairs_dev;
+    udx_wall_vel wvel;          // resting unless udx_wall_set_vel (src/sim/walls/imp/main.h:114-116)
+    float t;                    // time of the "shear sin" rate
 };
 
 static const SdfField *wall_sdf(const udx_wall *w) { return &w->sdf->f; }
@@ -125,6 +127,7 @@ struct udx_flu {
     float *seeds_host;  // pinned staging
     unsigned long long *pairs_dev;   // [0] ordered solvent pairs, [1] solvent-wall pairs
     struct udx_wall *wall;
+    float t;            // step clock, float like the reference's (src/scheme/time_line/imp/type.h:1-5)
     int timing;
     hipEvent_t ev[5];
     float tms[5];
@@ -508,11 +511,15 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
     UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
     const SdfField *sdf = nullptr;
+    WallVel wv{};
     if (f->wall) {
+        // wall_update_vel(t - tstart) at the start of the step (src/sim/imp/run.h:17-18)
+        f->wall->t = f->t;
+        wv = wall_vel_view(f->wall->wvel, f->t);
         sdf = wall_sdf(f->wall);
         UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
                             f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1,
-                            f->conf.mode == UDX_MODE_EXACT));
+                            f->conf.mode == UDX_MODE_EXACT, wv));
     }
     if (m.r && nvtx > 0) {
         // FSI: +f on the vertices, -f on the solvent (src/sim/objinter/imp/main.h:42-91)
@@ -522,7 +529,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
     }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
-                         f->pos[b], f->vel[b], sdf));
+                         f->pos[b], f->vel[b], sdf, wv));
     if (m.r && nvtx > 0) {
         // objects_advance (src/sim/objects/imp/main.h:40-61): per sub-step the
         // fast force starts as the FSI force, the membrane force is added and
@@ -536,6 +543,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
     }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[4], s));
     f->cur = b;
+    f->t += f->conf.dt;   // time_line_advance (src/scheme/time_line/imp/main.h:10-21)
     return UDX_OK;
 }
 
@@ -718,7 +726,7 @@ int udx_wall_force_seed(udx_wall *w, const udx_pair_dpd *p, int n, const float *
     if (hipMemcpyAsync(n_dev, &n, sizeof n, hipMemcpyHostToDevice, s) != hipSuccess) st = UDX_EHIP;
     if (!st)
         st = wall_forces(s, &w->sdf->f, &w->w, n, n_dev, w->pos, w->vel, *p, w->seed_dev, ff, 3, w->pairs_dev,
-                         w->mode == UDX_MODE_EXACT);
+                         w->mode == UDX_MODE_EXACT, wall_vel_view(w->wvel, w->t));
     if (!st && hipStreamSynchronize(s) != hipSuccess) st = UDX_EHIP;
     dev_free(n_dev);
     if (st) set_error("udx_wall_force: launch failed");
@@ -740,7 +748,35 @@ int udx_wall_pairs(udx_wall *w, unsigned long long *npairs, int reset) {
 
 int udx_wall_bounce(const udx_sdf *q, float dt, int n, float *pp) {
     UDX_CHECK(q && q->f.ready, UDX_EINVAL, "udx_wall_bounce: sdf not generated");
-    return wall_bounce_aos(q->ctx->stream, &q->f, dt, n, pp);
+    return wall_bounce_aos(q->ctx->stream, &q->f, dt, n, pp, WallVel{});
+}
+
+static int wall_vel_check(const udx_wall_vel *v) {
+    UDX_CHECK(v, UDX_EINVAL, "udx_wall_set_vel: null argument");
+    UDX_CHECK(v->type >= UDX_WVEL_NONE && v->type <= UDX_WVEL_HS, UDX_EINVAL, "udx_wall_set_vel: type %d", v->type);
+    if (v->type == UDX_WVEL_SHEAR || v->type == UDX_WVEL_SHEAR_SIN)
+        UDX_CHECK(v->vdir >= 0 && v->vdir < 3 && v->gdir >= 0 && v->gdir < 3, UDX_EINVAL,
+                  "udx_wall_set_vel: vdir %d, gdir %d", v->vdir, v->gdir);
+    if (v->type == UDX_WVEL_HS) UDX_CHECK(v->hs_h > 0, UDX_EINVAL, "udx_wall_set_vel: Hele-Shaw height %g", v->hs_h);
+    return UDX_OK;
+}
+
+int udx_wall_set_vel(udx_wall *w, const udx_wall_vel *v) {
+    UDX_CHECK(w, UDX_EINVAL, "udx_wall_set_vel: null handle");
+    UDX_TRY(wall_vel_check(v));
+    w->wvel = *v;
+    return UDX_OK;
+}
+
+int udx_wall_set_time(udx_wall *w, float t) {
+    UDX_CHECK(w, UDX_EINVAL, "udx_wall_set_time: null handle");
+    w->t = t;
+    return UDX_OK;
+}
+
+int udx_wall_bounce_wall(const udx_wall *w, float dt, int n, float *pp) {
+    UDX_CHECK(w && n >= 0 && (n == 0 || pp), UDX_EINVAL, "udx_wall_bounce_wall: bad arguments");
+    return wall_bounce_aos(w->ctx->stream, wall_sdf(w), dt, n, pp, wall_vel_view(w->wvel, w->t));
 }
 
 int udx_wall_repulse(const udx_sdf *q, float lambda, int n, const float *pp, float *ff) {
@@ -754,6 +790,12 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
               UDX_EINVAL, "udx_flu_set_wall: wall and solvent boxes differ");
     UDX_CHECK(!w || !f->mb.r, UDX_EINVAL, "udx_flu_set_wall: membranes together with walls are not supported");
     f->wall = w;
+    return UDX_OK;
+}
+
+int udx_flu_set_time(udx_flu *f, float t) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_time: null handle");
+    f->t = t;
     return UDX_OK;
 }
 

@@ -517,4 +517,33 @@ int udx_rbc_params_set_conf(const udx_conf *c, const char *ns, udx_rbc_params *p
     return UDX_OK;
 }
 
+// wvel.type and its keys: "constant" u; "shear" gdot, vdir, gdir, half;
+// "shear sin" the same and w; "hele shaw" u, h (src/wall/wvel/conf.cpp:10-47)
+int udx_wall_vel_set_conf(const udx_conf *c, udx_wall_vel *v) {
+    UDX_CHECK(c && v, UDX_EINVAL, "udx_wall_vel_set_conf: null argument");
+    const char *type;
+    UDX_TRY(udx_conf_lookup_string(c, "wvel.type", &type));
+    udx_wall_vel w{};
+    const std::string t(type);
+    if (t == "constant") {
+        w.type = UDX_WVEL_CSTE;
+        UDX_TRY(udx_conf_lookup_float3(c, "wvel.u", w.u));
+    } else if (t == "shear" || t == "shear sin") {
+        w.type = t == "shear" ? UDX_WVEL_SHEAR : UDX_WVEL_SHEAR_SIN;
+        UDX_TRY(udx_conf_lookup_float(c, "wvel.gdot", &w.gdot));
+        UDX_TRY(udx_conf_lookup_int(c, "wvel.vdir", &w.vdir));
+        UDX_TRY(udx_conf_lookup_int(c, "wvel.gdir", &w.gdir));
+        UDX_TRY(udx_conf_lookup_int(c, "wvel.half", &w.half));
+        if (w.type == UDX_WVEL_SHEAR_SIN) UDX_TRY(udx_conf_lookup_float(c, "wvel.w", &w.w));
+    } else if (t == "hele shaw") {
+        w.type = UDX_WVEL_HS;
+        UDX_TRY(udx_conf_lookup_float(c, "wvel.u", &w.hs_u));
+        UDX_TRY(udx_conf_lookup_float(c, "wvel.h", &w.hs_h));
+    } else {
+        UDX_CHECK(false, UDX_EINVAL, "unknown type <%s>", type);
+    }
+    *v = w;
+    return UDX_OK;
+}
+
 }  // extern "C"

@@ -30,7 +30,8 @@ __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, flo
 // src/sim/imp/run.h:43-54)
 __global__ __launch_bounds__(256) void k_move_f4_wall(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
                                                       float dt, float mass, float ax, float ay, float az,
-                                                      const float4 *ff, float4 *pos, float4 *vel, SdfView sdf) {
+                                                      const float4 *ff, float4 *pos, float4 *vel, SdfView sdf,
+                                                      WallVel wv) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nmax || i >= *n_dev) return;
     float4 p = pos[i], v = vel[i];
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(256) void k_move_f4_wall(int nmax, const int *n_dev
     kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
     kick_drift(dt, mass, az, f.z, &p.z, &v.z);
     float r[3] = {p.x, p.y, p.z}, u[3] = {v.x, v.y, v.z};
-    if (sdf_bounce(sdf, dt, r, u)) {
+    if (sdf_bounce(sdf, wv, dt, r, u)) {
         p.x = r[0]; p.y = r[1]; p.z = r[2];
         v.x = u[0]; v.y = u[1]; v.z = u[2];
     }
@@ -196,11 +197,12 @@ int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, f
 }
 
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q) {
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv) {
     if (!q) return move_f4(s, nmax, n_dev, b, dt, mass, accel, ff, pos, vel);
     if (nmax <= 0) return UDX_OK;
     hipLaunchKernelGGL(k_move_f4_wall, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
-                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, q->view);
+                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, q->view,
+                       wv);
     UDX_LAUNCHED();
     return UDX_OK;
 }

@@ -273,9 +273,50 @@ __device__ __forceinline__ bool sdf_far(const SdfView &v, float x, float y, floa
     return sdf_cheap(v, x, y, z) <= v.cheap_threshold - 1;
 }
 
-// bounce-back with a resting wall (src/wall/sdf/bounce/dev/main.h:1-114);
-// returns true when the particle was inside and has been moved
-__device__ __forceinline__ bool sdf_bounce(const SdfView &v, float dt, float r[3], float u[3]) {
+// wall velocity at r (src/wall/wvel/dev.h:1-56): resting, constant, shear
+// (a time-dependent rate resolved on the host) or Hele-Shaw.  On one domain
+// the centre coordinates are the local ones (src/coords/dev.h:17-33).
+enum { WVEL_NONE = 0, WVEL_CSTE = 1, WVEL_SHEAR = 2, WVEL_HS = 3 };
+struct WallVel {
+    int type;
+    float u[3];
+    float gdot;
+    int vdir, gdir, half;
+    float hu, hh;
+};
+
+__device__ __forceinline__ void wvel_at(const WallVel &w, float x, float y, float z, float v[3]) {
+    v[0] = v[1] = v[2] = 0.f;
+    if (w.type == WVEL_CSTE) {
+        v[0] = w.u[0];
+        v[1] = w.u[1];
+        v[2] = w.u[2];
+    } else if (w.type == WVEL_SHEAR) {
+        float d = w.gdir == 0 ? x : (w.gdir == 1 ? y : z);
+        if (w.half) d = d < 0.f ? d : 0.f;
+        const float s = d * w.gdot;
+        if (w.vdir == 0) v[0] = s;
+        else if (w.vdir == 1) v[1] = s;
+        else v[2] = s;
+    } else if (w.type == WVEL_HS) {
+        const float r2inv = 1.f / (x * x + y * y);
+        float hfac = w.hu * (1.f - z * z * 4 / (w.hh * w.hh));
+        hfac = hfac > 0.f ? hfac : 0.f;
+        v[0] = r2inv * hfac * x;
+        v[1] = r2inv * hfac * y;
+    }
+}
+
+// v = 2 u_wall(rw) - v (src/wall/wvel/dev.h:57-66)
+__device__ __forceinline__ void bounce_vel(const WallVel &w, const float rw[3], float u[3]) {
+    float vw[3];
+    wvel_at(w, rw[0], rw[1], rw[2], vw);
+    for (int c = 0; c < 3; c++) u[c] = 2.0f * vw[c] - u[c];
+}
+
+// bounce-back (src/wall/sdf/bounce/dev/main.h:1-114); returns true when the
+// particle was inside and has been moved
+__device__ __forceinline__ bool sdf_bounce(const SdfView &v, const WallVel &wv, float dt, float r[3], float u[3]) {
     if (sdf_cheap(v, r[0], r[1], r[2]) < v.cheap_threshold) return false;
     const float currsdf = sdf_eval(v, r[0], r[1], r[2]);
     if (!(currsdf >= 0)) return false;
@@ -292,7 +333,7 @@ __device__ __forceinline__ bool sdf_bounce(const SdfView &v, float dt, float r[3
         axpy(-sdf0, g, r);
         for (int l = 8; l >= 1; --l) {
             if (sdf_eval(v, r[0], r[1], r[2]) < 0) {
-                for (int c = 0; c < 3; c++) u[c] = 0.0f - u[c];
+                bounce_vel(wv, r, u);
                 return true;
             }
             const float jump = 1.1f * sdf0 / (float)(1 << l);
@@ -315,7 +356,7 @@ __device__ __forceinline__ bool sdf_bounce(const SdfView &v, float dt, float r[3
     }
     float rw[3] = {r[0], r[1], r[2]};
     axpy(t, u, rw);
-    for (int c = 0; c < 3; c++) u[c] = 0.0f - u[c];
+    bounce_vel(wv, rw, u);
     r[0] = rw[0]; r[1] = rw[1]; r[2] = rw[2];
     axpy(-t, u, r);
     if (sdf_eval(v, r[0], r[1], r[2]) >= 0) {

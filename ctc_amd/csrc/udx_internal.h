@@ -174,11 +174,14 @@ int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp);
 void wall_free(WallSet *w);
 int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, const int *n_dev, const float4 *pos,
                 const float4 *vel, const udx_pair_dpd &p, const float *seed_dev, float *ff, int ff_stride,
-                unsigned long long *pairs, int exact);
-int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp);
+                unsigned long long *pairs, int exact, const WallVel &wv);
+int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp, const WallVel &wv);
+// the wall velocity of v at time t: shear sin resolved to its rate gdot sin(w t)
+// (src/wall/wvel/imp/main.h:71-84)
+WallVel wall_vel_view(const udx_wall_vel &v, float t);
 int wall_repulse_aos(hipStream_t s, const SdfField *q, float lambda, int n, const float *pp, float *ff);
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q);
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv);
 int diag_aos(hipStream_t s, int n, const float *pp, double *partials_dev, double out[5]);
 
 // solvent trunk f of one rank: the halo fragments 0..25

@@ -189,11 +189,11 @@ __global__ __launch_bounds__(256) void k_image_scatter(int nw, const float *wall
 // for the pair body at nearly every candidate.
 constexpr int WALL_THREADS = 128, WQ = 16;
 template <int WR, bool EXACT>
-__global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, int nmax, const int *n_dev, const float4 *pos,
-                                                             const float4 *vel, const float4 *wpos,
-                                                             const int *wstarts, int Wx, int Wy, int Wz, DpdPar par,
-                                                             const float *seedp, float *ff, int ff_stride,
-                                                             unsigned long long *pairs) {
+__global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, WallVel wv, int nmax, const int *n_dev,
+                                                             const float4 *pos, const float4 *vel,
+                                                             const float4 *wpos, const int *wstarts, int Wx, int Wy,
+                                                             int Wz, DpdPar par, const float *seedp, float *ff,
+                                                             int ff_stride, unsigned long long *pairs) {
     __shared__ unsigned q[WQ * WALL_THREADS];
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     int npair = 0;
@@ -236,8 +236,14 @@ __global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, int nmax
         float x0 = seed - low_discrepancy(fa_lo, fa_hi, (float)((int)e0 & 0x3FF), (float)(int)e0);
         float x1 = seed - low_discrepancy(fa_lo, fa_hi, (float)((int)e1 & 0x3FF), (float)(int)e1);
         logistic18x2(&x0, &x1);
-        const float3 f0 = dpd_pair<WR, EXACT>(par, dx0, dy0, dz0, r20, va.x, va.y, va.z, x0 * R_SQRT2);
-        const float3 f1 = dpd_pair<WR, EXACT>(par, dx1, dy1, dz1, r21, va.x, va.y, va.z, x1 * R_SQRT2);
+        // wall particle velocities at their positions (src/wall/force/dev/main.h:1-12)
+        float u0[3], u1[3];
+        wvel_at(wv, w0.x, w0.y, w0.z, u0);
+        wvel_at(wv, w1.x, w1.y, w1.z, u1);
+        const float3 f0 = dpd_pair<WR, EXACT>(par, dx0, dy0, dz0, r20, va.x - u0[0], va.y - u0[1], va.z - u0[2],
+                                              x0 * R_SQRT2);
+        const float3 f1 = dpd_pair<WR, EXACT>(par, dx1, dy1, dz1, r21, va.x - u1[0], va.y - u1[1], va.z - u1[2],
+                                              x1 * R_SQRT2);
         if (ok0) {
             fx += f0.x;
             fy += f0.y;
@@ -300,12 +306,12 @@ __global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, int nmax
     }
 }
 
-__global__ __launch_bounds__(256) void k_wall_bounce_aos(SdfView v, float dt, int n, float *pp) {
+__global__ __launch_bounds__(256) void k_wall_bounce_aos(SdfView v, WallVel wv, float dt, int n, float *pp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float *p = pp + 6 * (long)i;
     float r[3] = {p[0], p[1], p[2]}, u[3] = {p[3], p[4], p[5]};
-    if (sdf_bounce(v, dt, r, u)) {
+    if (sdf_bounce(v, wv, dt, r, u)) {
         p[0] = r[0]; p[1] = r[1]; p[2] = r[2];
         p[3] = u[0]; p[4] = u[1]; p[5] = u[2];
     }
@@ -509,13 +515,13 @@ void wall_free(WallSet *w) {
 
 int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, const int *n_dev, const float4 *pos,
                 const float4 *vel, const udx_pair_dpd &p, const float *seed_dev, float *ff, int ff_stride,
-                unsigned long long *pairs, int exact) {
+                unsigned long long *pairs, int exact, const WallVel &wv) {
     if (nmax <= 0 || w->n <= 0) return UDX_OK;
     const Box &b = w->cl.box;
     DpdPar par{p.a, p.g, p.s, p.spow};
     const dim3 grid(ceil_div(nmax, WALL_THREADS)), blk(WALL_THREADS);
 #define UDX_WALL_LAUNCH(WR, EX)                                                                              \
-    hipLaunchKernelGGL((k_wall_force<WR, EX>), grid, blk, 0, s, q->view, nmax, n_dev, pos, vel, w->pos,       \
+    hipLaunchKernelGGL((k_wall_force<WR, EX>), grid, blk, 0, s, q->view, wv, nmax, n_dev, pos, vel, w->pos,   \
                        w->cl.starts, b.L[0], b.L[1], b.L[2], par, seed_dev, ff, ff_stride, pairs)
     const float sp = p.spow;
     if (exact) {
@@ -534,9 +540,36 @@ int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, co
     return UDX_OK;
 }
 
-int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp) {
+WallVel wall_vel_view(const udx_wall_vel &v, float t) {
+    WallVel w{};
+    w.type = WVEL_NONE;
+    switch (v.type) {
+        case UDX_WVEL_CSTE:
+            w.type = WVEL_CSTE;
+            for (int c = 0; c < 3; c++) w.u[c] = v.u[c];
+            break;
+        case UDX_WVEL_SHEAR:
+        case UDX_WVEL_SHEAR_SIN:
+            w.type = WVEL_SHEAR;
+            w.gdot = v.type == UDX_WVEL_SHEAR ? v.gdot : v.gdot * sinf(v.w * t);
+            w.vdir = v.vdir;
+            w.gdir = v.gdir;
+            w.half = v.half;
+            break;
+        case UDX_WVEL_HS:
+            w.type = WVEL_HS;
+            w.hu = v.hs_u;
+            w.hh = v.hs_h;
+            break;
+        default:
+            break;
+    }
+    return w;
+}
+
+int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp, const WallVel &wv) {
     if (n <= 0) return UDX_OK;
-    hipLaunchKernelGGL(k_wall_bounce_aos, dim3(ceil_div(n, 256)), dim3(256), 0, s, q->view, dt, n, pp);
+    hipLaunchKernelGGL(k_wall_bounce_aos, dim3(ceil_div(n, 256)), dim3(256), 0, s, q->view, wv, dt, n, pp);
     UDX_LAUNCHED();
     return UDX_OK;
 }

@@ -41,6 +41,26 @@ class RbcParams(C.Structure):
                                           "kv", "totArea", "totVolume")]
 
 
+class WallVel(C.Structure):
+    """udx_wall_vel: the wall velocity types of src/wall/wvel/imp/type.h"""
+    _fields_ = [("type", C.c_int), ("u", C.c_float * 3), ("gdot", C.c_float), ("vdir", C.c_int), ("gdir", C.c_int),
+                ("half", C.c_int), ("w", C.c_float), ("hs_u", C.c_float), ("hs_h", C.c_float)]
+
+
+WVEL_TYPES = {"none": 0, "constant": 1, "shear": 2, "shear sin": 3, "hele shaw": 4}
+
+
+def wall_vel(kind="none", u=(0.0, 0.0, 0.0), gdot=0.0, vdir=0, gdir=2, half=0, w=0.0, hs_u=0.0, hs_h=1.0):
+    """wvel_set_cste / wvel_set_shear / wvel_set_shear_sin / wvel_set_hs
+    (src/wall/wvel/imp.h); kind "none" keeps the walls at rest"""
+    v = WallVel()
+    v.type = WVEL_TYPES[kind]
+    for c in range(3):
+        v.u[c] = u[c]
+    v.gdot, v.vdir, v.gdir, v.half, v.w, v.hs_u, v.hs_h = gdot, vdir, gdir, half, w, hs_u, hs_h
+    return v
+
+
 _VP, _I, _F = C.c_void_p, C.c_int, C.c_float
 _SIGS = {
     "udx_last_error": (C.c_char_p, []),
@@ -95,6 +115,11 @@ _SIGS = {
     "udx_wall_force_seed": (_I, [_VP, C.POINTER(PairDpd), _I, _VP, _F, _VP]),
     "udx_wall_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
     "udx_wall_bounce": (_I, [_VP, _F, _I, _VP]),
+    "udx_wall_set_vel": (_I, [_VP, C.POINTER(WallVel)]),
+    "udx_wall_set_time": (_I, [_VP, _F]),
+    "udx_wall_bounce_wall": (_I, [_VP, _F, _I, _VP]),
+    "udx_flu_set_time": (_I, [_VP, _F]),
+    "udx_wall_vel_set_conf": (_I, [_VP, C.POINTER(WallVel)]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
     "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
     "udx_rbc_fin": (_I, [_VP]),
@@ -391,6 +416,10 @@ class Flu:
         self.wall = wall
         _ok(lib().udx_flu_set_wall(self.h, wall.h if wall is not None else None))
 
+    def set_time(self, t):
+        """the step clock (time of the next step), advanced by dt per step"""
+        _ok(lib().udx_flu_set_time(self.h, float(t)))
+
     def set_rbc(self, rbc, fsi=None, rbc_params=None, fsi_params=None, mass=0.5, substeps=1, nc=0, pp=None):
         """membranes in the step (udx_flu_set_rbc); pp: torch (nc * nv, 6)
         float32 on the device, advanced in place; rbc=None detaches"""
@@ -509,6 +538,17 @@ class Wall:
         v = C.c_ulonglong()
         _ok(lib().udx_wall_pairs(self.h, C.byref(v), 1 if reset else 0))
         return v.value
+
+    def set_vel(self, v):
+        """v: a WallVel (wall_vel(...)); the walls rest until this is called"""
+        _ok(lib().udx_wall_set_vel(self.h, C.byref(v)))
+
+    def set_time(self, t):
+        _ok(lib().udx_wall_set_time(self.h, float(t)))
+
+    def bounce(self, dt, pp):
+        """bounce-back off this wall's surface with its velocity"""
+        _ok(lib().udx_wall_bounce_wall(self.h, dt, pp.shape[0], _ptr(pp)))
 
     def close(self):
         if self.h:
@@ -720,6 +760,11 @@ class Conf:
         p = RbcParams()
         _ok(lib().udx_rbc_params_set_conf(self.h, ns.encode(), C.byref(p)))
         return p
+
+    def wall_vel(self):
+        v = WallVel()
+        _ok(lib().udx_wall_vel_set_conf(self.h, C.byref(v)))
+        return v
 
     def close(self):
         if self.h:

@@ -156,6 +156,20 @@ typedef struct udx_wall udx_wall;
  * the solvent (sdf < 0) in order and *n is updated; particles with
  * 0 <= sdf <= 2 become wall particles, deeper ones are dropped; the wall
  * set holds them and their periodic images in the 6-cell margin, cell-sorted */
+/* wall velocity (src/wall/wvel/imp.h, src/wall/wvel/imp/type.h): resting
+ * (the default), constant u, shear v[vdir] = gdot r[gdir] (r[gdir] clipped
+ * at 0 when half), shear with the rate gdot sin(w t), Hele-Shaw radial
+ * inflow (hs_u, channel height hs_h).  Used for the wall particles' velocity
+ * in the wall DPD and in the bounce-back, 2 u_wall - v. */
+enum { UDX_WVEL_NONE = 0, UDX_WVEL_CSTE = 1, UDX_WVEL_SHEAR = 2, UDX_WVEL_SHEAR_SIN = 3, UDX_WVEL_HS = 4 };
+typedef struct udx_wall_vel {
+    int type;
+    float u[3];
+    float gdot;
+    int vdir, gdir, half;
+    float w;
+    float hs_u, hs_h;
+} udx_wall_vel;
 int udx_wall_gen(udx_ctx *ctx, const udx_sdf *q, int *n, float *pp, udx_wall **w);
 int udx_wall_fin(udx_wall *w);
 int udx_wall_counts(const udx_wall *w, int *nfrozen, int *ntotal);
@@ -169,11 +183,21 @@ int udx_wall_force_seed(udx_wall *w, const udx_pair_dpd *p, int n, const float *
 int udx_wall_pairs(udx_wall *w, unsigned long long *npairs, int reset);
 /* bounce-back of particles inside the walls (src/wall/sdf/bounce/dev/main.h) */
 int udx_wall_bounce(const udx_sdf *q, float dt, int n, float *pp);
+/* wall_vel_set (wvel_set_cste/shear/shear_sin/hs, src/wall/wvel/imp.h) and
+ * the time of the "shear sin" rate (wvel_get_step, src/wall/wvel/imp.h);
+ * udx_flu_run sets the time of its walls from its own clock */
+int udx_wall_set_vel(udx_wall *w, const udx_wall_vel *v);
+int udx_wall_set_time(udx_wall *w, float t);
+/* bounce-back with the wall's velocity at its time */
+int udx_wall_bounce_wall(const udx_wall *w, float dt, int n, float *pp);
 /* wall repulsion of object particles, f += -min(1e4, e^{l(s+1)} - 1) grad s / |grad s| */
 int udx_wall_repulse(const udx_sdf *q, float lambda, int n, const float *pp, float *ff);
 /* walls in the solvent step: wall DPD after the solvent forces, bounce-back
  * after the drift (NULL removes them) */
 int udx_flu_set_wall(udx_flu *f, udx_wall *w);
+/* the step clock: time of the next step (advanced by dt per step; the walls'
+ * velocity is evaluated at the step's time) */
+int udx_flu_set_time(udx_flu *f, float t);
 /* per-step kernel timing of the last udx_flu_run (ms, HIP events on the
  * bound stream): [0] cell list + reorder + fragment tables, [1] solvent DPD,
  * [2] wall DPD, [3] integrate + bounce-back, [4] their sum */
@@ -305,6 +329,9 @@ int udx_conf_set_string(udx_conf *c, const char *desc, const char *a);
 int udx_pair_set_conf(const udx_conf *c, const char *ns, float kBT, float dt, udx_pair_dpd *p);
 /* <ns>.{gammaC, gammaT, kBT, kb, phi, ks, x0, mpow, ka, kd, kv, totArea, totVolume} */
 int udx_rbc_params_set_conf(const udx_conf *c, const char *ns, udx_rbc_params *p);
+/* wvel.type ("constant", "shear", "shear sin", "hele shaw") and its keys
+ * (wvel_set_conf, src/wall/wvel/conf.cpp:10-47) */
+int udx_wall_vel_set_conf(const udx_conf *c, udx_wall_vel *v);
 
 /* ---- ensemble: many independent periodic boxes advanced together ----
  * No counterpart in the reference (each box there is a separate run of the

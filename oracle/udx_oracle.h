@@ -126,14 +126,24 @@ int   orc_sdf_far(const orc_sdf *q, float x, float y, float z);           /* :70
 int   orc_wall_freeze(const orc_sdf *q, int *n, float *pp, float *wall);
 /* frozen particles + periodic images in the margin box (src/wall/exch/imp.cpp) */
 int   orc_wall_images(const int L[3], int nw, const float *wall, float *out);
+/* wall velocity (src/wall/wvel/dev.h, src/wall/wvel/imp/type.h): resting,
+ * constant u, shear (v[vdir] = gdot r[gdir], r[gdir] clipped at 0 when half)
+ * or Hele-Shaw (radial inflow hu, channel height hh); a time-dependent shear
+ * rate is resolved by the caller.  NULL means resting walls. */
+enum { ORC_WVEL_NONE = 0, ORC_WVEL_CSTE = 1, ORC_WVEL_SHEAR = 2, ORC_WVEL_HS = 3 };
+typedef struct { int type; float u[3]; float gdot; int vdir, gdir, half; float hu, hh; } orc_wvel;
+void  orc_wvel_at(const orc_wvel *w, const float r[3], float v[3]);
+/* the "shear sin" rate at time t, gdot sin(w t) (src/wall/wvel/imp/main.h:71-84) */
+float orc_wvel_sin_rate(float gdot, float w, float t);
 /* solvent-wall DPD (src/wall/force/dev/main.h:1-50); wall clist on the
  * margin box L + 2 margin */
 long  orc_wall_forces(const int L[3], int n, const float *pp, const orc_sdf *q, int nw, const float *wpp,
-                      const int *wstarts, orc_dpd p, float seed, float *ff);
-void  orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp);     /* src/wall/sdf/bounce/dev/main.h */
+                      const int *wstarts, orc_dpd p, float seed, float *ff, const orc_wvel *wv);
+/* src/wall/sdf/bounce/dev/main.h */
+void  orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp, const orc_wvel *wv);
 /* kick-drift (+ body force), bounce-back when q != NULL, periodic wrap */
 void  orc_integrate_wall(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
-                         float *pp, const orc_sdf *q);
+                         float *pp, const orc_sdf *q, const orc_wvel *wv);
 void  orc_wall_repulse(const orc_sdf *q, float lambda, int n, const float *pp, float *ff);
 
 /* ---- RBC membrane (udx_oracle_rbc.c) ------------------------------------ */

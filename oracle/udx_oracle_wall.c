@@ -269,13 +269,46 @@ int orc_wall_images(const int L[3], int nw, const float *wall, float *out) {
 }
 
 /* ======================================================================== */
+/* wall velocity: src/wall/wvel/dev.h (one domain: centre coordinates are   */
+/* the local ones, src/coords/dev.h:17-33)                                  */
+/* ======================================================================== */
+void orc_wvel_at(const orc_wvel *w, const float r[3], float v[3]) {
+    v[0] = v[1] = v[2] = 0;
+    if (!w) return;
+    if (w->type == ORC_WVEL_CSTE) {
+        v[0] = w->u[0]; v[1] = w->u[1]; v[2] = w->u[2];
+    } else if (w->type == ORC_WVEL_SHEAR) {
+        float d = r[w->gdir];
+        if (w->half) d = d < 0.f ? d : 0.f;
+        v[w->vdir] = d * w->gdot;
+    } else if (w->type == ORC_WVEL_HS) {
+        float r2inv = 1.f / (r[0] * r[0] + r[1] * r[1]);
+        float hfac = w->hu * (1.f - r[2] * r[2] * 4 / (w->hh * w->hh));
+        hfac = hfac > 0.f ? hfac : 0.f;
+        v[0] = r2inv * hfac * r[0];
+        v[1] = r2inv * hfac * r[1];
+        v[2] = 0;
+    }
+}
+
+float orc_wvel_sin_rate(float gdot, float w, float t) { return gdot * sinf(w * t); }
+
+/* v = 2 u_wall(rw) - v (src/wall/wvel/dev.h:57-66) */
+static void bounce_vel(const orc_wvel *w, const float rw[3], float v[3]) {
+    float u[3];
+    orc_wvel_at(w, rw, u);
+    for (int c = 0; c < 3; c++) v[c] = 2 * u[c] - v[c];
+}
+
+/* ======================================================================== */
 /* solvent-wall DPD: src/wall/force/dev/main.h:1-50, map/ini.h:1-50         */
 /* ======================================================================== */
 /* pp: n cell-ordered solvent particles; wpp: nw wall particles cell-ordered
- * on the margin box (L + 2 M) with wstarts; wall velocity 0.  Forces are
- * added to ff.  Returns the number of pairs. */
+ * on the margin box (L + 2 M) with wstarts; wall particle velocities from
+ * wv at their positions (NULL: resting walls).  Forces are added to ff.
+ * Returns the number of pairs. */
 long orc_wall_forces(const int L[3], int n, const float *pp, const orc_sdf *q, int nw, const float *wpp,
-                     const int *wstarts, orc_dpd p, float seed, float *ff) {
+                     const int *wstarts, orc_dpd p, float seed, float *ff, const orc_wvel *wv) {
     int M = ORC_WALL_MARGIN, W[3] = {L[0] + 2 * M, L[1] + 2 * M, L[2] + 2 * M};
     long npairs = 0;
     (void)nw;
@@ -294,6 +327,7 @@ long orc_wall_forces(const int L[3], int n, const float *pp, const orc_sdf *q, i
                 for (int b = wstarts[c0]; b < wstarts[c0 + 3]; b++) {
                     const float *pw = wpp + 6 * b;
                     float wb[6] = {pw[0], pw[1], pw[2], 0, 0, 0};
+                    orc_wvel_at(wv, pw, wb + 3);
                     float f[3];
                     float rnd = orc_mean0var1ii(seed, a, b);
                     if (orc_pair_force(&p, pa, wb, rnd, f)) {
@@ -308,7 +342,7 @@ long orc_wall_forces(const int L[3], int n, const float *pp, const orc_sdf *q, i
 }
 
 /* ======================================================================== */
-/* bounce-back: src/wall/sdf/bounce/dev/main.h:1-114 (wall velocity 0)      */
+/* bounce-back: src/wall/sdf/bounce/dev/main.h:1-114                       */
 /* ======================================================================== */
 static void ugrad(const orc_sdf *q, const float r[3], float g[3]) {
     orc_sdf_grad(q, r[0], r[1], r[2], g);
@@ -320,7 +354,7 @@ static void axpy(float a, const float x[3], float y[3]) {
     for (int c = 0; c < 3; c++) y[c] = fmaf(a, x[c], y[c]);
 }
 
-static void bounce_1p(const orc_sdf *q, float dt, float currsdf, float r[3], float v[3]) {
+static void bounce_1p(const orc_sdf *q, const orc_wvel *wv, float dt, float currsdf, float r[3], float v[3]) {
     float r0[3] = {r[0], r[1], r[2]};
     axpy(-dt, v, r0);
     if (orc_sdf_eval(q, r0[0], r0[1], r0[2]) >= 0) {
@@ -331,7 +365,7 @@ static void bounce_1p(const orc_sdf *q, float dt, float currsdf, float r[3], flo
         axpy(-sdf0, g, r);
         for (int l = 8; l >= 1; --l) {
             if (orc_sdf_eval(q, r[0], r[1], r[2]) < 0) {
-                for (int c = 0; c < 3; c++) v[c] = 0.0f - v[c];   /* 2 u_wall - v, u_wall = 0 */
+                bounce_vel(wv, r, v);
                 return;
             }
             float jump = 1.1f * sdf0 / (float)(1 << l);
@@ -353,18 +387,18 @@ static void bounce_1p(const orc_sdf *q, float dt, float currsdf, float r[3], flo
     }
     float rw[3] = {r[0], r[1], r[2]};
     axpy(t, v, rw);
-    for (int c = 0; c < 3; c++) v[c] = 0.0f - v[c];
+    bounce_vel(wv, rw, v);
     r[0] = rw[0]; r[1] = rw[1]; r[2] = rw[2];
     axpy(-t, v, r);
     if (orc_sdf_eval(q, r[0], r[1], r[2]) >= 0) { r[0] = r0[0]; r[1] = r0[1]; r[2] = r0[2]; }
 }
 
-void orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp) {
+void orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp, const orc_wvel *wv) {
     for (int i = 0; i < n; i++) {
         float *r = pp + 6 * i, *v = r + 3;
         if (orc_sdf_cheap(q, r[0], r[1], r[2]) >= q->cheap_threshold) {
             float s = orc_sdf_eval(q, r[0], r[1], r[2]);
-            if (s >= 0) bounce_1p(q, dt, s, r, v);
+            if (s >= 0) bounce_1p(q, wv, dt, s, r, v);
         }
     }
 }
@@ -372,7 +406,7 @@ void orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp) {
 /* kick-drift with the constant body force, bounce-back, periodic wrap:
  * the solvent update of a wall-bounded step (src/sim/imp/run.h:43-54) */
 void orc_integrate_wall(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
-                        float *pp, const orc_sdf *q) {
+                        float *pp, const orc_sdf *q, const orc_wvel *wv) {
     for (int i = 0; i < n; i++) {
         float *r = pp + 6 * i, *v = r + 3;
         const float *f = ff + 3 * i;
@@ -383,7 +417,7 @@ void orc_integrate_wall(const int L[3], int n, float dt, float mass, const float
         }
         if (q && orc_sdf_cheap(q, r[0], r[1], r[2]) >= q->cheap_threshold) {
             float s = orc_sdf_eval(q, r[0], r[1], r[2]);
-            if (s >= 0) bounce_1p(q, dt, s, r, v);
+            if (s >= 0) bounce_1p(q, wv, dt, s, r, v);
         }
         for (int k = 0; k < 3; k++) {
             float Lk = (float)L[k], h = 0.5f * Lk;

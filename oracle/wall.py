@@ -16,6 +16,31 @@ class SdfStruct(C.Structure):
                 ("cheap_threshold", C.c_float), ("tex", C.POINTER(C.c_float))]
 
 
+class Wvel(C.Structure):
+    """wall velocity (orc_wvel): type 0 resting, 1 constant u, 2 shear, 3 Hele-Shaw"""
+    _fields_ = [("type", C.c_int), ("u", C.c_float * 3), ("gdot", C.c_float), ("vdir", C.c_int), ("gdir", C.c_int),
+                ("half", C.c_int), ("hu", C.c_float), ("hh", C.c_float)]
+
+
+def wvel(kind="none", u=(0.0, 0.0, 0.0), gdot=0.0, vdir=0, gdir=2, half=0, hu=0.0, hh=1.0):
+    w = Wvel()
+    w.type = {"none": 0, "constant": 1, "shear": 2, "hele shaw": 3}[kind]
+    for c in range(3):
+        w.u[c] = u[c]
+    w.gdot, w.vdir, w.gdir, w.half, w.hu, w.hh = gdot, vdir, gdir, half, hu, hh
+    return w
+
+
+def wvel_shear_sin(gdot, w, t, vdir=0, gdir=2, half=0):
+    """the shear view of a "shear sin" wall at time t (float32 t)"""
+    rate = _lib().orc_wvel_sin_rate(np.float32(gdot), np.float32(w), np.float32(t))
+    return wvel("shear", gdot=rate, vdir=vdir, gdir=gdir, half=half)
+
+
+def _wv(w):
+    return C.byref(w) if w is not None else None
+
+
 _declared = False
 
 
@@ -34,8 +59,11 @@ def _lib():
             "orc_sdf_far": (C.c_int, [P, C.c_float, C.c_float, C.c_float]),
             "orc_wall_freeze": (C.c_int, [P, I32P, F32P, F32P]),
             "orc_wall_images": (C.c_int, [I32P, C.c_int, F32P, F32P]),
-            "orc_wall_forces": (C.c_long, [I32P, C.c_int, F32P, P, C.c_int, F32P, I32P, Dpd, C.c_float, F32P]),
-            "orc_wall_bounce": (None, [P, C.c_float, C.c_int, F32P]),
+            "orc_wall_forces": (C.c_long, [I32P, C.c_int, F32P, P, C.c_int, F32P, I32P, Dpd, C.c_float, F32P,
+                                           C.POINTER(Wvel)]),
+            "orc_wall_bounce": (None, [P, C.c_float, C.c_int, F32P, C.POINTER(Wvel)]),
+            "orc_wvel_at": (None, [C.POINTER(Wvel), F32P, F32P]),
+            "orc_wvel_sin_rate": (C.c_float, [C.c_float, C.c_float, C.c_float]),
             "orc_wall_repulse": (None, [P, C.c_float, C.c_int, F32P, F32P]),
         }
         for name, (res, args) in sig.items():
@@ -118,9 +146,9 @@ class Sdf:
         nw = _lib().orc_wall_freeze(C.byref(self.st), _ptr(n, I32P), _ptr(pp, F32P), _ptr(wall, F32P))
         return pp[:int(n[0])].copy(), wall[:nw].copy()
 
-    def bounce(self, pp, dt):
+    def bounce(self, pp, dt, wv=None):
         pp = _f32(pp).reshape(-1, 6).copy()
-        _lib().orc_wall_bounce(C.byref(self.st), np.float32(dt), len(pp), _ptr(pp, F32P))
+        _lib().orc_wall_bounce(C.byref(self.st), np.float32(dt), len(pp), _ptr(pp, F32P), _wv(wv))
         return pp
 
     def repulse(self, pp, lam, ff=None):
@@ -148,31 +176,41 @@ def margin_box(L):
     return tuple(int(x) + 2 * MARGIN for x in L)
 
 
-def wall_forces(L, pp, sdf, wpp, wstarts, params, seed, ff=None):
+def wall_forces(L, pp, sdf, wpp, wstarts, params, seed, ff=None, wv=None):
     pp = _f32(pp).reshape(-1, 6)
     wpp = _f32(wpp).reshape(-1, 6)
     ff = np.zeros((len(pp), 3), np.float32) if ff is None else _f32(ff).copy()
     npair = _lib().orc_wall_forces(_ptr(_i32(L), I32P), len(pp), _ptr(pp, F32P), C.byref(sdf.st), len(wpp),
                                    _ptr(wpp, F32P), _ptr(_i32(wstarts), I32P), Dpd(*params), np.float32(seed),
-                                   _ptr(ff, F32P))
+                                   _ptr(ff, F32P), _wv(wv))
     return ff, int(npair)
 
 
-def integrate_wall(L, pp, ff, dt, sdf=None, mass=1.0, accel=(0.0, 0.0, 0.0)):
+def wvel_at(wv, r):
+    r = _f32(r).reshape(-1, 3)
+    out = np.zeros_like(r)
+    for k in range(len(r)):
+        _lib().orc_wvel_at(_wv(wv), _ptr(r[k], F32P), _ptr(out[k], F32P))
+    return out
+
+
+def integrate_wall(L, pp, ff, dt, sdf=None, mass=1.0, accel=(0.0, 0.0, 0.0), wv=None):
     """Kick-drift, bounce-back (when sdf is given), periodic wrap."""
     L_ = _lib()
     if not hasattr(L_.orc_integrate_wall, "_udx_decl"):
         L_.orc_integrate_wall.restype = None
         L_.orc_integrate_wall.argtypes = [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P,
-                                          C.POINTER(SdfStruct)]
+                                          C.POINTER(SdfStruct), C.POINTER(Wvel)]
         L_.orc_integrate_wall._udx_decl = True
     pp = _f32(pp).reshape(-1, 6).copy()
     L_.orc_integrate_wall(_ptr(_i32(L), I32P), len(pp), np.float32(dt), np.float32(mass), _ptr(_f32(accel), F32P),
-                          _ptr(_f32(ff), F32P), _ptr(pp, F32P), C.byref(sdf.st) if sdf is not None else None)
+                          _ptr(_f32(ff), F32P), _ptr(pp, F32P), C.byref(sdf.st) if sdf is not None else None,
+                          _wv(wv))
     return pp
 
 
-def flu_wall_step(L, pp, params, seeds28, dt, sdf, wpp, wstarts, mass=1.0, accel=(0.0, 0.0, 0.0), nthreads=1):
+def flu_wall_step(L, pp, params, seeds28, dt, sdf, wpp, wstarts, mass=1.0, accel=(0.0, 0.0, 0.0), nthreads=1,
+                  wv=None):
     """One wall-bounded solvent step: clist, DPD forces, wall forces, kick-drift,
     bounce-back, wrap.  Returns (cell-ordered new pp, solvent pairs, wall pairs)."""
     from . import oracle as O
@@ -180,6 +218,6 @@ def flu_wall_step(L, pp, params, seeds28, dt, sdf, wpp, wstarts, mass=1.0, accel
     ff, npair = O.dpd_forces(L, srt, starts, counts, params, seeds28[:27], nthreads=nthreads)
     nw = 0
     if sdf is not None:
-        ff, nw = wall_forces(L, srt, sdf, wpp, wstarts, params, seeds28[27], ff)
-    out = integrate_wall(L, srt, ff, dt, sdf, mass, accel)
+        ff, nw = wall_forces(L, srt, sdf, wpp, wstarts, params, seeds28[27], ff, wv)
+    out = integrate_wall(L, srt, ff, dt, sdf, mass, accel, wv)
     return out, npair // 2, nw

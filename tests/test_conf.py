@@ -107,3 +107,21 @@ def test_syntax():
     for bad in ("a = 1 a = 2", "a = [1, 2.0]", "a = {", "a = [1, {b = 1}]", "= 3"):
         with pytest.raises(U.UdxError):
             U.Conf(args=bad)
+
+
+def test_wall_vel_conf():
+    """wvel.type and its keys (src/wall/wvel/conf.cpp:10-47)"""
+    v = U.Conf(args='wvel = { type = "constant" u = [0.5, 0.0, -1.0] }').wall_vel()
+    assert v.type == U.WVEL_TYPES["constant"] and list(v.u) == [0.5, 0.0, -1.0]
+    v = U.Conf(args='wvel = { type = "shear" gdot = 0.25 vdir = 0 gdir = 2 half = 1 }').wall_vel()
+    assert (v.type, v.gdot, v.vdir, v.gdir, v.half) == (U.WVEL_TYPES["shear"], 0.25, 0, 2, 1)
+    v = U.Conf(args='wvel = { type = "shear sin" gdot = 2.0 vdir = 1 gdir = 0 half = 0 w = 3.5 }').wall_vel()
+    assert (v.type, v.gdot, v.vdir, v.gdir, v.half, v.w) == (U.WVEL_TYPES["shear sin"], 2.0, 1, 0, 0, 3.5)
+    v = U.Conf(args='wvel = { type = "hele shaw" u = 0.125 h = 6.0 }').wall_vel()
+    assert (v.type, v.hs_u, v.hs_h) == (U.WVEL_TYPES["hele shaw"], 0.125, 6.0)
+    with pytest.raises(U.UdxError, match="unknown type <poiseuille>"):
+        U.Conf(args='wvel = { type = "poiseuille" }').wall_vel()
+    with pytest.raises(U.UdxError, match="could not find"):
+        U.Conf(args='wvel = { type = "shear sin" gdot = 2.0 vdir = 1 gdir = 0 half = 0 }').wall_vel()
+    with pytest.raises(U.UdxError, match="wrong type"):
+        U.Conf(args='wvel = { type = "shear" gdot = 1 vdir = 0 gdir = 2 half = 0 }').wall_vel()

@@ -265,3 +265,117 @@ def test_wall_solver_trajectory_bit_exact(ctx, L, h, rho, accel):
     assert np.array_equal(bits(got), bits(ref))
     flu.close()
     s.close()
+
+
+# wall velocities (src/wall/wvel): the library's udx_wall_vel and the oracle's
+# resolved view at time t; "shear sin" is the shear view with rate gdot sin(w t)
+WVEL_CASES = {
+    "constant": dict(kind="constant", u=(0.3, -0.2, 0.1)),
+    "shear": dict(kind="shear", gdot=0.5, vdir=0, gdir=2),
+    "shear half": dict(kind="shear", gdot=-0.4, vdir=1, gdir=2, half=1),
+    "shear sin": dict(kind="shear sin", gdot=0.5, vdir=0, gdir=2, w=3.0),
+    "hele shaw": dict(kind="hele shaw", hs_u=0.05, hs_h=30.0),
+}
+
+
+def wvel_pair(name, t):
+    c = dict(WVEL_CASES[name])
+    v = U.wall_vel(**c)
+    kind = c.pop("kind")
+    if kind == "shear sin":
+        ref = W.wvel_shear_sin(c["gdot"], c["w"], t, c["vdir"], c["gdir"], c.get("half", 0))
+    elif kind == "hele shaw":
+        ref = W.wvel(kind, hu=c["hs_u"], hh=c["hs_h"])
+    else:
+        ref = W.wvel(kind, **c)
+    return v, ref
+
+
+@pytest.mark.parametrize("name", list(WVEL_CASES))
+def test_wall_vel_forces_bit_exact(ctx, name):
+    """Exact-mode wall force with moving wall particles, bit for bit."""
+    L, h, rho = (16, 12, 24), 8.0, 4
+    say("wall force, wall velocity", name)
+    s = Setup(ctx, L, h, rho, 1, exact=True)
+    starts, counts, srt, _ = O.clist_build(L, s.solvent)
+    par = (25.0, 4.5, O.dpd_sigma(0.0444, 4.5, 5e-4), 0.25)
+    seed_f = np.float32(0.3141)
+    t = np.float32(0.4)
+    v, wv = wvel_pair(name, t)
+    s.w.set_vel(v)
+    s.w.set_time(t)
+    want, npair = W.wall_forces(L, srt, s.ref, s.wpp, s.wstarts, par, seed_f, None, wv)
+    rest, _ = W.wall_forces(L, srt, s.ref, s.wpp, s.wstarts, par, seed_f)
+    assert not np.array_equal(want, rest)
+    ffd = torch.zeros((len(srt), 3), dtype=torch.float32, device="cuda")
+    s.w.pairs(reset=True)
+    s.w.force(U.PairDpd(*par), dev(srt), ffd, seed=seed_f)
+    assert s.w.pairs() == npair and npair > 0
+    assert np.array_equal(bits(host(ffd)), bits(want))
+    s.close()
+
+
+@pytest.mark.parametrize("name", list(WVEL_CASES))
+def test_wall_vel_bounce_bit_exact(ctx, name):
+    L = (16, 12, 24)
+    say("bounce, wall velocity", name)
+    s = Setup(ctx, L, 8.0, 2, 3)
+    rng = np.random.default_rng(8)
+    n = 4000
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, 0] = rng.uniform(-8, 8, n)
+    pp[:, 1] = rng.uniform(-6, 6, n)
+    side = rng.choice([-1, 1], n)
+    pp[:, 2] = side * rng.uniform(6.0, 8.3, n)
+    pp[:, 3:5] = rng.normal(0, 1, (n, 2))
+    pp[:, 5] = side * rng.uniform(-2.0, 2.0, n)
+    dt = np.float32(0.05)
+    t = np.float32(1.3)
+    v, wv = wvel_pair(name, t)
+    s.w.set_vel(v)
+    s.w.set_time(t)
+    ppd = dev(pp)
+    s.w.bounce(float(dt), ppd)
+    want = s.ref.bounce(pp, dt, wv)
+    assert not np.array_equal(want, s.ref.bounce(pp, dt))
+    assert np.array_equal(bits(host(ppd)), bits(want))
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["constant", "shear", "shear half", "shear sin"])
+def test_wall_vel_trajectory_bit_exact(ctx, name):
+    """Moving walls in the solvent step (wall DPD and bounce-back with the
+    wall velocity at the step's time; the clock starts at t0 and advances by
+    dt in float) over 12 steps in exact mode: equal to the oracle bit for bit."""
+    L, h, rho, accel = (16, 12, 24), 8.0, 4, (0.0, 0.0, 0.0)
+    say("wall trajectory, wall velocity", name)
+    nsteps, dt, t0 = 12, 1e-3, np.float32(0.25)
+    s = Setup(ctx, L, h, rho, 9)
+    params = (25.0, 4.5, O.dpd_sigma(1.0, 4.5, dt), 0.25)
+    seeds = O.seeds_table(nsteps)
+    wtrunk = O.kiss_fill(42, 42, 42, 42, nsteps)
+    seeds28 = np.concatenate([seeds, np.asarray(wtrunk, np.float32)[:, None]], axis=1)
+    ref, t = s.keep.copy(), t0
+    pairs_ref = wpairs_ref = 0
+    for k in range(nsteps):
+        v, wv = wvel_pair(name, t)
+        ref, npair, nw = W.flu_wall_step(L, ref, params, seeds28[k], dt, s.ref, s.wpp, s.wstarts, accel=accel,
+                                         nthreads=8, wv=wv)
+        pairs_ref += npair
+        wpairs_ref += nw
+        t = np.float32(t + np.float32(dt))
+    flu = U.Flu(ctx, L, len(s.solvent), U.PairDpd(*params), dt, 1.0, accel, exact=True)
+    ppd = dev(s.solvent)
+    flu.set_pp(ppd)
+    s.w.set_vel(v)
+    flu.set_wall(s.w)
+    flu.set_time(t0)
+    flu.run(nsteps, seeds28)
+    out = torch.zeros_like(ppd)
+    n = flu.get_pp(out)
+    got = host(out)[:n]
+    assert flu.pairs() == pairs_ref
+    assert flu.wall_pairs() == wpairs_ref and wpairs_ref > 0
+    assert np.array_equal(bits(got), bits(ref))
+    flu.close()
+    s.close()

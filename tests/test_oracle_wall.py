@@ -167,3 +167,87 @@ def test_bounce_returns_particles_to_the_fluid():
     assert np.all(s.eval(out[:, :3]) < 0)
     assert np.all(out[:, 5] * side < 0)               # normal velocity reversed
     assert np.array_equal(out[:, 3:], -pp[:, 3:])     # resting wall: v -> -v
+
+
+def test_wvel_closed_forms():
+    """Wall velocities (src/wall/wvel/dev.h:1-52) at hand-worked points."""
+    r = np.array([[1.5, -2.0, 3.0], [3.0, 4.0, 0.0], [3.0, 4.0, 1.0], [3.0, 4.0, 2.5], [0.5, 0.0, -1.25]], np.float32)
+    v = W.wvel_at(W.wvel("constant", u=(0.3, -0.2, 0.1)), r)
+    assert np.array_equal(v, np.tile(np.float32([0.3, -0.2, 0.1]), (len(r), 1)))
+    # shear: v[vdir] = gdot r[gdir]; half keeps r[gdir] <= 0
+    v = W.wvel_at(W.wvel("shear", gdot=0.5, vdir=0, gdir=2), r)
+    assert np.array_equal(v[:, 0], np.float32(0.5) * r[:, 2]) and not v[:, 1:].any()
+    v = W.wvel_at(W.wvel("shear", gdot=0.5, vdir=1, gdir=0, half=1), r)
+    assert np.array_equal(v[:, 1], np.float32(0.5) * np.minimum(r[:, 0], 0)) and not v[:, [0, 2]].any()
+    assert not v[:, 1].any()                          # every x here is positive
+    v = W.wvel_at(W.wvel("shear", gdot=-2.0, vdir=2, gdir=1, half=1), r)
+    assert np.array_equal(v[:, 2], np.float32(-2.0) * np.minimum(r[:, 1], 0))
+    # Hele-Shaw, u = 0.5, h = 4: radial u (1 - 4 z^2 / h^2) / |r_xy|^2, zero
+    # from |z| = h / 2 on
+    v = W.wvel_at(W.wvel("hele shaw", hu=0.5, hh=4.0), r)
+    want = np.zeros((len(r), 3))
+    for k, (x, y, z) in enumerate(r.astype(np.float64)):
+        hfac = max(0.0, 0.5 * (1 - 4 * z * z / 16.0))
+        want[k, :2] = hfac * np.array([x, y]) / (x * x + y * y)
+    assert np.allclose(v, want, rtol=4e-7, atol=0)
+    assert np.array_equal(v[1], np.float32([0.06, 0.08, 0])) and not v[3].any()
+    assert not W.wvel_at(W.wvel("none"), r).any() and not W.wvel_at(None, r).any()
+
+
+def test_wvel_shear_sin_rate():
+    for gdot, w, t in [(0.5, 3.0, 0.4), (2.0, 0.1, 12.5), (-1.0, 7.0, 0.0)]:
+        rate = W.wvel_shear_sin(gdot, w, t).gdot
+        assert abs(rate - gdot * np.sin(np.float64(np.float32(w) * np.float32(t)))) <= 1e-6 * abs(gdot)
+
+
+def moving_wall_particles(n, seed):
+    rng = np.random.default_rng(seed)
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, 0] = rng.uniform(-8, 8, n)
+    pp[:, 1] = rng.uniform(-6, 6, n)
+    side = rng.choice([-1, 1], n)
+    pp[:, 2] = side * rng.uniform(8.0, 8.05, n)       # just inside a wall of the |z| < 8 channel
+    pp[:, 3:5] = rng.normal(0, 1, (n, 2))
+    pp[:, 5] = side * rng.uniform(0.5, 2.0, n)        # moving into it
+    return pp, side
+
+
+def test_bounce_off_moving_walls():
+    """v' = 2 u_wall(r_w) - v at the crossing point r_w on the plane |z| = 8."""
+    s, _ = channel((16, 12, 24))
+    pp, side = moving_wall_particles(500, 6)
+    dt = 0.05
+    u = np.float32([0.3, -0.2, 0.1])
+    out = s.bounce(pp, dt, W.wvel("constant", u=u))
+    assert np.all(s.eval(out[:, :3]) < 0)
+    assert np.allclose(out[:, 3:] + pp[:, 3:], 2 * u, rtol=0, atol=1e-6)
+    # Couette walls v_x = gdot z: the lower wall moves at -8 gdot, the upper at +8 gdot
+    gdot = np.float32(0.2)
+    out = s.bounce(pp, dt, W.wvel("shear", gdot=gdot, vdir=0, gdir=2))
+    assert np.all(s.eval(out[:, :3]) < 0)
+    assert np.allclose(out[:, 3] + pp[:, 3], 2 * gdot * 8 * side, rtol=0, atol=2 * gdot * 0.01)
+    assert np.array_equal(out[:, 4:], -pp[:, 4:])
+    # half: only the lower (z < 0) wall moves
+    out = s.bounce(pp, dt, W.wvel("shear", gdot=gdot, vdir=0, gdir=2, half=1))
+    up = side > 0
+    assert np.array_equal(out[up, 3:], -pp[up, 3:])
+    assert np.allclose(out[~up, 3] + pp[~up, 3], -2 * gdot * 8, rtol=0, atol=2 * gdot * 0.01)
+
+
+def test_wall_forces_galilean():
+    """A constant wall velocity u enters only through v_solvent - u: the
+    forces with walls at u equal those of solvent velocities v - u beside
+    resting walls, bit for bit."""
+    L = (16, 12, 24)
+    s, srt, wpp, ws = wall_setup(L)
+    par = (25.0, 4.5, 3.0, 0.25)
+    seed = np.float32(0.37)
+    u = np.float32([0.4, -0.3, 0.2])
+    moved = srt.copy()
+    moved[:, 3:] = srt[:, 3:] - u
+    f_u, n_u = W.wall_forces(L, srt, s, wpp, ws, par, seed, wv=W.wvel("constant", u=u))
+    f_0, n_0 = W.wall_forces(L, moved, s, wpp, ws, par, seed)
+    assert n_u == n_0 > 0
+    assert np.array_equal(f_u, f_0)
+    f_r, _ = W.wall_forces(L, srt, s, wpp, ws, par, seed)
+    assert not np.array_equal(f_u, f_r)
