@@ -687,6 +687,49 @@ int udx_wall_gen(udx_ctx *ctx, const udx_sdf *q, int *n, float *pp, udx_wall **o
     return UDX_OK;
 }
 
+// wall strt (src/wall/imp/strt.h:1-60): the frozen particles as template
+// "wall.pp"/templ; reading it rebuilds the images and the wall cell list
+int udx_wall_strt_dump_templ(const udx_wall *w, const char *base) {
+    UDX_CHECK(w && base, UDX_EINVAL, "udx_wall_strt_dump_templ: null argument");
+    const int n = w->w.nfrozen;
+    std::vector<float> hst(6 * (size_t)(n > 0 ? n : 1));
+    if (n > 0) {
+        UDX_HIP(hipMemcpyAsync(hst.data(), w->w.frozen, 6 * sizeof(float) * (size_t)n, hipMemcpyDeviceToHost,
+                               w->ctx->stream));
+        UDX_HIP(hipStreamSynchronize(w->ctx->stream));
+    }
+    return udx_restart_write_pp(base, "wall.pp", UDX_RESTART_TEMPL, n, hst.data());
+}
+
+int udx_wall_strt_gen(udx_ctx *ctx, const udx_sdf *q, const char *base, int maxn, udx_wall **out) {
+    UDX_CHECK(ctx && q && q->f.ready && base && out && maxn >= 0, UDX_EINVAL, "udx_wall_strt_gen: bad arguments");
+    std::vector<float> hst(6 * (size_t)(maxn > 0 ? maxn : 1));
+    int n = 0;
+    UDX_TRY(udx_restart_read_pp(base, "wall.pp", UDX_RESTART_TEMPL, maxn, &n, hst.data()));
+    float *frozen = nullptr;
+    UDX_TRY(dev_alloc(&frozen, 6 * (size_t)(n > 0 ? n : 1)));
+    if (n > 0 && (hipMemcpyAsync(frozen, hst.data(), 6 * sizeof(float) * (size_t)n, hipMemcpyHostToDevice,
+                                 ctx->stream) != hipSuccess)) {
+        dev_free(frozen);
+        set_error("udx_wall_strt_gen: copy failed");
+        return UDX_EHIP;
+    }
+    udx_wall *w = new udx_wall();
+    w->ctx = ctx;
+    w->sdf = q;
+    udx_rnd_ini(42, 42, 42, 42, &w->trunk);
+    int s = wall_build_frozen(ctx->stream, &w->w, &q->f, n, frozen);
+    if (!s) s = dev_alloc(&w->seed_dev, 1);
+    if (!s) s = dev_alloc(&w->pairs_dev, 1);
+    if (!s && hipMemset(w->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess) s = UDX_EHIP;
+    if (s) {
+        udx_wall_fin(w);
+        return s;
+    }
+    *out = w;
+    return UDX_OK;
+}
+
 int udx_wall_counts(const udx_wall *w, int *nfrozen, int *ntotal) {
     UDX_CHECK(w, UDX_EINVAL, "udx_wall_counts: null handle");
     if (nfrozen) *nfrozen = w->w.nfrozen;
@@ -790,6 +833,43 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
               UDX_EINVAL, "udx_flu_set_wall: wall and solvent boxes differ");
     UDX_CHECK(!w || !f->mb.r, UDX_EINVAL, "udx_flu_set_wall: membranes together with walls are not supported");
     f->wall = w;
+    return UDX_OK;
+}
+
+// flu_strt_dump / flu_strt_quants (src/flu/imp/start.h:20-41): code "flu.pp"
+int udx_flu_strt_dump(udx_flu *f, const char *base, int id) {
+    UDX_CHECK(f && base, UDX_EINVAL, "udx_flu_strt_dump: null argument");
+    const int n = f->n;
+    std::vector<float> hst(6 * (size_t)(n > 0 ? n : 1));
+    if (n > 0) {
+        float *d = nullptr;
+        UDX_TRY(dev_alloc(&d, 6 * (size_t)n));
+        int st = f4_to_aos(f->ctx->stream, n, nullptr, f->pos[f->cur], f->vel[f->cur], d);
+        if (!st && (hipMemcpyAsync(hst.data(), d, 6 * sizeof(float) * (size_t)n, hipMemcpyDeviceToHost,
+                                   f->ctx->stream) != hipSuccess ||
+                    hipStreamSynchronize(f->ctx->stream) != hipSuccess))
+            st = UDX_EHIP;
+        dev_free(d);
+        UDX_CHECK(!st, st, "udx_flu_strt_dump: copy failed");
+    }
+    return udx_restart_write_pp(base, "flu.pp", id, n, hst.data());
+}
+
+int udx_flu_strt_quants(udx_flu *f, const char *base, int id) {
+    UDX_CHECK(f && base, UDX_EINVAL, "udx_flu_strt_quants: null argument");
+    std::vector<float> hst(6 * (size_t)f->conf.maxp);
+    int n = 0;
+    UDX_TRY(udx_restart_read_pp(base, "flu.pp", id, f->conf.maxp, &n, hst.data()));
+    float *d = nullptr;
+    UDX_TRY(dev_alloc(&d, 6 * (size_t)(n > 0 ? n : 1)));
+    int st = UDX_OK;
+    if (n > 0 && hipMemcpyAsync(d, hst.data(), 6 * sizeof(float) * (size_t)n, hipMemcpyHostToDevice,
+                                f->ctx->stream) != hipSuccess)
+        st = UDX_EHIP;
+    if (!st) st = udx_flu_set_pp(f, n, d);
+    if (!st && hipStreamSynchronize(f->ctx->stream) != hipSuccess) st = UDX_EHIP;
+    dev_free(d);
+    UDX_CHECK(!st, st, "udx_flu_strt_quants: copy failed");
     return UDX_OK;
 }
 

@@ -165,12 +165,14 @@ struct WallSet {
     ClistBufs cl;    // cell list of the wall particles on the margin box
     float4 *pos;     // cell-ordered wall positions
     float *pp_aos;   // the same in AoS (velocity 0)
+    float *frozen;   // the frozen particles in freeze order (AoS), the restart template
     int n, nfrozen;  // all wall particles (frozen + periodic images), frozen ones
 };
 void sdf_texture_size(const int L[3], int T[3]);
 int sdf_build(hipStream_t s, SdfField *q, const float ext[3], const int N[3], const float *D_host);
 int sdf_probe(hipStream_t s, const SdfField *q, int n, const float *r3, int what, float *out);
 int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp);
+int wall_build_frozen(hipStream_t s, WallSet *w, const SdfField *q, int nw, float *frozen);
 void wall_free(WallSet *w);
 int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, const int *n_dev, const float4 *pos,
                 const float4 *vel, const udx_pair_dpd &p, const float *seed_dev, float *ff, int ff_stride,

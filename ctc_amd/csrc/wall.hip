@@ -449,6 +449,15 @@ int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp) 
         return st;
     }
     *n = nb;
+    return wall_build_frozen(s, w, q, nw, frozen);
+}
+
+// the frozen particles (nw, AoS, device; owned by w from here on) -> periodic
+// images and the wall cell list on the margin box
+int wall_build_frozen(hipStream_t s, WallSet *w, const SdfField *q, int nw, float *frozen) {
+    int st = UDX_OK;
+    w->frozen = frozen;
+    w->nfrozen = nw;
     // images
     const long ni = 26L * nw;
     int *iflag = nullptr, *ipos = nullptr, *itmp = nullptr;
@@ -479,7 +488,6 @@ int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp) 
     dev_free(iflag);
     dev_free(ipos);
     dev_free(itmp);
-    dev_free(frozen);
     if (st) {
         dev_free(all);
         set_error("wall: images failed");
@@ -488,7 +496,6 @@ int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp) 
     // wall cell list on the margin box
     Box wb;
     for (int c = 0; c < 3; c++) wb.L[c] = q->L[c] + 2 * WALL_MARGIN;
-    w->nfrozen = nw;
     w->n = ntot;
     st = clist_alloc(&w->cl, wb, ntot > 0 ? ntot : 1);
     float *sorted = nullptr;
@@ -508,6 +515,7 @@ int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp) 
 }
 
 void wall_free(WallSet *w) {
+    dev_free(w->frozen);
     clist_free(&w->cl);
     dev_free(w->pos);
     dev_free(w->pp_aos);

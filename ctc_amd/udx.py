@@ -120,6 +120,14 @@ _SIGS = {
     "udx_wall_bounce_wall": (_I, [_VP, _F, _I, _VP]),
     "udx_flu_set_time": (_I, [_VP, _F]),
     "udx_wall_vel_set_conf": (_I, [_VP, C.POINTER(WallVel)]),
+    "udx_restart_write_pp": (_I, [C.c_char_p, C.c_char_p, _I, C.c_long, _VP]),
+    "udx_restart_read_pp": (_I, [C.c_char_p, C.c_char_p, _I, _I, C.POINTER(_I), _VP]),
+    "udx_restart_write_ii": (_I, [C.c_char_p, C.c_char_p, _I, C.c_long, _VP]),
+    "udx_restart_read_ii": (_I, [C.c_char_p, C.c_char_p, _I, _I, C.POINTER(_I), _VP]),
+    "udx_flu_strt_dump": (_I, [_VP, C.c_char_p, _I]),
+    "udx_flu_strt_quants": (_I, [_VP, C.c_char_p, _I]),
+    "udx_wall_strt_dump_templ": (_I, [_VP, C.c_char_p]),
+    "udx_wall_strt_gen": (_I, [_VP, _VP, C.c_char_p, _I, C.POINTER(_VP)]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
     "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
     "udx_rbc_fin": (_I, [_VP]),
@@ -420,6 +428,12 @@ class Flu:
         """the step clock (time of the next step), advanced by dt per step"""
         _ok(lib().udx_flu_set_time(self.h, float(t)))
 
+    def strt_dump(self, base, id):
+        _ok(lib().udx_flu_strt_dump(self.h, str(base).encode(), id))
+
+    def strt_quants(self, base, id):
+        _ok(lib().udx_flu_strt_quants(self.h, str(base).encode(), id))
+
     def set_rbc(self, rbc, fsi=None, rbc_params=None, fsi_params=None, mass=0.5, substeps=1, nc=0, pp=None):
         """membranes in the step (udx_flu_set_rbc); pp: torch (nc * nv, 6)
         float32 on the device, advanced in place; rbc=None detaches"""
@@ -498,15 +512,23 @@ class Sdf:
 class Wall:
     """wall_gen + wall_force (src/wall/imp.h): frozen wall particles."""
 
-    def __init__(self, ctx, sdf, pp, exact=False, mode=None):
+    def __init__(self, ctx, sdf, pp, exact=False, mode=None, strt=None, maxn=0):
         """pp: solvent (n, 6) cuda tensor; on return the first self.n_solvent
-        rows hold the solvent that stays (order kept)."""
+        rows hold the solvent that stays (order kept).  With strt=base the
+        frozen particles come from the restart template instead (pp unused)."""
         self.ctx, self.sdf = ctx, sdf
         self.h = C.c_void_p()
-        n = C.c_int(pp.shape[0])
-        _ok(lib().udx_wall_gen(ctx.h, sdf.h, C.byref(n), _ptr(pp), C.byref(self.h)))
-        self.n_solvent = n.value
+        if strt is not None:
+            _ok(lib().udx_wall_strt_gen(ctx.h, sdf.h, str(strt).encode(), maxn, C.byref(self.h)))
+            self.n_solvent = None
+        else:
+            n = C.c_int(pp.shape[0])
+            _ok(lib().udx_wall_gen(ctx.h, sdf.h, C.byref(n), _ptr(pp), C.byref(self.h)))
+            self.n_solvent = n.value
         _ok(lib().udx_wall_set_mode(self.h, _mode(exact, mode)))
+
+    def strt_dump_templ(self, base):
+        _ok(lib().udx_wall_strt_dump_templ(self.h, str(base).encode()))
 
     def counts(self):
         a, b = C.c_int(), C.c_int()
@@ -554,6 +576,38 @@ class Wall:
         if self.h:
             lib().udx_wall_fin(self.h)
             self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- restart
+RESTART_TEMPL, RESTART_FINAL = -1, -2
+
+
+def restart_write_pp(base, code, id, pp):
+    import numpy as np
+    pp = np.ascontiguousarray(pp, np.float32).reshape(-1, 6)
+    _ok(lib().udx_restart_write_pp(str(base).encode(), code.encode(), id, len(pp), pp.ctypes.data))
+
+
+def restart_read_pp(base, code, id, maxn):
+    import numpy as np
+    out = np.zeros((max(maxn, 1), 6), np.float32)
+    n = C.c_int()
+    _ok(lib().udx_restart_read_pp(str(base).encode(), code.encode(), id, maxn, C.byref(n), out.ctypes.data))
+    return out[:n.value].copy()
+
+
+def restart_write_ii(base, code, id, ii):
+    import numpy as np
+    ii = np.ascontiguousarray(ii, np.int32).ravel()
+    _ok(lib().udx_restart_write_ii(str(base).encode(), code.encode(), id, len(ii), ii.ctypes.data))
+
+
+def restart_read_ii(base, code, id, maxn):
+    import numpy as np
+    out = np.zeros(max(maxn, 1), np.int32)
+    n = C.c_int()
+    _ok(lib().udx_restart_read_ii(str(base).encode(), code.encode(), id, maxn, C.byref(n), out.ctypes.data))
+    return out[:n.value].copy()
 
 
 # ---------------------------------------------------------------- RBC

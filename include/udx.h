@@ -333,6 +333,28 @@ int udx_rbc_params_set_conf(const udx_conf *c, const char *ns, udx_rbc_params *p
  * (wvel_set_conf, src/wall/wvel/conf.cpp:10-47) */
 int udx_wall_vel_set_conf(const udx_conf *c, udx_wall_vel *v);
 
+/* ---- restart files: replaces restart_write_pp / restart_read_pp /
+ *      restart_write_ii / restart_read_ii (src/io/restart/imp.h:14-32),
+ *      flu_strt_dump / flu_strt_quants (src/flu/imp/start.h:20-41) and the
+ *      wall template (src/wall/imp/strt.h:1-60) ----
+ * Files BASE/CODE/ID.bop (text header) and BASE/CODE/ID.values (n x 6
+ * floats "x y z vx vy vz", or n ints "i"), ID = %05d, "templ" or "final";
+ * the directories must exist.  The reader takes the binary float/int and the
+ * ascii/iascii value formats and fails (UDX_ECAP) above maxn entries.
+ * udx_restart_* are host only (host buffers, no GPU work). */
+enum { UDX_RESTART_TEMPL = -1, UDX_RESTART_FINAL = -2 };
+int udx_restart_write_pp(const char *base, const char *code, int id, long n, const float *pp);
+int udx_restart_read_pp(const char *base, const char *code, int id, int maxn, int *n, float *pp);
+int udx_restart_write_ii(const char *base, const char *code, int id, long n, const int *ii);
+int udx_restart_read_ii(const char *base, const char *code, int id, int maxn, int *n, int *ii);
+/* the solvent's particles as CODE "flu.pp" (cell order of the last step) */
+int udx_flu_strt_dump(udx_flu *f, const char *base, int id);
+int udx_flu_strt_quants(udx_flu *f, const char *base, int id);
+/* the frozen wall particles as CODE "wall.pp", ID templ; udx_wall_strt_gen
+ * builds the wall (images, cell list) from them instead of freezing */
+int udx_wall_strt_dump_templ(const udx_wall *w, const char *base);
+int udx_wall_strt_gen(udx_ctx *ctx, const udx_sdf *q, const char *base, int maxn, udx_wall **w);
+
 /* ---- ensemble: many independent periodic boxes advanced together ----
  * No counterpart in the reference (each box there is a separate run of the
  * solvent step, src/sim/imp/run.h:8-68).  Every kernel of the step is
