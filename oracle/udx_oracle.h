@@ -194,6 +194,30 @@ long   orc_fsi_forces(const int L[3], int ns, const float *spp, const int *start
                       orc_dpd p, const float seeds[2], float *fo, float *fs);
 void   orc_fsi_seeds(int nsteps, float *out /* 2 per step: bulk, halo */);
 
+/* ---- membrane bounce-back: src/mesh_bounce (udx_oracle_mbb.c) ---------- */
+enum { ORC_BB_SUCCESS = 0, ORC_BB_NOCROSS = 1, ORC_BB_WTRIANGLE = 2, ORC_BB_HFAIL = 3 };
+/* out = (h, u, v, side) */
+int   orc_mbb_intersect(float dt, const float s1[3], const float s2[3], const float s3[3], const float vs1[3],
+                        const float vs2[3], const float vs3[3], const float pr[3], const float pv[3], float out[4]);
+/* meshes + periodic images: returns the instance count (at most 8 nm) */
+int   orc_mbb_instances(const int L[3], int nm, int nv, const float *rc, int *inst_mesh, float *inst_shift);
+void  orc_mbb_find(float dt, const int L[3], int ninst, const int *inst_mesh, const float *inst_shift, int nt, int nv,
+                   const int *tt, const float *rp, const float *rc, const int *starts, const float *pp,
+                   const float *pp0, uint64_t *keys);
+void  orc_mbb_inertia(float mass, const float a[3], const float b[3], const float c[3], float I[6]);
+void  orc_mbb_inverse(const float A[6], float I[6]);
+int   orc_mbb_nonzero(const float m[6]);
+void  orc_mbb_m2v(float mass, const float m[6], const float a[3], const float b[3], const float c[3], float va[3],
+                  float vb[3], float vc[3]);
+/* rp: start-of-step vertex positions (3 per vertex); rc: vertices (6 per
+ * vertex) after the advance, velocities updated in place; pp0/pp: solvent at
+ * the start of the step / after the move (unwrapped), pp bounced in place;
+ * mm: 6 per triangle (P, L); col_tri: the triangle each particle bounced off
+ * (-1: none).  Returns the number of bounced particles. */
+long  orc_mbb_bounce(float dt, float flu_mass, float rbc_mass, const int L[3], int nm, int nt, int nv, const int *tt,
+                     const float *rp, float *rc, const int *starts, int n, const float *pp0, float *pp, float *mm,
+                     int *col_tri);
+
 #ifdef __cplusplus
 }
 #endif

@@ -154,3 +154,41 @@ class C4:
             gx, _, _ = R.adjoint(p, self.m, xs[t], self.nc, g, self.sf, self.l0, self.a0)
             lam = (lam + gx.astype(np.float32)).astype(np.float32)
         return lam.astype(np.float64)
+
+
+def rotation(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def bounce_case(seed=1, L=(16, 16, 16), rho=10, dt=0.02, vsig=1.0, wobble=0.01, centres=None):
+    """A membrane bounce-back set-up: 498-vertex cells (one straddling the +x
+    face, so that its periodic image acts too) moving rigidly plus vertex noise
+    over the step (rp -> rc), and solvent moving in straight lines (pp0 -> pp,
+    cell-ordered by the start positions)."""
+    import oracle as O
+    m = mesh(498)
+    rng = np.random.default_rng(seed)
+    if centres is None:
+        centres = [(L[0] / 2 - 1.0, 0.5, -0.3), (-3.0, -4.0, 3.5), (-2.5, 4.5, -4.0)]
+    nm, nv = len(centres), m.nv
+    rp = np.zeros((nm * nv, 3), np.float32)
+    rc = np.zeros((nm * nv, 6), np.float32)
+    for k, c in enumerate(centres):
+        x = m.rr.astype(np.float64) @ rotation(rng).T + np.array(c)
+        rp[k * nv:(k + 1) * nv] = x.astype(np.float32)
+        shift = rng.normal(0, 0.01, 3)
+        small = np.eye(3) + 0.004 * rng.normal(size=(3, 3))
+        y = (x - np.array(c)) @ small.T + np.array(c) + shift + rng.normal(0, wobble, x.shape)
+        rc[k * nv:(k + 1) * nv, :3] = y.astype(np.float32)
+    rc[:, 3:] = rng.normal(0, 0.1, (nm * nv, 3)).astype(np.float32)
+    n = int(rho * np.prod(L))
+    pp0 = np.zeros((n, 6), np.float32)
+    pp0[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    starts, counts, srt0, order = O.clist_build(L, pp0)
+    v = rng.normal(0, vsig, (len(srt0), 3)).astype(np.float32)
+    pp = np.zeros_like(srt0)
+    pp[:, :3] = srt0[:, :3] + v * np.float32(dt)
+    pp[:, 3:] = v
+    srt0[:, 3:] = v
+    return dict(L=L, dt=dt, nm=nm, nv=nv, tt=m.tt, rp=rp, rc=rc, starts=starts, pp0=srt0, pp=pp)
