@@ -142,19 +142,34 @@ struct udx_flu {
         float *pp = nullptr;                               // the caller's vertices (AoS), advanced in place
         float *fo = nullptr, *ffast = nullptr, *tmp = nullptr;   // FSI force, sub-step force, redistribution staging
         int *cfid = nullptr, *cnew = nullptr;
+        // bounce-back (udx_flu_set_rbc_bounce): the vertices and the solvent
+        // positions at the start of the step
+        udx_mbb *bb = nullptr;
+        float *start_pp = nullptr;
+        float4 *pos0 = nullptr;
     } mb;
 };
 
 // FSI seeds of step k: after the NSEED x seed_cap solvent table
 static const float *flu_fsi_seeds(const udx_flu *f, int k) { return f->seeds_dev + NSEED * f->seed_cap + 2 * k; }
 
-static void flu_free_membranes(udx_flu *f) {
+// keep_bounce: a new membrane capacity keeps the bounce-back set-up
+static void flu_free_membranes(udx_flu *f, bool keep_bounce = false) {
     dev_free(f->mb.fo);
     dev_free(f->mb.ffast);
     dev_free(f->mb.tmp);
     dev_free(f->mb.cfid);
     dev_free(f->mb.cnew);
+    dev_free(f->mb.start_pp);
+    udx_mbb *bb = f->mb.bb;
+    float4 *pos0 = f->mb.pos0;
     f->mb = udx_flu::Membranes{};
+    if (keep_bounce) {
+        f->mb.bb = bb;
+        f->mb.pos0 = pos0;
+    } else {
+        dev_free(pos0);
+    }
 }
 
 extern "C" {
@@ -493,6 +508,10 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
     UDX_TRY(gather_f4(s, &f->cl, f->n, f->pos[a], f->vel[a], f->pos[b], f->vel[b]));
     UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts));
     if (m.r) UDX_TRY(rbc_redistribute(s, f->box, m.nc, m.nv, m.pp, m.tmp, m.cfid, m.cnew));
+    const bool bounce = m.r && m.bb;
+    // objects_save_mesh (src/sim/imp/run.h:30-31): the vertices at the start of the step
+    if (bounce && nvtx > 0)
+        UDX_HIP(hipMemcpyAsync(m.start_pp, m.pp, 6 * sizeof(float) * (size_t)nvtx, hipMemcpyDeviceToDevice, s));
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[1], s));
     DpdLaunch d;
     d.pos = f->pos[b];
@@ -528,8 +547,11 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
                            m.fo, (float *)f->ff, 4, fsi_pairs_dev(m.fsi), f->conf.mode == UDX_MODE_EXACT));
     }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
+    // store_solvent (src/sim/imp/update.h:11-16): positions before the move
+    if (bounce && f->n > 0)
+        UDX_HIP(hipMemcpyAsync(m.pos0, f->pos[b], sizeof(float4) * (size_t)f->n, hipMemcpyDeviceToDevice, s));
     UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
-                         f->pos[b], f->vel[b], sdf, wv));
+                         f->pos[b], f->vel[b], sdf, wv, bounce ? 0 : 1));
     if (m.r && nvtx > 0) {
         // objects_advance (src/sim/objects/imp/main.h:40-61): per sub-step the
         // fast force starts as the FSI force, the membrane force is added and
@@ -541,6 +563,11 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
             UDX_TRY(move_aos(s, nvtx, sub_dt, m.mass, m.ffast, m.pp));
         }
     }
+    // objects_bounce (src/sim/objects/imp/bounce.h:236-248); also the wrap
+    // the move left out
+    if (bounce)
+        UDX_TRY(mbb_step(s, m.bb, f->conf.dt, f->conf.mass, m.mass, f->box, m.nc, m.start_pp, m.pp, f->cl.starts,
+                         f->n, f->cl.starts + nc, f->pos[b], f->vel[b], m.pos0));
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[4], s));
     f->cur = b;
     f->t += f->conf.dt;   // time_line_advance (src/scheme/time_line/imp/main.h:10-21)
@@ -904,8 +931,9 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
     const int n = nc * nv;
     if (n > m.cap) {
         UDX_HIP(hipStreamSynchronize(f->ctx->stream));
-        flu_free_membranes(f);
+        flu_free_membranes(f, true);
         UDX_TRY(dev_alloc(&m.fo, 3 * (size_t)n));
+        UDX_TRY(dev_alloc(&m.start_pp, 6 * (size_t)(n > 0 ? n : 1)));
         UDX_TRY(dev_alloc(&m.ffast, 3 * (size_t)n));
         UDX_TRY(dev_alloc(&m.tmp, 6 * (size_t)n));
         UDX_TRY(dev_alloc(&m.cfid, (size_t)nc));
@@ -921,6 +949,26 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
     m.nc = nc;
     m.nv = nv;
     m.pp = pp;
+    if (m.bb && mbb_nv(m.bb) != nv) {
+        m.bb = nullptr;
+        UDX_CHECK(false, UDX_EINVAL, "udx_flu_set_rbc: the bounce-back template differs; bounce-back switched off");
+    }
+    return UDX_OK;
+}
+
+int udx_flu_set_rbc_bounce(udx_flu *f, udx_mbb *b) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_rbc_bounce: null handle");
+    udx_flu::Membranes &m = f->mb;
+    if (!b) {
+        m.bb = nullptr;
+        return UDX_OK;
+    }
+    UDX_CHECK(m.r, UDX_EINVAL, "udx_flu_set_rbc_bounce: no membranes (udx_flu_set_rbc first)");
+    UDX_CHECK(mbb_ctx(b) == f->ctx, UDX_EINVAL, "udx_flu_set_rbc_bounce: bounce-back and solvent contexts differ");
+    UDX_CHECK(mbb_nv(b) == m.nv, UDX_EINVAL, "udx_flu_set_rbc_bounce: template of %d vertices, membranes of %d",
+              mbb_nv(b), m.nv);
+    if (!m.pos0) UDX_TRY(dev_alloc(&m.pos0, (size_t)f->conf.maxp));
+    m.bb = b;
     return UDX_OK;
 }
 

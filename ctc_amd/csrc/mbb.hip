@@ -1,0 +1,859 @@
+// Bounce-back of the solvent on moving membrane meshes (src/mesh_bounce) on
+// one periodic domain.
+//
+// Per call: the periodic images of the meshes that leave the domain cropped
+// by 1 (the mesh halo of src/exch/mesh), the earliest crossing of every
+// solvent particle with a triangle instance (one thread per triangle
+// instance over the solvent cells its start-of-step bounding box + 0.1
+// covers; 64-bit atomicMin of (time bits, instance triangle) per particle),
+// the bounce of each crossing particle (one thread per particle; in the
+// solvent step this pass also applies the periodic wrap the move left out),
+// the momentum of each triangle summed in ascending particle order (counting
+// sort of the crossings by triangle), and the velocity change of the
+// vertices (one thread per vertex over its triangles in ascending order).
+//
+// Arithmetic follows the reference's mixed precision: float particle and
+// vertex data, double plane/root computations, float results; -ffp-contract
+// is off, so every product and sum rounds as written.
+#include <cstdint>
+#include <vector>
+
+#include "udx_internal.h"
+
+namespace udx {
+namespace {
+
+constexpr int MBB_THREADS = 256;
+constexpr int MBB_MAX_IMAGES = 8;   // a mesh and at most 7 images (3 faces, 3 edges, 1 corner)
+constexpr unsigned long long MBB_EMPTY = ~0ull;
+constexpr float MBB_MAX_SPAN = 8.0f;   // the widest triangle searched (a start-of-step bounding box side)
+enum { BB_SUCCESS = 0, BB_NOCROSS = 1, BB_WTRIANGLE = 2, BB_HFAIL = 3 };
+
+struct MPart {
+    float r[3], v[3];
+};
+
+// ---- roots (src/mesh_bounce/dev/roots.h) ---------------------------------
+__device__ __forceinline__ bool valid_time(double dt, double t) { return t >= 0 && t <= dt; }
+
+__device__ bool quadratic(double a, double b, double c, double *h0, double *h1) {
+    const int sgnb = b > 0 ? 1 : -1;
+    const double D = b * b - 4 * a * c;
+    if (D < 0) return false;
+    *h0 = (-b - sgnb * sqrt(D)) / (2 * a);
+    *h1 = c / (a * *h0);
+    if (*h0 > *h1) {
+        const double t = *h0;
+        *h0 = *h1;
+        *h1 = t;
+    }
+    return true;
+}
+
+// real roots of x^3 + a x^2 + b x + c in ascending order (GSL's method)
+__device__ int cubic(double a, double b, double c, double *x0, double *x1, double *x2) {
+    const double q = a * a - 3 * b;
+    const double r = 2 * a * a * a - 9 * a * b + 27 * c;
+    const double Q = q / 9;
+    const double R = r / 54;
+    const double Q3 = Q * Q * Q;
+    const double R2 = R * R;
+    const double CR2 = 729 * r * r;
+    const double CQ3 = 2916 * q * q * q;
+    if (R == 0 && Q == 0) {
+        *x0 = *x1 = *x2 = -a / 3;
+        return 3;
+    } else if (CR2 == CQ3) {
+        const double sqrtQ = sqrt(Q);
+        if (R > 0) {
+            *x0 = -2 * sqrtQ - a / 3;
+            *x1 = sqrtQ - a / 3;
+            *x2 = sqrtQ - a / 3;
+        } else {
+            *x0 = -sqrtQ - a / 3;
+            *x1 = -sqrtQ - a / 3;
+            *x2 = 2 * sqrtQ - a / 3;
+        }
+        return 3;
+    } else if (R2 < Q3) {
+        const double pi = 3.14159265358979323846;
+        const double sgnR = R >= 0 ? 1 : -1;
+        const double ratio = sgnR * sqrt(R2 / Q3);
+        const double theta = acos(ratio);
+        const double norm = -2 * sqrt(Q);
+        double t;
+        *x0 = norm * cos(theta / 3) - a / 3;
+        *x1 = norm * cos((theta + 2.0 * pi) / 3) - a / 3;
+        *x2 = norm * cos((theta - 2.0 * pi) / 3) - a / 3;
+        if (*x0 > *x1) { t = *x0; *x0 = *x1; *x1 = t; }
+        if (*x1 > *x2) {
+            t = *x1; *x1 = *x2; *x2 = t;
+            if (*x0 > *x1) { t = *x0; *x0 = *x1; *x1 = t; }
+        }
+        return 3;
+    } else {
+        const double sgnR = R >= 0 ? 1 : -1;
+        const double A = -sgnR * pow(fabs(R) + sqrt(R2 - Q3), 1.0 / 3.0);
+        const double B = Q / A;
+        *x0 = A + B - a / 3;
+        return 1;
+    }
+}
+
+// the first root of a t^3 + b t^2 + c t + d in [0, dt]
+// (src/mesh_bounce/dev/intersection.h:1-28)
+__device__ bool cubic_root(double dt, double a, double b, double c, double d, double *h) {
+    const double eps = 1e-8;
+    double h1, h2, h3;
+    if (fabs(a) > eps) {
+        b /= a;
+        c /= a;
+        d /= a;
+        const int nsol = cubic(b, c, d, &h1, &h2, &h3);
+        if (valid_time(dt, h1)) { *h = h1; return true; }
+        if (nsol > 1 && valid_time(dt, h2)) { *h = h2; return true; }
+        if (nsol > 2 && valid_time(dt, h3)) { *h = h3; return true; }
+    } else if (fabs(b) > eps) {
+        if (!quadratic(b, c, d, &h1, &h2)) return false;
+        if (valid_time(dt, h1)) { *h = h1; return true; }
+        if (valid_time(dt, h2)) { *h = h2; return true; }
+    } else if (fabs(c) > eps) {
+        h1 = -d / c;
+        if (valid_time(dt, h1)) { *h = h1; return true; }
+    }
+    return false;
+}
+
+__device__ __forceinline__ double ddot(const double a[3], const double b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ void dcross(const double a[3], const double b[3], double c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Fedosov's moving-triangle crossing (src/mesh_bounce/dev/intersection.h:30-127)
+__device__ int intersect(float dt_f, const MPart &A, const MPart &B, const MPart &C, const MPart &p0, float out[4]) {
+    const double dt = dt_f;
+    double a1[3], a2[3], at1[3], at2[3], n0[3], nt[3], ntt[3], dr0[3];
+    for (int c = 0; c < 3; c++) {
+        a1[c] = (double)(B.r[c] - A.r[c]);
+        a2[c] = (double)(C.r[c] - A.r[c]);
+        at1[c] = (double)(B.v[c] - A.v[c]);
+        at2[c] = (double)(C.v[c] - A.v[c]);
+    }
+    dcross(a1, a2, n0);
+    dcross(at1, at2, ntt);
+    nt[0] = a1[1] * at2[2] - a1[2] * at2[1] + at1[1] * a2[2] - at1[2] * a2[1];
+    nt[1] = a1[2] * at2[0] - a1[0] * at2[2] + at1[2] * a2[0] - at1[0] * a2[2];
+    nt[2] = a1[0] * at2[1] - a1[1] * at2[0] + at1[0] * a2[1] - at1[1] * a2[0];
+    for (int c = 0; c < 3; c++) dr0[c] = (double)(p0.r[c] - A.r[c]);
+    {
+        double r1[3], s11[3], n1[3], dr1[3];
+        for (int c = 0; c < 3; c++) {
+            r1[c] = (double)p0.r[c] + dt * (double)p0.v[c];
+            s11[c] = (double)A.r[c] + dt * (double)A.v[c];
+            n1[c] = n0[c] + dt * (nt[c] + dt * ntt[c]);
+        }
+        for (int c = 0; c < 3; c++) dr1[c] = r1[c] - s11[c];
+        const double b0 = ddot(dr0, n0), b1 = ddot(dr1, n1);
+        out[3] = b0 > 0 ? 1.f : -1.f;
+        if (b0 * b1 > 0) return BB_NOCROSS;
+    }
+    double hl;
+    {
+        double dv[3];
+        for (int c = 0; c < 3; c++) dv[c] = (double)(p0.v[c] - A.v[c]);
+        const double a = ddot(ntt, dv);
+        const double b = ddot(ntt, dr0) + ddot(nt, dv);
+        const double c = ddot(nt, dr0) + ddot(n0, dv);
+        const double d = ddot(n0, dr0);
+        if (!cubic_root(dt, a, b, c, d, &hl)) return BB_HFAIL;
+    }
+    double rwl[3], g[3], a1_[3], a2_[3];
+    for (int c = 0; c < 3; c++) {
+        rwl[c] = (double)p0.r[c] + hl * (double)p0.v[c];
+        g[c] = rwl[c] - (double)A.r[c] - hl * (double)A.v[c];
+        a1_[c] = a1[c] + hl * at1[c];
+        a2_[c] = a2[c] + hl * at2[c];
+    }
+    const double ga1 = ddot(g, a1_), ga2 = ddot(g, a2_);
+    const double a11 = ddot(a1_, a1_), a12 = ddot(a1_, a2_), a22 = ddot(a2_, a2_);
+    const double fac = 1.0 / (a11 * a22 - a12 * a12);
+    const double ul = (ga1 * a22 - ga2 * a12) * fac;
+    const double vl = (ga2 * a11 - ga1 * a12) * fac;
+    if ((ul < 0) || (vl < 0) || (ul + vl > 1)) return BB_WTRIANGLE;
+    out[0] = (float)hl;
+    out[1] = (float)ul;
+    out[2] = (float)vl;
+    return BB_SUCCESS;
+}
+
+// ---- particle and mesh access --------------------------------------------
+// solvent as AoS particles (the reference's Particle) or as the solvent
+// step's float4 position/velocity streams
+struct PartsAoS {
+    float *pp;
+    const float *pp0;
+    __device__ void load(int i, float r[3], float v[3]) const {
+        for (int c = 0; c < 3; c++) {
+            r[c] = pp[6 * (size_t)i + c];
+            v[c] = pp[6 * (size_t)i + 3 + c];
+        }
+    }
+    __device__ void load_r0(int i, float r[3]) const {
+        for (int c = 0; c < 3; c++) r[c] = pp0[6 * (size_t)i + c];
+    }
+    __device__ float r1(int i, int c) const { return pp[6 * (size_t)i + c]; }
+    __device__ void store(int i, const float r[3], const float v[3]) const {
+        for (int c = 0; c < 3; c++) {
+            pp[6 * (size_t)i + c] = r[c];
+            pp[6 * (size_t)i + 3 + c] = v[c];
+        }
+    }
+};
+
+struct PartsF4 {
+    float4 *pos, *vel;
+    const float4 *pos0;
+    __device__ void load(int i, float r[3], float v[3]) const {
+        const float4 p = pos[i], u = vel[i];
+        r[0] = p.x; r[1] = p.y; r[2] = p.z;
+        v[0] = u.x; v[1] = u.y; v[2] = u.z;
+    }
+    __device__ void load_r0(int i, float r[3]) const {
+        const float4 p = pos0[i];
+        r[0] = p.x; r[1] = p.y; r[2] = p.z;
+    }
+    __device__ float r1(int i, int c) const {
+        const float4 p = pos[i];
+        return c == 0 ? p.x : (c == 1 ? p.y : p.z);
+    }
+    __device__ void store(int i, const float r[3], const float v[3]) const {
+        const float4 p = pos[i], u = vel[i];
+        pos[i] = make_float4(r[0], r[1], r[2], p.w);
+        vel[i] = make_float4(v[0], v[1], v[2], u.w);
+    }
+};
+
+// the particle's start and its straight-line velocity (dev/utils.h:43-48)
+template <class P>
+__device__ void particle_path(const P &parts, float dt, int i, MPart *p0) {
+    float r1[3], v1[3];
+    parts.load(i, r1, v1);
+    parts.load_r0(i, p0->r);
+    for (int c = 0; c < 3; c++) p0->v[c] = (float)((double)(r1[c] - p0->r[c]) / (double)dt);
+}
+
+struct MeshView {
+    int nt, nv;
+    const int *tt;             // 3 per triangle
+    const float *rp, *rc;      // vertices (6 floats) at the start of the step / now
+    const int *inst_mesh;
+    const float *inst_shift;   // 3 per instance
+};
+
+// vertex j of instance k: r = rc + shift, v = (r - (rp + shift)) / dt
+__device__ void fetch_vertex(const MeshView &m, float dt, const float *sh, int j, MPart *o) {
+    for (int c = 0; c < 3; c++) {
+        const float pc = m.rc[6 * (size_t)j + c] + sh[c];
+        const float pp = m.rp[6 * (size_t)j + c] + sh[c];
+        o->r[c] = pc;
+        o->v[c] = (pc - pp) / dt;
+    }
+}
+
+__device__ void fetch_tri(const MeshView &m, float dt, int gid, MPart *A, MPart *B, MPart *C) {
+    const int k = gid / m.nt, t = gid - k * m.nt;
+    const int base = m.inst_mesh[k] * m.nv;
+    const float *sh = m.inst_shift + 3 * k;
+    fetch_vertex(m, dt, sh, base + m.tt[3 * t], A);
+    fetch_vertex(m, dt, sh, base + m.tt[3 * t + 1], B);
+    fetch_vertex(m, dt, sh, base + m.tt[3 * t + 2], C);
+}
+
+__device__ __forceinline__ void revert(float h, MPart *p) {
+    for (int c = 0; c < 3; c++) p->r[c] = p->r[c] - p->v[c] * h;
+}
+
+__device__ __forceinline__ int cell_of(int L, float x) {
+    const int c = (int)floor((double)x + L / 2);
+    return c < 0 ? 0 : (c > L - 1 ? L - 1 : c);
+}
+
+__device__ __forceinline__ unsigned long long make_key(float h, int gid) {
+    unsigned b = __float_as_uint(h);
+    if (h == 0) b = 0;
+    return ((unsigned long long)b << 32) | (unsigned)gid;
+}
+
+// ---- instances: meshes and their periodic images --------------------------
+__device__ __forceinline__ int frag_d2i(int x, int y, int z) {
+    return ((x + 2) % 3) + 3 * ((y + 2) % 3) + 9 * ((z + 2) % 3);
+}
+
+// per mesh: the box code of its current bounding box against the domain
+// cropped by 1 (src/exch/map/dev.h:21-30)
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_codes(int nv, int Lx, int Ly, int Lz, const float *rc,
+                                                           int *codes) {
+    const int m = blockIdx.x;
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f}, hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    for (int v = threadIdx.x; v < nv; v += MBB_THREADS)
+        for (int c = 0; c < 3; c++) {
+            const float x = rc[6 * ((size_t)m * nv + v) + c];
+            lo[c] = fminf(lo[c], x);
+            hi[c] = fmaxf(hi[c], x);
+        }
+    __shared__ float slo[3][MBB_THREADS], shi[3][MBB_THREADS];
+    for (int c = 0; c < 3; c++) {
+        slo[c][threadIdx.x] = lo[c];
+        shi[c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (int s = MBB_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; c++) {
+                slo[c][threadIdx.x] = fminf(slo[c][threadIdx.x], slo[c][threadIdx.x + s]);
+                shi[c][threadIdx.x] = fmaxf(shi[c][threadIdx.x], shi[c][threadIdx.x + s]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int L[3] = {Lx, Ly, Lz};
+        for (int c = 0; c < 3; c++) {
+            const int h = (L[c] - 2) / 2;
+            codes[3 * m + c] = -1 + (slo[c][0] >= -h) + (shi[c][0] >= h);
+        }
+    }
+}
+
+// does the box code send to fragment fid (a face, edge or corner it reaches)
+__device__ bool code_sends(const int *code, const int d[3]) {
+    bool any = false;
+    for (int c = 0; c < 3; c++) {
+        if (d[c] != 0 && d[c] != code[c]) return false;
+        any |= d[c] != 0;
+    }
+    return any;
+}
+
+// one block: meshes unshifted, then images by fragment id, meshes ascending
+__global__ __launch_bounds__(1024) void k_mbb_instances(int nm, int Lx, int Ly, int Lz, const int *codes,
+                                                        int *inst_mesh, float *inst_shift, int *n_inst) {
+    __shared__ int wsum[16];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int m = tid; m < nm; m += 1024) {
+        inst_mesh[m] = m;
+        inst_shift[3 * m] = inst_shift[3 * m + 1] = inst_shift[3 * m + 2] = 0.f;
+    }
+    if (tid == 0) base = nm;
+    __syncthreads();
+    const int L[3] = {Lx, Ly, Lz};
+    for (int fid = 0; fid < 26; fid++) {
+        const int t3[3] = {fid % 3, (fid / 3) % 3, fid / 9};
+        int d[3];
+        for (int c = 0; c < 3; c++) d[c] = t3[c] == 0 ? 1 : (t3[c] == 1 ? -1 : 0);
+        for (int m0 = 0; m0 < nm; m0 += 1024) {
+            const int m = m0 + tid;
+            const int hit = m < nm && code_sends(codes + 3 * m, d) ? 1 : 0;
+            const unsigned long long bal = __ballot(hit);
+            const int rank = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[wid] = __popcll(bal);
+            __syncthreads();
+            int off = base;
+            for (int w = 0; w < wid; w++) off += wsum[w];
+            if (hit) {
+                const int k = off + rank;
+                inst_mesh[k] = m;
+                for (int c = 0; c < 3; c++) inst_shift[3 * k + c] = (float)(-d[c] * L[c]);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                int tot = 0;
+                for (int w = 0; w < 16; w++) tot += wsum[w];
+                base += tot;
+            }
+            __syncthreads();
+        }
+    }
+    if (tid == 0) *n_inst = base;
+}
+
+// ---- search --------------------------------------------------------------
+template <class P>
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_find(MeshView m, int max_gid, const int *n_inst, float dt,
+                                                          int Lx, int Ly, int Lz, const int *starts, P parts,
+                                                          unsigned long long *keys) {
+    const int gid = blockIdx.x * MBB_THREADS + threadIdx.x;
+    if (gid >= max_gid || gid >= *n_inst * m.nt) return;
+    MPart A, B, C;
+    fetch_tri(m, dt, gid, &A, &B, &C);
+    revert(dt, &A);
+    revert(dt, &B);
+    revert(dt, &C);
+    const float tol = 0.1f;
+    const int L[3] = {Lx, Ly, Lz};
+    int lo[3], hi[3];
+    for (int c = 0; c < 3; c++) {
+        const float mn = fminf(fminf(A.r[c], B.r[c]), C.r[c]);
+        const float mx = fmaxf(fmaxf(A.r[c], B.r[c]), C.r[c]);
+        // a triangle wider than MBB_MAX_SPAN (or not finite) is no membrane
+        // triangle: skipped rather than swept over the whole domain
+        if (!(mx - mn <= MBB_MAX_SPAN)) return;
+        lo[c] = cell_of(L[c], mn - tol);
+        hi[c] = cell_of(L[c], mx + tol);
+    }
+    for (int iz = lo[2]; iz <= hi[2]; iz++)
+        for (int iy = lo[1]; iy <= hi[1]; iy++)
+            for (int ix = lo[0]; ix <= hi[0]; ix++) {
+                const int cid = ix + Lx * (iy + Ly * iz);
+                const int e = starts[cid + 1];
+                for (int i = starts[cid]; i < e; i++) {
+                    MPart p0;
+                    particle_path(parts, dt, i, &p0);
+                    float o[4];
+                    if (intersect(dt, A, B, C, p0, o) != BB_SUCCESS) continue;
+                    atomicMin(keys + i, make_key(o[0], gid));
+                }
+            }
+}
+
+// ---- bounce ----------------------------------------------------------------
+__device__ __forceinline__ float wrap1(float x, float L) {
+    const float h = 0.5f * L;
+    if (x < -h) x = x + L;
+    if (x >= h) x = x - L;
+    return x;
+}
+
+// bounce of particle i off instance triangle gid (src/mesh_bounce/dev/main.h:
+// 150-210); returns its momentum change (P, L) and writes the particle
+template <class P>
+__device__ void perform_one(const MeshView &m, const P &parts, float dt, float mass, int i, int gid, float pos[3],
+                            float vel[3], float dm[6]) {
+    MPart p1, p0, A, B, C;
+    parts.load(i, p1.r, p1.v);
+    particle_path(parts, dt, i, &p0);
+    fetch_tri(m, dt, gid, &A, &B, &C);
+    float d[4];
+    {
+        MPart a = A, b = B, c = C;
+        revert(dt, &a);
+        revert(dt, &b);
+        revert(dt, &c);
+        (void)intersect(dt, a, b, c, p0, d);
+    }
+    float rw[3], vw[3];
+    {
+        MPart a = A, b = B, c = C;
+        const float h = (float)((double)dt - (double)d[0]);
+        revert(h, &a);
+        revert(h, &b);
+        revert(h, &c);
+        const float u = d[1], v = d[2], w = 1 - u - v;
+        for (int k = 0; k < 3; k++) {
+            rw[k] = w * a.r[k] + u * b.r[k] + v * c.r[k];
+            vw[k] = w * a.v[k] + u * b.v[k] + v * c.v[k];
+        }
+    }
+    MPart pn;
+    for (int k = 0; k < 3; k++) {
+        pn.v[k] = 2 * vw[k] - p0.v[k];
+        pn.r[k] = (float)((double)rw[k] + ((double)dt - (double)d[0]) * (double)pn.v[k]);
+    }
+    {
+        float a[3], b[3], n[3];
+        for (int k = 0; k < 3; k++) {
+            a[k] = B.r[k] - A.r[k];
+            b[k] = C.r[k] - A.r[k];
+        }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        float s = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        s = 1.0f / sqrtf(s);
+        const float l = d[3] * 1e-2f;
+        for (int k = 0; k < 3; k++) pn.r[k] = pn.r[k] + l * (n[k] * s);
+    }
+    float rcen[3], r0[3], r1[3];
+    for (int k = 0; k < 3; k++) {
+        rcen[k] = (float)(0.333333 * (double)(A.r[k] + B.r[k] + C.r[k]));
+        r0[k] = p1.r[k] - rcen[k];
+        r1[k] = pn.r[k] - rcen[k];
+        dm[k] = -mass * (pn.v[k] - p1.v[k]);
+        pos[k] = pn.r[k];
+        vel[k] = pn.v[k];
+    }
+    const float *v0 = p1.v, *v1 = pn.v;
+    dm[3] = -mass * (r1[1] * v1[2] - r1[2] * v1[1] - r0[1] * v0[2] + r0[2] - v0[1]);
+    dm[4] = -mass * (r1[2] * v1[0] - r1[0] * v1[2] - r0[2] * v0[0] + r0[0] - v0[2]);
+    dm[5] = -mass * (r1[0] * v1[1] - r1[1] * v1[0] - r0[0] * v0[1] + r0[1] - v0[0]);
+}
+
+// one thread per particle: the bounce of the crossing particles, the reset
+// of the search keys and, when wrap is set, the periodic wrap of every
+// particle
+template <class P>
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_perform(MeshView m, int nmax, const int *n_dev, float dt,
+                                                             float mass, int wrap, float Lx, float Ly, float Lz,
+                                                             P parts, unsigned long long *keys, int *rec_tri,
+                                                             float *rec_m, int *tri_cnt,
+                                                             unsigned long long *ncol) {
+    const int i = blockIdx.x * MBB_THREADS + threadIdx.x;
+    if (i >= nmax || i >= *n_dev) return;
+    const unsigned long long key = keys[i];
+    int tri = -1;
+    float r[3], v[3];
+    if (key != MBB_EMPTY) {
+        keys[i] = MBB_EMPTY;
+        const int gid = (int)(key & 0xffffffffu);
+        float dm[6];
+        perform_one(m, parts, dt, mass, i, gid, r, v, dm);
+        const int k = gid / m.nt;
+        tri = m.inst_mesh[k] * m.nt + (gid - k * m.nt);
+        for (int c = 0; c < 6; c++) rec_m[6 * (size_t)i + c] = dm[c];
+        atomicAdd(tri_cnt + tri, 1);
+        atomicAdd(ncol, 1ull);
+    } else if (wrap) {
+        float u[3];
+        parts.load(i, r, u);
+        for (int c = 0; c < 3; c++) v[c] = u[c];
+    }
+    rec_tri[i] = tri;
+    if (tri >= 0 || wrap) {
+        if (wrap) {
+            r[0] = wrap1(r[0], Lx);
+            r[1] = wrap1(r[1], Ly);
+            r[2] = wrap1(r[2], Lz);
+        }
+        parts.store(i, r, v);
+    }
+}
+
+// slot of each crossing in its triangle's segment (order fixed afterwards)
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_scatter(int nmax, const int *n_dev, const int *rec_tri,
+                                                             const int *tri_start, int *tri_fill, int *tri_ids) {
+    const int i = blockIdx.x * MBB_THREADS + threadIdx.x;
+    if (i >= nmax || i >= *n_dev) return;
+    const int t = rec_tri[i];
+    if (t < 0) return;
+    const int slot = atomicAdd(tri_fill + t, 1);
+    tri_ids[tri_start[t] + slot] = i;
+}
+
+// ---- momentum to the vertices (src/mesh_bounce/dev/collect.h:84-103) ------
+enum { XX, XY, XZ, YY, YZ, ZZ };
+
+__device__ void inertia(float mass, const float a[3], const float b[3], const float c[3], float I[6]) {
+    const float inv_12 = 1.f / 12.f, inv_24 = 1.f / 24.f;
+    float M[3], d[3], e[3];
+    for (int k = 0; k < 3; k++)
+        M[k] = inv_12 * (a[k] * a[k] - a[k] * b[k] + b[k] * b[k] - c[k] * (a[k] + b[k]) + c[k] * c[k]);
+    d[0] = 2 * a[0] - b[0] - c[0];
+    d[1] = 2 * b[0] - a[0] - c[0];
+    d[2] = 2 * c[0] - a[0] - b[0];
+    e[0] = 2 * a[1] - b[1] - c[1];
+    e[1] = 2 * b[1] - a[1] - c[1];
+    e[2] = 2 * c[1] - a[1] - b[1];
+    I[XX] = mass * (M[1] + M[2]);
+    I[YY] = mass * (M[0] + M[2]);
+    I[ZZ] = mass * (M[0] + M[1]);
+    I[XY] = -inv_24 * mass * (d[0] * a[1] + d[1] * b[1] + d[2] * c[1]);
+    I[XZ] = -inv_24 * mass * (d[0] * a[2] + d[1] * b[2] + d[2] * c[2]);
+    I[YZ] = -inv_24 * mass * (e[0] * a[2] + e[1] * b[2] + e[2] * c[2]);
+}
+
+__device__ void inverse(const float A[6], float I[6]) {
+    const float mx = A[YY] * A[ZZ] - A[YZ] * A[YZ];
+    const float my = A[XY] * A[ZZ] - A[XZ] * A[YZ];
+    const float mz = A[XY] * A[YZ] - A[XZ] * A[YY];
+    float idet = mx * A[XX] - my * A[XY] + mz * A[XZ];
+    idet = 1.f / idet;
+    I[XX] = idet * mx;
+    I[XY] = -idet * my;
+    I[XZ] = idet * mz;
+    I[YY] = idet * (A[XX] * A[ZZ] - A[XZ] * A[XZ]);
+    I[YZ] = idet * (A[XY] * A[XZ] - A[XX] * A[YZ]);
+    I[ZZ] = idet * (A[XX] * A[YY] - A[XY] * A[XY]);
+}
+
+__device__ __forceinline__ bool nz(float a) { return fabsf(a) > 1e-6f; }
+
+// one thread per triangle: its momentum (crossings in ascending particle
+// order), and for a nonzero momentum the velocity change of its vertices
+// (rbc_M2v, src/mesh_bounce/dev/utils.h:172-213)
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_triangles(int ntri, int nt, int nv, const int *tt,
+                                                               const float *rc, float rbc_mass, const int *tri_start,
+                                                               int *tri_cnt, int *tri_fill, int *tri_ids,
+                                                               const float *rec_m, float *mm, float *tri_dv,
+                                                               int *tri_active) {
+    const int t = blockIdx.x * MBB_THREADS + threadIdx.x;
+    if (t >= ntri) return;
+    const int s = tri_start[t], e = s + tri_cnt[t];
+    tri_cnt[t] = 0;
+    tri_fill[t] = 0;
+    for (int a = s + 1; a < e; a++) {   // insertion sort: segments hold a few crossings
+        const int x = tri_ids[a];
+        int b = a - 1;
+        while (b >= s && tri_ids[b] > x) {
+            tri_ids[b + 1] = tri_ids[b];
+            b--;
+        }
+        tri_ids[b + 1] = x;
+    }
+    float m[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = s; a < e; a++) {
+        const int i = tri_ids[a];
+        for (int c = 0; c < 6; c++) m[c] += rec_m[6 * (size_t)i + c];
+    }
+    if (mm)
+        for (int c = 0; c < 6; c++) mm[6 * (size_t)t + c] = m[c];
+    const bool act = nz(m[0]) && nz(m[1]) && nz(m[2]) && nz(m[3]) && nz(m[4]) && nz(m[5]);
+    tri_active[t] = act ? 1 : 0;
+    if (!act) return;
+    const int cell = t / nt, tl = t - cell * nt;
+    float x[3][3];
+    for (int q = 0; q < 3; q++) {
+        const int j = cell * nv + tt[3 * tl + q];
+        for (int c = 0; c < 3; c++) x[q][c] = rc[6 * (size_t)j + c];
+    }
+    float I[6], Ii[6], om[3], v[3], com[3];
+    const float one_third = 1.f / 3.f;
+    inertia(rbc_mass, x[0], x[1], x[2], I);
+    inverse(I, Ii);
+    const float *L = m + 3;
+    om[0] = one_third * (Ii[XX] * L[0] + Ii[XY] * L[1] + Ii[XZ] * L[2]);
+    om[1] = one_third * (Ii[XY] * L[0] + Ii[YY] * L[1] + Ii[YZ] * L[2]);
+    om[2] = one_third * (Ii[XZ] * L[0] + Ii[YZ] * L[1] + Ii[ZZ] * L[2]);
+    const float inv_mass = (float)(1.0 / (double)rbc_mass);
+    for (int k = 0; k < 3; k++) {
+        v[k] = one_third * m[k] * inv_mass;
+        com[k] = one_third * (x[0][k] + x[1][k] + x[2][k]);
+    }
+    for (int q = 0; q < 3; q++) {
+        float r[3];
+        for (int k = 0; k < 3; k++) r[k] = x[q][k] - com[k];
+        float *o = tri_dv + 9 * (size_t)t + 3 * q;
+        o[0] = v[0] + r[1] * om[2] - r[2] * om[1];
+        o[1] = v[1] + r[2] * om[0] - r[0] * om[2];
+        o[2] = v[2] + r[0] * om[1] - r[1] * om[0];
+    }
+}
+
+// one thread per vertex: += the changes of its triangles in ascending order
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_vertices(int nvtx, int nt, int nv, const int *vstart,
+                                                              const int *vadj, const int *tri_active,
+                                                              const float *tri_dv, float *rc) {
+    const int j = blockIdx.x * MBB_THREADS + threadIdx.x;
+    if (j >= nvtx) return;
+    const int cell = j / nv, jl = j - cell * nv;
+    float v[3];
+    bool any = false;
+    for (int a = vstart[jl]; a < vstart[jl + 1]; a++) {
+        const int t = cell * nt + (vadj[a] >> 2), q = vadj[a] & 3;
+        if (!tri_active[t]) continue;
+        if (!any) {
+            for (int c = 0; c < 3; c++) v[c] = rc[6 * (size_t)j + 3 + c];
+            any = true;
+        }
+        for (int c = 0; c < 3; c++) v[c] += tri_dv[9 * (size_t)t + 3 * q + c];
+    }
+    if (any)
+        for (int c = 0; c < 3; c++) rc[6 * (size_t)j + 3 + c] = v[c];
+}
+
+}  // namespace
+}  // namespace udx
+
+using namespace udx;
+
+struct udx_mbb {
+    udx_ctx *ctx;
+    int nt, nv, max_meshes, maxp;
+    int *tt;                       // 3 per triangle
+    int *vstart, *vadj;            // vertex -> (triangle << 2 | corner), ascending triangle
+    int *codes, *inst_mesh, *n_inst;
+    float *inst_shift;
+    unsigned long long *keys, *ncol;
+    int *rec_tri, *tri_cnt, *tri_start, *tri_fill, *tri_ids, *tri_active, *scan_tmp;
+    float *rec_m, *tri_dv;
+    int *n_dev;                    // the particle count for the AoS entry point
+};
+
+namespace udx {
+
+static void mbb_free(udx_mbb *b) {
+    dev_free(b->tt);
+    dev_free(b->vstart);
+    dev_free(b->vadj);
+    dev_free(b->codes);
+    dev_free(b->inst_mesh);
+    dev_free(b->n_inst);
+    dev_free(b->inst_shift);
+    dev_free(b->keys);
+    dev_free(b->ncol);
+    dev_free(b->rec_tri);
+    dev_free(b->tri_cnt);
+    dev_free(b->tri_start);
+    dev_free(b->tri_fill);
+    dev_free(b->tri_ids);
+    dev_free(b->tri_active);
+    dev_free(b->scan_tmp);
+    dev_free(b->rec_m);
+    dev_free(b->tri_dv);
+    dev_free(b->n_dev);
+}
+
+udx_ctx *mbb_ctx(const udx_mbb *b) { return b->ctx; }
+int mbb_nv(const udx_mbb *b) { return b->nv; }
+
+// the bounce on either particle layout; wrap: apply the periodic wrap to
+// every particle (the solvent step defers it to here)
+template <class P>
+static int mbb_run(hipStream_t s, udx_mbb *b, float dt, float flu_mass, float rbc_mass, const int L[3], int nm,
+                   const float *rp, float *rc, const int *starts, int nmax, const int *n_dev, P parts, int wrap,
+                   float *mm) {
+    MeshView m{b->nt, b->nv, b->tt, rp, rc, b->inst_mesh, b->inst_shift};
+    const int ntri = nm * b->nt;
+    UDX_HIP(hipMemsetAsync(b->ncol, 0, sizeof(unsigned long long), s));
+    if (nm > 0) {
+        hipLaunchKernelGGL(k_mbb_codes, dim3(nm), dim3(MBB_THREADS), 0, s, b->nv, L[0], L[1], L[2], rc, b->codes);
+        hipLaunchKernelGGL(k_mbb_instances, dim3(1), dim3(1024), 0, s, nm, L[0], L[1], L[2], b->codes, b->inst_mesh,
+                           b->inst_shift, b->n_inst);
+        const int max_gid = MBB_MAX_IMAGES * ntri;
+        hipLaunchKernelGGL((k_mbb_find<P>), dim3(ceil_div(max_gid, MBB_THREADS)), dim3(MBB_THREADS), 0, s, m,
+                           max_gid, b->n_inst, dt, L[0], L[1], L[2], starts, parts, b->keys);
+    }
+    if (nmax > 0)
+        hipLaunchKernelGGL((k_mbb_perform<P>), dim3(ceil_div(nmax, MBB_THREADS)), dim3(MBB_THREADS), 0, s, m, nmax,
+                           n_dev, dt, flu_mass, wrap, (float)L[0], (float)L[1], (float)L[2], parts, b->keys,
+                           b->rec_tri, b->rec_m, b->tri_cnt, b->ncol);
+    if (nm > 0) {
+        UDX_TRY(scan_exclusive(s, b->tri_cnt, ntri, b->tri_start, b->scan_tmp));
+        if (nmax > 0)
+            hipLaunchKernelGGL(k_mbb_scatter, dim3(ceil_div(nmax, MBB_THREADS)), dim3(MBB_THREADS), 0, s, nmax,
+                               n_dev, b->rec_tri, b->tri_start, b->tri_fill, b->tri_ids);
+        hipLaunchKernelGGL(k_mbb_triangles, dim3(ceil_div(ntri, MBB_THREADS)), dim3(MBB_THREADS), 0, s, ntri, b->nt,
+                           b->nv, b->tt, rc, rbc_mass, b->tri_start, b->tri_cnt, b->tri_fill, b->tri_ids, b->rec_m,
+                           mm, b->tri_dv, b->tri_active);
+        hipLaunchKernelGGL(k_mbb_vertices, dim3(ceil_div(nm * b->nv, MBB_THREADS)), dim3(MBB_THREADS), 0, s,
+                           nm * b->nv, b->nt, b->nv, b->vstart, b->vadj, b->tri_active, b->tri_dv, rc);
+    }
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+int mbb_step(hipStream_t s, udx_mbb *b, float dt, float flu_mass, float rbc_mass, const Box &box, int nm,
+             const float *rp, float *rc, const int *starts, int nmax, const int *n_dev, float4 *pos, float4 *vel,
+             const float4 *pos0) {
+    UDX_CHECK(nm <= b->max_meshes && nmax <= b->maxp, UDX_ECAP, "membrane bounce: %d meshes / %d particles above "
+              "the capacity %d / %d", nm, nmax, b->max_meshes, b->maxp);
+    PartsF4 parts{pos, vel, pos0};
+    return mbb_run(s, b, dt, flu_mass, rbc_mass, box.L, nm, rp, rc, starts, nmax, n_dev, parts, 1, nullptr);
+}
+
+}  // namespace udx
+
+extern "C" {
+
+int udx_mbb_ini(udx_ctx *ctx, int nv, int nt, const int *tt_host, int max_meshes, int maxp, udx_mbb **out) {
+    UDX_CHECK(ctx && tt_host && out && nv > 0 && nt > 0 && max_meshes >= 0 && maxp >= 0, UDX_EINVAL,
+              "udx_mbb_ini: bad arguments");
+    UDX_CHECK((long)MBB_MAX_IMAGES * max_meshes * nt < (1L << 31), UDX_ECAP, "udx_mbb_ini: %d meshes of %d "
+              "triangles exceed the triangle index range", max_meshes, nt);
+    for (int k = 0; k < 3 * nt; k++)
+        UDX_CHECK(tt_host[k] >= 0 && tt_host[k] < nv, UDX_EINVAL, "udx_mbb_ini: vertex index %d out of range",
+                  tt_host[k]);
+    // vertex -> (triangle, corner) in ascending triangle order
+    std::vector<int> cnt(nv + 1, 0), adj(3 * (size_t)nt);
+    for (int k = 0; k < 3 * nt; k++) cnt[tt_host[k] + 1]++;
+    for (int v = 0; v < nv; v++) cnt[v + 1] += cnt[v];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int t = 0; t < nt; t++)
+        for (int q = 0; q < 3; q++) adj[fill[tt_host[3 * t + q]]++] = t << 2 | q;
+    udx_mbb *b = new udx_mbb();
+    b->ctx = ctx;
+    b->nt = nt;
+    b->nv = nv;
+    b->max_meshes = max_meshes;
+    b->maxp = maxp;
+    const int mm1 = max_meshes > 0 ? max_meshes : 1, np1 = maxp > 0 ? maxp : 1;
+    const int ntri = mm1 * nt;
+    int st = dev_alloc(&b->tt, 3 * (size_t)nt);
+    if (!st) st = dev_alloc(&b->vstart, (size_t)nv + 1);
+    if (!st) st = dev_alloc(&b->vadj, 3 * (size_t)nt);
+    if (!st) st = dev_alloc(&b->codes, 3 * (size_t)mm1);
+    if (!st) st = dev_alloc(&b->inst_mesh, (size_t)MBB_MAX_IMAGES * mm1);
+    if (!st) st = dev_alloc(&b->inst_shift, 3 * (size_t)MBB_MAX_IMAGES * mm1);
+    if (!st) st = dev_alloc(&b->n_inst, 1);
+    if (!st) st = dev_alloc(&b->keys, (size_t)np1);
+    if (!st) st = dev_alloc(&b->ncol, 1);
+    if (!st) st = dev_alloc(&b->rec_tri, (size_t)np1);
+    if (!st) st = dev_alloc(&b->rec_m, 6 * (size_t)np1);
+    if (!st) st = dev_alloc(&b->tri_cnt, (size_t)ntri + 1);
+    if (!st) st = dev_alloc(&b->tri_start, (size_t)ntri + 1);
+    if (!st) st = dev_alloc(&b->tri_fill, (size_t)ntri);
+    if (!st) st = dev_alloc(&b->tri_ids, (size_t)np1);
+    if (!st) st = dev_alloc(&b->tri_active, (size_t)ntri);
+    if (!st) st = dev_alloc(&b->tri_dv, 9 * (size_t)ntri);
+    if (!st) st = dev_alloc(&b->scan_tmp, (size_t)scan_scratch_size(ntri));
+    if (!st) st = dev_alloc(&b->n_dev, 1);
+    hipStream_t s = ctx->stream;
+    if (!st && (hipMemcpyAsync(b->tt, tt_host, 3 * sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(b->vstart, cnt.data(), sizeof(int) * ((size_t)nv + 1), hipMemcpyHostToDevice, s) !=
+                    hipSuccess ||
+                hipMemcpyAsync(b->vadj, adj.data(), 3 * sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s) !=
+                    hipSuccess ||
+                hipMemsetAsync(b->keys, 0xff, sizeof(unsigned long long) * (size_t)np1, s) != hipSuccess ||
+                hipMemsetAsync(b->tri_cnt, 0, sizeof(int) * ((size_t)ntri + 1), s) != hipSuccess ||
+                hipMemsetAsync(b->tri_fill, 0, sizeof(int) * (size_t)ntri, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        st = UDX_EHIP;
+    if (st) {
+        mbb_free(b);
+        delete b;
+        set_error("udx_mbb_ini: allocation failed");
+        return st;
+    }
+    *out = b;
+    return UDX_OK;
+}
+
+int udx_mbb_fin(udx_mbb *b) {
+    if (!b) return UDX_OK;
+    UDX_HIP(hipStreamSynchronize(b->ctx->stream));
+    mbb_free(b);
+    delete b;
+    return UDX_OK;
+}
+
+int udx_mbb_bounce(udx_mbb *b, float dt, float flu_mass, float rbc_mass, int Lx, int Ly, int Lz, int nm,
+                   const float *rp, float *rc, const int *starts, int n, const float *pp0, float *pp, float *mm) {
+    UDX_CHECK(b && dt > 0 && flu_mass > 0 && rbc_mass > 0 && Lx > 0 && Ly > 0 && Lz > 0 && nm >= 0 && n >= 0,
+              UDX_EINVAL, "udx_mbb_bounce: bad arguments");
+    UDX_CHECK(nm == 0 || (rp && rc), UDX_EINVAL, "udx_mbb_bounce: null vertices");
+    UDX_CHECK(n == 0 || (pp && pp0 && starts), UDX_EINVAL, "udx_mbb_bounce: null solvent");
+    UDX_CHECK(nm <= b->max_meshes && n <= b->maxp, UDX_ECAP, "udx_mbb_bounce: %d meshes / %d particles above the "
+              "capacity %d / %d", nm, n, b->max_meshes, b->maxp);
+    hipStream_t s = b->ctx->stream;
+    UDX_HIP(hipMemcpyAsync(b->n_dev, &n, sizeof n, hipMemcpyHostToDevice, s));
+    const int L[3] = {Lx, Ly, Lz};
+    PartsAoS parts{pp, pp0};
+    UDX_TRY(mbb_run(s, b, dt, flu_mass, rbc_mass, L, nm, rp, rc, starts, n, b->n_dev, parts, 0, mm));
+    UDX_HIP(hipStreamSynchronize(s));
+    return UDX_OK;
+}
+
+int udx_mbb_count(udx_mbb *b, unsigned long long *ncol) {
+    UDX_CHECK(b && ncol, UDX_EINVAL, "udx_mbb_count: null argument");
+    UDX_HIP(hipMemcpyAsync(ncol, b->ncol, sizeof *ncol, hipMemcpyDeviceToHost, b->ctx->stream));
+    UDX_HIP(hipStreamSynchronize(b->ctx->stream));
+    return UDX_OK;
+}
+
+const int *udx_mbb_collisions(const udx_mbb *b) { return b ? b->rec_tri : nullptr; }
+
+}  // extern "C"

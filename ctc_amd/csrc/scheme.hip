@@ -10,7 +10,7 @@ namespace {
 
 __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
                                                  float dt, float mass, float ax, float ay, float az,
-                                                 const float4 *ff, float4 *pos, float4 *vel) {
+                                                 const float4 *ff, float4 *pos, float4 *vel, int wrap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nmax || i >= *n_dev) return;
     float4 p = pos[i], v = vel[i];
@@ -18,9 +18,11 @@ __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, flo
     kick_drift(dt, mass, ax, f.x, &p.x, &v.x);
     kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
     kick_drift(dt, mass, az, f.z, &p.z, &v.z);
-    p.x = wrap_coord(p.x, Lx);
-    p.y = wrap_coord(p.y, Ly);
-    p.z = wrap_coord(p.z, Lz);
+    if (wrap) {
+        p.x = wrap_coord(p.x, Lx);
+        p.y = wrap_coord(p.y, Ly);
+        p.z = wrap_coord(p.z, Lz);
+    }
     pos[i] = p;
     vel[i] = v;
 }
@@ -31,7 +33,7 @@ __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, flo
 __global__ __launch_bounds__(256) void k_move_f4_wall(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
                                                       float dt, float mass, float ax, float ay, float az,
                                                       const float4 *ff, float4 *pos, float4 *vel, SdfView sdf,
-                                                      WallVel wv) {
+                                                      WallVel wv, int wrap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nmax || i >= *n_dev) return;
     float4 p = pos[i], v = vel[i];
@@ -44,9 +46,11 @@ __global__ __launch_bounds__(256) void k_move_f4_wall(int nmax, const int *n_dev
         p.x = r[0]; p.y = r[1]; p.z = r[2];
         v.x = u[0]; v.y = u[1]; v.z = u[2];
     }
-    p.x = wrap_coord(p.x, Lx);
-    p.y = wrap_coord(p.y, Ly);
-    p.z = wrap_coord(p.z, Lz);
+    if (wrap) {
+        p.x = wrap_coord(p.x, Lx);
+        p.y = wrap_coord(p.y, Ly);
+        p.z = wrap_coord(p.z, Lz);
+    }
     pos[i] = p;
     vel[i] = v;
 }
@@ -188,21 +192,21 @@ int f4_to_force_aos(hipStream_t s, int n, const int *n_dev, const float4 *ff, fl
 }
 
 int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-            const float4 *ff, float4 *pos, float4 *vel) {
+            const float4 *ff, float4 *pos, float4 *vel, int wrap) {
     if (nmax <= 0) return UDX_OK;
     hipLaunchKernelGGL(k_move_f4, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
-                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel);
+                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, wrap);
     UDX_LAUNCHED();
     return UDX_OK;
 }
 
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv) {
-    if (!q) return move_f4(s, nmax, n_dev, b, dt, mass, accel, ff, pos, vel);
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv, int wrap) {
+    if (!q) return move_f4(s, nmax, n_dev, b, dt, mass, accel, ff, pos, vel, wrap);
     if (nmax <= 0) return UDX_OK;
     hipLaunchKernelGGL(k_move_f4_wall, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
                        (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, q->view,
-                       wv);
+                       wv, wrap);
     UDX_LAUNCHED();
     return UDX_OK;
 }

@@ -148,7 +148,7 @@ int aos_to_f4(hipStream_t s, int n, const float *pp, float4 *pos, float4 *vel);
 int f4_to_aos(hipStream_t s, int n, const int *n_dev, const float4 *pos, const float4 *vel, float *pp);
 int f4_to_force_aos(hipStream_t s, int n, const int *n_dev, const float4 *ff, float *out);
 int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass,
-            const float accel[3], const float4 *ff, float4 *pos, float4 *vel);
+            const float accel[3], const float4 *ff, float4 *pos, float4 *vel, int wrap = 1);
 int move_aos(hipStream_t s, int n, float dt, float mass, const float *ff, float *pp);
 int bforce_aos(hipStream_t s, int n, float mass, const float accel[3], float *ff);
 int wrap_aos(hipStream_t s, const Box &b, int n, float *pp);
@@ -183,7 +183,16 @@ int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp
 WallVel wall_vel_view(const udx_wall_vel &v, float t);
 int wall_repulse_aos(hipStream_t s, const SdfField *q, float lambda, int n, const float *pp, float *ff);
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv);
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv, int wrap = 1);
+// membrane bounce-back inside the solvent step: rp, rc the vertices (AoS) at
+// the start of the step and after the membrane advance (velocities updated),
+// pos0 the solvent positions at the start of the step; applies the periodic
+// wrap to every particle
+udx_ctx *mbb_ctx(const udx_mbb *b);
+int mbb_nv(const udx_mbb *b);
+int mbb_step(hipStream_t s, udx_mbb *b, float dt, float flu_mass, float rbc_mass, const Box &box, int nm,
+             const float *rp, float *rc, const int *starts, int nmax, const int *n_dev, float4 *pos, float4 *vel,
+             const float4 *pos0);
 int diag_aos(hipStream_t s, int n, const float *pp, double *partials_dev, double out[5]);
 
 // solvent trunk f of one rank: the halo fragments 0..25

@@ -128,6 +128,12 @@ _SIGS = {
     "udx_flu_strt_quants": (_I, [_VP, C.c_char_p, _I]),
     "udx_wall_strt_dump_templ": (_I, [_VP, C.c_char_p]),
     "udx_wall_strt_gen": (_I, [_VP, _VP, C.c_char_p, _I, C.POINTER(_VP)]),
+    "udx_mbb_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, _I, C.POINTER(_VP)]),
+    "udx_mbb_fin": (_I, [_VP]),
+    "udx_mbb_bounce": (_I, [_VP, _F, _F, _F, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP]),
+    "udx_mbb_count": (_I, [_VP, C.POINTER(C.c_ulonglong)]),
+    "udx_mbb_collisions": (_VP, [_VP]),
+    "udx_flu_set_rbc_bounce": (_I, [_VP, _VP]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
     "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
     "udx_rbc_fin": (_I, [_VP]),
@@ -428,6 +434,11 @@ class Flu:
         """the step clock (time of the next step), advanced by dt per step"""
         _ok(lib().udx_flu_set_time(self.h, float(t)))
 
+    def set_rbc_bounce(self, mbb):
+        """membrane bounce-back in the step (udx_flu_set_rbc_bounce); None: off"""
+        self._mbb = mbb
+        _ok(lib().udx_flu_set_rbc_bounce(self.h, mbb.h if mbb is not None else None))
+
     def strt_dump(self, base, id):
         _ok(lib().udx_flu_strt_dump(self.h, str(base).encode(), id))
 
@@ -575,6 +586,44 @@ class Wall:
     def close(self):
         if self.h:
             lib().udx_wall_fin(self.h)
+            self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- membrane bounce-back
+class Mbb:
+    """mesh_bounce_* (src/mesh_bounce/imp.h): solvent bounce-back on moving
+    membrane meshes of one template"""
+
+    def __init__(self, ctx, nv, tt, max_meshes, maxp):
+        import numpy as np
+        self.ctx, self.nv = ctx, nv
+        tt = np.ascontiguousarray(tt, np.int32)
+        self.nt = len(tt)
+        self.h = C.c_void_p()
+        _ok(lib().udx_mbb_ini(ctx.h, nv, self.nt, tt.ctypes.data_as(C.POINTER(C.c_int)), max_meshes, maxp,
+                              C.byref(self.h)))
+
+    def bounce(self, dt, flu_mass, rbc_mass, L, nm, rp, rc, starts, pp0, pp, mm=None):
+        """rp, rc: (nm nv, 6) cuda tensors (rc's velocities updated); starts:
+        int32 cell starts of pp0; pp0, pp: (n, 6) cuda tensors (pp bounced)"""
+        _ok(lib().udx_mbb_bounce(self.h, dt, flu_mass, rbc_mass, L[0], L[1], L[2], nm, _ptr(rp), _ptr(rc),
+                                 _ptr(starts), pp.shape[0], _ptr(pp0), _ptr(pp),
+                                 _ptr(mm) if mm is not None else None))
+
+    def count(self):
+        v = C.c_ulonglong()
+        _ok(lib().udx_mbb_count(self.h, C.byref(v)))
+        return v.value
+
+    def collisions(self, n):
+        import torch
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
+        _copy_dev(out, lib().udx_mbb_collisions(self.h), n * 4)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().udx_mbb_fin(self.h)
             self.h = C.c_void_p()
 
 

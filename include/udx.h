@@ -295,6 +295,40 @@ int udx_fsi_pairs(udx_fsi *f, unsigned long long *npairs, int reset);
 int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *rp, const udx_pair_dpd *fp,
                     float mass, int substeps, int nc, float *pp);
 
+/* ---- membrane bounce-back: replaces mesh_bounce_ini / _fin /
+ *      _find_collisions / _select_collisions / _bounce /
+ *      _collect_rbc_momentum (src/mesh_bounce/imp.h:17-31) and the mesh
+ *      halo of src/exch/mesh on one periodic domain ----
+ * Solvent particles that cross a membrane triangle during the step are
+ * reflected with the triangle's velocity at the crossing point and pushed
+ * 0.01 back to their side; the momentum they give each triangle changes the
+ * velocities of its three vertices.  Triangles move linearly from rp (the
+ * vertices at the start of the step) to rc (after the membrane advance);
+ * particles from pp0 (start of the step) to pp (after the move, not
+ * wrapped).  Meshes leaving the domain cropped by 1 also act through their
+ * periodic images.  starts: the solvent cell list of pp0 (L^3 + 1 entries,
+ * cells of unit size, x fastest); pp must be in that order.  All pointers
+ * are device memory; rp and rc hold 6 floats per vertex (rc's velocities are
+ * updated), mm (optional) receives P and L per triangle (6 floats).  A
+ * particle takes its earliest crossing (ties: the lower triangle instance),
+ * a triangle's momentum sums its crossings in particle order and a vertex
+ * adds its triangles' changes in triangle order, so results do not depend
+ * on scheduling. */
+typedef struct udx_mbb udx_mbb;
+int udx_mbb_ini(udx_ctx *ctx, int nv, int nt, const int *tt_host, int max_meshes, int maxp, udx_mbb **b);
+int udx_mbb_fin(udx_mbb *b);
+int udx_mbb_bounce(udx_mbb *b, float dt, float flu_mass, float rbc_mass, int Lx, int Ly, int Lz, int nm,
+                   const float *rp, float *rc, const int *starts, int n, const float *pp0, float *pp, float *mm);
+/* particles bounced by the last call */
+int udx_mbb_count(udx_mbb *b, unsigned long long *ncol);
+/* per particle of the last call: the triangle (mesh * nt + t) it bounced
+ * off, -1 for none (device memory, n entries) */
+const int *udx_mbb_collisions(const udx_mbb *b);
+/* bounce-back inside udx_flu_run after the membrane advance
+ * (objects_bounce, src/sim/objects/imp/bounce.h:236-248); NULL switches it
+ * off.  Needs the membranes of udx_flu_set_rbc; the same template. */
+int udx_flu_set_rbc_bounce(udx_flu *f, udx_mbb *b);
+
 /* ---- configuration: replaces conf_ini / conf_read / conf_lookup_* /
  *      conf_set_* (src/conf/imp.h) and the parameter readers pair_set_conf
  *      (src/pair/conf.cpp:17-57) and rbc_params_set_conf

@@ -215,13 +215,26 @@ def dpd_forces(L, pp, starts, counts, params, seeds, periodic=True, nthreads=1, 
     return ff, int(np_)
 
 
-def integrate(L, pp, ff, dt, mass=1.0, accel=(0.0, 0.0, 0.0)):
+def integrate(L, pp, ff, dt, mass=1.0, accel=(0.0, 0.0, 0.0), wrap=True):
     L = _L(L)
     pp = _f32(pp).reshape(-1, 6).copy()
     ff = _f32(ff)
     acc = _f32(accel)
-    lib().orc_integrate(_ptr(L, I32P), pp.shape[0], np.float32(dt), np.float32(mass),
-                        _ptr(acc, F32P), _ptr(ff, F32P), _ptr(pp, F32P))
+    f = lib().orc_integrate_ex
+    f.restype = None
+    f.argtypes = [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P, C.c_int]
+    f(_ptr(L, I32P), pp.shape[0], np.float32(dt), np.float32(mass), _ptr(acc, F32P), _ptr(ff, F32P),
+      _ptr(pp, F32P), 1 if wrap else 0)
+    return pp
+
+
+def wrap(L, pp):
+    L = _L(L)
+    pp = _f32(pp).reshape(-1, 6).copy()
+    f = lib().orc_wrap
+    f.restype = None
+    f.argtypes = [I32P, C.c_int, F32P]
+    f(_ptr(L, I32P), pp.shape[0], _ptr(pp, F32P))
     return pp
 
 

@@ -234,22 +234,30 @@ def redistribute(L, opp, nc, nv):
 
 
 def suspension_step(L, spp, opp, mesh, nc, dpd, seeds28, fsi_par, fsi_seed2, p, dt, mass=1.0, rbc_mass=0.5,
-                    substeps=1, stress_free=None, l0=0.0, a0=0.0, nthreads=1):
+                    substeps=1, stress_free=None, l0=0.0, a0=0.0, nthreads=1, bounce=False):
     """One step of solvent + membranes on one periodic domain in the order of
     src/sim/imp/run.h:8-68: solvent cell list, membrane redistribution,
     solvent DPD, FSI (+f vertices, -f solvent), solvent kick-drift + wrap,
     then per sub-step the vertex force = FSI force + float(membrane force)
     and the vertex kick-drift (src/sim/objects/imp/main.h:40-61).  Returns
-    (cell-ordered solvent, vertices, solvent pairs, FSI pairs)."""
+    (cell-ordered solvent, vertices, solvent pairs, FSI pairs).  With bounce,
+    the solvent bounces back off the membranes after the advance
+    (src/sim/imp/run.h:54, src/sim/objects/imp/bounce.h:236-248) before the
+    wrap, and the number of bounced particles is returned fifth."""
     from . import oracle as O
     starts, counts, srt, _ = O.clist_build(L, spp)
     opp, _ = redistribute(L, opp, nc, mesh.nv)
+    start = opp.copy()
     ff, npair = O.dpd_forces(L, srt, starts, counts, dpd, seeds28[:27], nthreads=nthreads)
     fo, ff, nfsi = fsi_forces(L, srt, starts, opp, fsi_par, fsi_seed2, fs=ff)
-    sout = O.integrate(L, srt, ff, dt, mass)
+    sout = O.integrate(L, srt, ff, dt, mass, wrap=not bounce)
     sub_dt = np.float32(np.float32(dt) / np.float32(substeps))
     for _ in range(substeps):
         F = forces(p, mesh, opp, nc, area_volume(mesh, opp, nc), stress_free=stress_free, l0=l0, a0=a0)
         ffast = fo + F.astype(np.float32)
         opp = O.move(opp, ffast, sub_dt, rbc_mass)
-    return sout, opp, npair // 2, nfsi
+    if not bounce:
+        return sout, opp, npair // 2, nfsi
+    from . import mbb as B
+    sout, opp, _, _, ncol = B.bounce(dt, mass, rbc_mass, L, nc, mesh.tt, start[:, :3], opp, starts, srt, sout)
+    return O.wrap(L, sout), opp, npair // 2, nfsi, ncol

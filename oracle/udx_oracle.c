@@ -458,8 +458,8 @@ static float wrap_coord(float x, float L) {
     return x;
 }
 
-void orc_integrate(const int L[3], int n, float dt, float mass, const float accel[3],
-                   const float *ff, float *pp) {
+void orc_integrate_ex(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
+                      float *pp, int wrap) {
     for (int i = 0; i < n; i++) {
         float *r = pp + 6 * i, *v = r + 3;
         const float *f = ff + 3 * i;
@@ -467,9 +467,19 @@ void orc_integrate(const int L[3], int n, float dt, float mass, const float acce
             float ft = fmaf(mass, accel[k], f[k]);
             v[k] = fmaf(ft / mass, dt, v[k]);
             r[k] = fmaf(v[k], dt, r[k]);
-            r[k] = wrap_coord(r[k], (float)L[k]);
+            if (wrap) r[k] = wrap_coord(r[k], (float)L[k]);
         }
     }
+}
+
+void orc_integrate(const int L[3], int n, float dt, float mass, const float accel[3],
+                   const float *ff, float *pp) {
+    orc_integrate_ex(L, n, dt, mass, accel, ff, pp, 1);
+}
+
+void orc_wrap(const int L[3], int n, float *pp) {
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) pp[6 * i + k] = wrap_coord(pp[6 * i + k], (float)L[k]);
 }
 
 /* kick-drift without wrap, for membrane vertices (src/scheme/move/dev/vv.h:1-10) */

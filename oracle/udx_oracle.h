@@ -87,6 +87,11 @@ long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, c
 void orc_integrate(const int L[3], int n, float dt, float mass, const float accel[3],
                    const float *ff, float *pp);
 void orc_move(int n, float dt, float mass, const float *ff, float *pp);   /* no wrap */
+/* orc_integrate with the wrap optional (the membrane bounce-back wraps after
+ * the bounce), and the wrap on its own */
+void orc_integrate_ex(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
+                      float *pp, int wrap);
+void orc_wrap(const int L[3], int n, float *pp);
 
 /* ---- diagnostics: src/io/diag/part/imp/main.h:37-63 (FP64) -------------- */
 /* out: [0..2] momentum (sum v), [3] kBT = 0.5*sum|v|^2/(1.5 n), [4] max |v|^2 */

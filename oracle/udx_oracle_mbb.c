@@ -35,7 +35,8 @@
  *  - a triangle's momentum sums its collisions (all its instances) in
  *    ascending particle index; a vertex adds the velocity changes of its
  *    triangles in ascending triangle index.
- * Other definitions: 1/sqrt for rsqrtf in the push; the "nonzero" test of
+ * Other definitions: a triangle whose start-of-step bounding box is wider
+ * than 8 (or not finite) is not searched; 1/sqrt for rsqrtf in the push; the "nonzero" test of
  * src/mesh_bounce/dev/utils.h:94-99 (every component of P and L above 1e-6
  * in magnitude) on a triangle's total momentum; the angular momentum change
  * as the reference writes it (dev/utils.h:76-84).
@@ -329,16 +330,18 @@ void orc_mbb_find(float dt, const int L[3], int ninst, const int *inst_mesh, con
         revert(dt, &A);
         revert(dt, &B);
         revert(dt, &C);
-        int lo[3], hi[3];
+        int lo[3], hi[3], skip = 0;
         for (int c = 0; c < 3; c++) {
             float mn = A.r[c], mx = A.r[c];
             mn = B.r[c] < mn ? B.r[c] : mn;
             mn = C.r[c] < mn ? C.r[c] : mn;
             mx = B.r[c] > mx ? B.r[c] : mx;
             mx = C.r[c] > mx ? C.r[c] : mx;
+            if (!(mx - mn <= 8.0f)) skip = 1;   /* no membrane triangle: not searched */
             lo[c] = cell_of(L[c], mn - tol);
             hi[c] = cell_of(L[c], mx + tol);
         }
+        if (skip) continue;
         for (int iz = lo[2]; iz <= hi[2]; iz++)
             for (int iy = lo[1]; iy <= hi[1]; iy++)
                 for (int ix = lo[0]; ix <= hi[0]; ix++) {
