@@ -486,7 +486,22 @@ def suspension_leg(ctx, steps, warmup, cpu):
     flu.timing(True)
     flu.run(3)
     ph = flu.timing(False)
+    # the same steps with the solvent bounced back off the membranes
+    bb = U.Mbb(ctx, nv, tt, nc, n)
+    flu.set_rbc_bounce(bb)
+    flu.run(2)
+    torch.cuda.synchronize()
+    bounced = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        flu.run(1)
+        bounced += bb.count()
+    torch.cuda.synchronize()
+    sec_bb = time.perf_counter() - t0
+    flu.set_rbc_bounce(None)
+    bb.close()
     out = {"cells": nc, "vertices": nc * nv, "solvent": n, "steps": steps, "ms_per_step": sec / steps * 1e3,
+           "ms_per_step_with_bounce": sec_bb / steps * 1e3, "bounced_per_step": bounced / steps,
            "steps_per_s": steps / sec, "pairs_per_s": (npair + nfsi) / sec,
            "solvent_pairs_per_step": npair / steps, "fsi_pairs_per_step": nfsi / steps,
            "phase_ms": {"clist_and_redistribution": ph[0], "dpd": ph[1], "fsi": ph[2],

@@ -26,7 +26,9 @@ namespace {
 constexpr int MBB_THREADS = 256;
 constexpr int MBB_MAX_IMAGES = 8;   // a mesh and at most 7 images (3 faces, 3 edges, 1 corner)
 constexpr unsigned long long MBB_EMPTY = ~0ull;
-constexpr float MBB_MAX_SPAN = 8.0f;   // the widest triangle searched (a start-of-step bounding box side)
+constexpr float MBB_MAX_SPAN = 8.0f;     // the widest triangle searched (a start-of-step bounding box side)
+constexpr int MBB_WQ = 256;              // per-wave LDS list of candidate pairs (4 per lane)
+constexpr int MBB_CAND_BLOCKS = 1024;    // grid of the dense crossing-test pass
 enum { BB_SUCCESS = 0, BB_NOCROSS = 1, BB_WTRIANGLE = 2, BB_HFAIL = 3 };
 
 struct MPart {
@@ -339,86 +341,279 @@ __device__ bool code_sends(const int *code, const int d[3]) {
     return any;
 }
 
-// one block: meshes unshifted, then images by fragment id, meshes ascending
-__global__ __launch_bounds__(1024) void k_mbb_instances(int nm, int Lx, int Ly, int Lz, const int *codes,
-                                                        int *inst_mesh, float *inst_shift, int *n_inst) {
-    __shared__ int wsum[16];
-    __shared__ int base;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int m = tid; m < nm; m += 1024) {
-        inst_mesh[m] = m;
-        inst_shift[3 * m] = inst_shift[3 * m + 1] = inst_shift[3 * m + 2] = 0.f;
-    }
-    if (tid == 0) base = nm;
-    __syncthreads();
-    const int L[3] = {Lx, Ly, Lz};
+// one block: meshes unshifted, then images by fragment id, meshes ascending.
+// Pass 1 counts the meshes of every fragment, pass 2 places them: fragment
+// f's images start after the meshes and the images of fragments below f.
+__device__ __forceinline__ unsigned code_mask(const int *code) {
+    unsigned mask = 0;
     for (int fid = 0; fid < 26; fid++) {
         const int t3[3] = {fid % 3, (fid / 3) % 3, fid / 9};
         int d[3];
         for (int c = 0; c < 3; c++) d[c] = t3[c] == 0 ? 1 : (t3[c] == 1 ? -1 : 0);
-        for (int m0 = 0; m0 < nm; m0 += 1024) {
-            const int m = m0 + tid;
-            const int hit = m < nm && code_sends(codes + 3 * m, d) ? 1 : 0;
-            const unsigned long long bal = __ballot(hit);
-            const int rank = __popcll(bal & ((1ull << lane) - 1ull));
-            if (lane == 0) wsum[wid] = __popcll(bal);
-            __syncthreads();
-            int off = base;
-            for (int w = 0; w < wid; w++) off += wsum[w];
-            if (hit) {
-                const int k = off + rank;
-                inst_mesh[k] = m;
-                for (int c = 0; c < 3; c++) inst_shift[3 * k + c] = (float)(-d[c] * L[c]);
-            }
-            __syncthreads();
-            if (tid == 0) {
-                int tot = 0;
-                for (int w = 0; w < 16; w++) tot += wsum[w];
-                base += tot;
-            }
-            __syncthreads();
-        }
+        if (code_sends(code, d)) mask |= 1u << fid;
     }
-    if (tid == 0) *n_inst = base;
+    return mask;
+}
+
+__global__ __launch_bounds__(1024) void k_mbb_instances(int nm, int Lx, int Ly, int Lz, const int *codes,
+                                                        int *inst_mesh, float *inst_shift, int *n_inst) {
+    __shared__ int fcount[26], fbase[26];
+    __shared__ int wcount[16][26];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid < 26) fcount[tid] = 0;
+    for (int m = tid; m < nm; m += 1024) {
+        inst_mesh[m] = m;
+        inst_shift[3 * m] = inst_shift[3 * m + 1] = inst_shift[3 * m + 2] = 0.f;
+    }
+    __syncthreads();
+    for (int m = tid; m < nm; m += 1024) {
+        const unsigned mask = code_mask(codes + 3 * m);
+        for (int fid = 0; fid < 26; fid++)
+            if (mask >> fid & 1u) atomicAdd(&fcount[fid], 1);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int b = nm;
+        for (int fid = 0; fid < 26; fid++) {
+            fbase[fid] = b;
+            b += fcount[fid];
+        }
+        *n_inst = b;
+    }
+    __syncthreads();
+    const int L[3] = {Lx, Ly, Lz};
+    for (int m0 = 0; m0 < nm; m0 += 1024) {
+        const int m = m0 + tid;
+        const unsigned mask = m < nm ? code_mask(codes + 3 * m) : 0u;
+        unsigned long long bal[26];
+        for (int fid = 0; fid < 26; fid++) {
+            bal[fid] = __ballot((mask >> fid) & 1u);
+            if (lane == 0) wcount[wid][fid] = __popcll(bal[fid]);
+        }
+        __syncthreads();
+        for (int fid = 0; fid < 26; fid++) {
+            if (!((mask >> fid) & 1u)) continue;
+            int off = fbase[fid] + __popcll(bal[fid] & ((1ull << lane) - 1ull));
+            for (int w = 0; w < wid; w++) off += wcount[w][fid];
+            const int t3[3] = {fid % 3, (fid / 3) % 3, fid / 9};
+            inst_mesh[off] = m;
+            for (int c = 0; c < 3; c++) {
+                const int d = t3[c] == 0 ? 1 : (t3[c] == 1 ? -1 : 0);
+                inst_shift[3 * off + c] = (float)(-d * L[c]);
+            }
+        }
+        __syncthreads();
+        if (tid < 26) {
+            int tot = 0;
+            for (int w = 0; w < 16; w++) tot += wcount[w][tid];
+            fbase[tid] += tot;
+        }
+        __syncthreads();
+    }
 }
 
 // ---- search --------------------------------------------------------------
+// the straight-line velocity of every particle over the step, once per call
+// (the search tests each particle against several triangles)
+template <class P>
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_paths(int nmax, const int *n_dev, float dt, P parts,
+                                                           float4 *v0) {
+    const int i = blockIdx.x * MBB_THREADS + threadIdx.x;
+    if (i >= nmax || i >= *n_dev) return;
+    MPart p0;
+    particle_path(parts, dt, i, &p0);
+    v0[i] = make_float4(p0.v[0], p0.v[1], p0.v[2], 0.f);
+}
+
+// The plane test (the particle on both sides of the triangle's plane over
+// the step) rejects almost every candidate; the few that pass go through the
+// full crossing test in a second, dense pass, so that no wave serialises on
+// one lane's cubic.  Each wave gathers its (triangle, particle) pairs in LDS
+// and appends them to a global list with one atomic per MBB_WQ pairs.
 template <class P>
 __global__ __launch_bounds__(MBB_THREADS) void k_mbb_find(MeshView m, int max_gid, const int *n_inst, float dt,
                                                           int Lx, int Ly, int Lz, const int *starts, P parts,
-                                                          unsigned long long *keys) {
+                                                          const float4 *v0, unsigned long long *keys, int cap,
+                                                          unsigned *cand_n, uint2 *cand) {
+    __shared__ uint2 wq_all[MBB_THREADS / 64][MBB_WQ];
+    const int lane = threadIdx.x & 63;
+    uint2 *wq = wq_all[threadIdx.x >> 6];
+    const unsigned long long below = (1ull << lane) - 1ull;
     const int gid = blockIdx.x * MBB_THREADS + threadIdx.x;
-    if (gid >= max_gid || gid >= *n_inst * m.nt) return;
+    bool active = gid < max_gid && gid < *n_inst * m.nt;
+    if (!wave_any(active)) return;   // whole waves only: the queue below is wave-wide
     MPart A, B, C;
-    fetch_tri(m, dt, gid, &A, &B, &C);
-    revert(dt, &A);
-    revert(dt, &B);
-    revert(dt, &C);
-    const float tol = 0.1f;
-    const int L[3] = {Lx, Ly, Lz};
-    int lo[3], hi[3];
-    for (int c = 0; c < 3; c++) {
-        const float mn = fminf(fminf(A.r[c], B.r[c]), C.r[c]);
-        const float mx = fmaxf(fmaxf(A.r[c], B.r[c]), C.r[c]);
-        // a triangle wider than MBB_MAX_SPAN (or not finite) is no membrane
-        // triangle: skipped rather than swept over the whole domain
-        if (!(mx - mn <= MBB_MAX_SPAN)) return;
-        lo[c] = cell_of(L[c], mn - tol);
-        hi[c] = cell_of(L[c], mx + tol);
+    const double ddt = dt;
+    double n0[3] = {0, 0, 0}, n1[3] = {0, 0, 0}, s11[3] = {0, 0, 0};
+    int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+    if (active) {
+        fetch_tri(m, dt, gid, &A, &B, &C);
+        revert(dt, &A);
+        revert(dt, &B);
+        revert(dt, &C);
+        const float tol = 0.1f;
+        const int L[3] = {Lx, Ly, Lz};
+        for (int c = 0; c < 3; c++) {
+            const float mn = fminf(fminf(A.r[c], B.r[c]), C.r[c]);
+            const float mx = fmaxf(fmaxf(A.r[c], B.r[c]), C.r[c]);
+            // a triangle wider than MBB_MAX_SPAN (or not finite) is no
+            // membrane triangle: skipped rather than swept over the domain
+            if (!(mx - mn <= MBB_MAX_SPAN)) active = false;
+            lo[c] = cell_of(L[c], mn - tol);
+            hi[c] = cell_of(L[c], mx + tol);
+        }
+        // the plane terms that depend on the triangle only (the operations
+        // of intersect(), so the same decisions)
+        double a1[3], a2[3], at1[3], at2[3], nt[3], ntt[3];
+        for (int c = 0; c < 3; c++) {
+            a1[c] = (double)(B.r[c] - A.r[c]);
+            a2[c] = (double)(C.r[c] - A.r[c]);
+            at1[c] = (double)(B.v[c] - A.v[c]);
+            at2[c] = (double)(C.v[c] - A.v[c]);
+        }
+        dcross(a1, a2, n0);
+        dcross(at1, at2, ntt);
+        nt[0] = a1[1] * at2[2] - a1[2] * at2[1] + at1[1] * a2[2] - at1[2] * a2[1];
+        nt[1] = a1[2] * at2[0] - a1[0] * at2[2] + at1[2] * a2[0] - at1[0] * a2[2];
+        nt[2] = a1[0] * at2[1] - a1[1] * at2[0] + at1[0] * a2[1] - at1[1] * a2[0];
+        for (int c = 0; c < 3; c++) {
+            s11[c] = (double)A.r[c] + ddt * (double)A.v[c];
+            n1[c] = n0[c] + ddt * (nt[c] + ddt * ntt[c]);
+        }
     }
-    for (int iz = lo[2]; iz <= hi[2]; iz++)
-        for (int iy = lo[1]; iy <= hi[1]; iy++)
-            for (int ix = lo[0]; ix <= hi[0]; ix++) {
-                const int cid = ix + Lx * (iy + Ly * iz);
-                const int e = starts[cid + 1];
-                for (int i = starts[cid]; i < e; i++) {
-                    MPart p0;
-                    particle_path(parts, dt, i, &p0);
-                    float o[4];
-                    if (intersect(dt, A, B, C, p0, o) != BB_SUCCESS) continue;
-                    atomicMin(keys + i, make_key(o[0], gid));
+    const int ny = active ? hi[1] - lo[1] + 1 : 0, nrows = active ? ny * (hi[2] - lo[2] + 1) : 0;
+    int fill = 0;   // the same in every lane of the wave
+    auto flush = [&]() {
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(cand_n, (unsigned)fill);
+        base = __shfl(base, 0);
+        __builtin_amdgcn_wave_barrier();
+        for (int k = lane; k < fill; k += 64)
+            if (base + k < (unsigned)cap) cand[base + k] = wq[k];   // beyond: k_mbb_find_all redoes the search
+        __builtin_amdgcn_wave_barrier();
+        fill = 0;
+    };
+    // rows of cells: the particles of cells lo.x..hi.x are contiguous
+    for (int r = 0; wave_any(r < nrows); r++) {
+        int b = 0, len = 0;
+        if (r < nrows) {
+            const int iy = lo[1] + r % ny, iz = lo[2] + r / ny;
+            const int row = Lx * (iy + Ly * iz);
+            b = starts[row + lo[0]];
+            len = starts[row + hi[0] + 1] - b;
+        }
+        // four candidates per iteration: their loads in flight together
+        for (int c0 = 0; wave_any(c0 < len); c0 += 4) {
+            unsigned pass = 0;
+            MPart p[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = b + (c0 + u < len ? c0 + u : (len > 0 ? len - 1 : 0));
+                if (len > 0) {
+                    parts.load_r0(i, p[u].r);
+                    const float4 w = v0[i];
+                    p[u].v[0] = w.x;
+                    p[u].v[1] = w.y;
+                    p[u].v[2] = w.z;
                 }
             }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (c0 + u >= len) continue;
+                double dr0[3], dr1[3];
+                for (int c = 0; c < 3; c++) {
+                    dr0[c] = (double)(p[u].r[c] - A.r[c]);
+                    dr1[c] = ((double)p[u].r[c] + ddt * (double)p[u].v[c]) - s11[c];
+                }
+                if (!(ddot(dr0, n0) * ddot(dr1, n1) > 0)) pass |= 1u << u;   // not BB_NOCROSS
+            }
+            unsigned long long bu[4];
+            int tot = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                bu[u] = __ballot((pass >> u) & 1u);
+                tot += __popcll(bu[u]);
+            }
+            if (tot == 0) continue;
+            if (fill + tot > MBB_WQ) flush();
+            int off = fill;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if ((pass >> u) & 1u) wq[off + __popcll(bu[u] & below)] = make_uint2((unsigned)gid, (unsigned)(b + c0 + u));
+                off += __popcll(bu[u]);
+            }
+            fill += tot;
+        }
+    }
+    if (fill > 0) flush();
+}
+
+// the whole search with the full test in place, run only when the pair list
+// overflowed (the list pass keeps its results: atomicMin is idempotent)
+template <class P>
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_find_all(MeshView m, int max_gid, const int *n_inst, float dt,
+                                                              int Lx, int Ly, int Lz, const int *starts, P parts,
+                                                              const float4 *v0, unsigned long long *keys, int cap,
+                                                              const unsigned *cand_n) {
+    if (*cand_n <= (unsigned)cap) return;
+    const int ngid = *n_inst * m.nt < max_gid ? *n_inst * m.nt : max_gid;
+    for (int gid = blockIdx.x * MBB_THREADS + threadIdx.x; gid < ngid; gid += gridDim.x * MBB_THREADS) {
+        MPart A, B, C;
+        fetch_tri(m, dt, gid, &A, &B, &C);
+        revert(dt, &A);
+        revert(dt, &B);
+        revert(dt, &C);
+        const float tol = 0.1f;
+        const int L[3] = {Lx, Ly, Lz};
+        int lo[3], hi[3];
+        bool ok = true;
+        for (int c = 0; c < 3; c++) {
+            const float mn = fminf(fminf(A.r[c], B.r[c]), C.r[c]);
+            const float mx = fmaxf(fmaxf(A.r[c], B.r[c]), C.r[c]);
+            if (!(mx - mn <= MBB_MAX_SPAN)) ok = false;
+            lo[c] = cell_of(L[c], mn - tol);
+            hi[c] = cell_of(L[c], mx + tol);
+        }
+        if (!ok) continue;
+        for (int iz = lo[2]; iz <= hi[2]; iz++)
+            for (int iy = lo[1]; iy <= hi[1]; iy++) {
+                const int row = Lx * (iy + Ly * iz);
+                const int e = starts[row + hi[0] + 1];
+                for (int i = starts[row + lo[0]]; i < e; i++) {
+                    MPart p0;
+                    parts.load_r0(i, p0.r);
+                    const float4 w = v0[i];
+                    p0.v[0] = w.x;
+                    p0.v[1] = w.y;
+                    p0.v[2] = w.z;
+                    float o[4];
+                    if (intersect(dt, A, B, C, p0, o) == BB_SUCCESS) atomicMin(keys + i, make_key(o[0], gid));
+                }
+            }
+    }
+}
+
+// the full crossing test of the listed (triangle, particle) pairs
+template <class P>
+__global__ __launch_bounds__(MBB_THREADS) void k_mbb_cand(MeshView m, float dt, P parts, const float4 *v0, int cap,
+                                                          const unsigned *cand_n, const uint2 *cand,
+                                                          unsigned long long *keys) {
+    const unsigned n = *cand_n < (unsigned)cap ? *cand_n : (unsigned)cap;
+    for (unsigned k = blockIdx.x * MBB_THREADS + threadIdx.x; k < n; k += gridDim.x * MBB_THREADS) {
+        const uint2 e = cand[k];
+        MPart a, b, c, p0;
+        fetch_tri(m, dt, (int)e.x, &a, &b, &c);
+        revert(dt, &a);
+        revert(dt, &b);
+        revert(dt, &c);
+        parts.load_r0((int)e.y, p0.r);
+        const float4 w = v0[e.y];
+        p0.v[0] = w.x;
+        p0.v[1] = w.y;
+        p0.v[2] = w.z;
+        float o[4];
+        if (intersect(dt, a, b, c, p0, o) == BB_SUCCESS) atomicMin(keys + e.y, make_key(o[0], (int)e.x));
+    }
 }
 
 // ---- bounce ----------------------------------------------------------------
@@ -680,6 +875,10 @@ struct udx_mbb {
     unsigned long long *keys, *ncol;
     int *rec_tri, *tri_cnt, *tri_start, *tri_fill, *tri_ids, *tri_active, *scan_tmp;
     float *rec_m, *tri_dv;
+    float4 *v0;                    // the particles' straight-line velocities
+    unsigned *cand_n;              // (triangle instance, particle) pairs past the plane test
+    uint2 *cand;
+    int cand_cap;
     int *n_dev;                    // the particle count for the AoS entry point
 };
 
@@ -705,6 +904,9 @@ static void mbb_free(udx_mbb *b) {
     dev_free(b->rec_m);
     dev_free(b->tri_dv);
     dev_free(b->n_dev);
+    dev_free(b->v0);
+    dev_free(b->cand_n);
+    dev_free(b->cand);
 }
 
 udx_ctx *mbb_ctx(const udx_mbb *b) { return b->ctx; }
@@ -724,8 +926,17 @@ static int mbb_run(hipStream_t s, udx_mbb *b, float dt, float flu_mass, float rb
         hipLaunchKernelGGL(k_mbb_instances, dim3(1), dim3(1024), 0, s, nm, L[0], L[1], L[2], b->codes, b->inst_mesh,
                            b->inst_shift, b->n_inst);
         const int max_gid = MBB_MAX_IMAGES * ntri;
+        if (nmax > 0)
+            hipLaunchKernelGGL((k_mbb_paths<P>), dim3(ceil_div(nmax, MBB_THREADS)), dim3(MBB_THREADS), 0, s, nmax,
+                               n_dev, dt, parts, b->v0);
+        UDX_HIP(hipMemsetAsync(b->cand_n, 0, sizeof(unsigned), s));
         hipLaunchKernelGGL((k_mbb_find<P>), dim3(ceil_div(max_gid, MBB_THREADS)), dim3(MBB_THREADS), 0, s, m,
-                           max_gid, b->n_inst, dt, L[0], L[1], L[2], starts, parts, b->keys);
+                           max_gid, b->n_inst, dt, L[0], L[1], L[2], starts, parts, b->v0, b->keys, b->cand_cap,
+                           b->cand_n, b->cand);
+        hipLaunchKernelGGL((k_mbb_cand<P>), dim3(MBB_CAND_BLOCKS), dim3(MBB_THREADS), 0, s, m, dt, parts, b->v0,
+                           b->cand_cap, b->cand_n, b->cand, b->keys);
+        hipLaunchKernelGGL((k_mbb_find_all<P>), dim3(MBB_CAND_BLOCKS), dim3(MBB_THREADS), 0, s, m, max_gid,
+                           b->n_inst, dt, L[0], L[1], L[2], starts, parts, b->v0, b->keys, b->cand_cap, b->cand_n);
     }
     if (nmax > 0)
         hipLaunchKernelGGL((k_mbb_perform<P>), dim3(ceil_div(nmax, MBB_THREADS)), dim3(MBB_THREADS), 0, s, m, nmax,
@@ -801,6 +1012,12 @@ int udx_mbb_ini(udx_ctx *ctx, int nv, int nt, const int *tt_host, int max_meshes
     if (!st) st = dev_alloc(&b->tri_dv, 9 * (size_t)ntri);
     if (!st) st = dev_alloc(&b->scan_tmp, (size_t)scan_scratch_size(ntri));
     if (!st) st = dev_alloc(&b->n_dev, 1);
+    if (!st) st = dev_alloc(&b->v0, (size_t)np1);
+    // the pairs past the plane test: a few per crossing particle; beyond the
+    // list the search tests pairs in place
+    b->cand_cap = 2 * np1 > (1 << 20) ? 2 * np1 : (1 << 20);
+    if (!st) st = dev_alloc(&b->cand_n, 1);
+    if (!st) st = dev_alloc(&b->cand, (size_t)b->cand_cap);
     hipStream_t s = ctx->stream;
     if (!st && (hipMemcpyAsync(b->tt, tt_host, 3 * sizeof(int) * (size_t)nt, hipMemcpyHostToDevice, s) != hipSuccess ||
                 hipMemcpyAsync(b->vstart, cnt.data(), sizeof(int) * ((size_t)nv + 1), hipMemcpyHostToDevice, s) !=
@@ -855,5 +1072,15 @@ int udx_mbb_count(udx_mbb *b, unsigned long long *ncol) {
 }
 
 const int *udx_mbb_collisions(const udx_mbb *b) { return b ? b->rec_tri : nullptr; }
+
+int udx_mbb_set_pair_capacity(udx_mbb *b, int cap) {
+    UDX_CHECK(b && cap >= 1, UDX_EINVAL, "udx_mbb_set_pair_capacity: bad arguments");
+    UDX_HIP(hipStreamSynchronize(b->ctx->stream));
+    dev_free(b->cand);
+    b->cand_cap = 0;
+    UDX_TRY(dev_alloc(&b->cand, (size_t)cap));
+    b->cand_cap = cap;
+    return UDX_OK;
+}
 
 }  // extern "C"

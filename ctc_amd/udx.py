@@ -133,6 +133,7 @@ _SIGS = {
     "udx_mbb_bounce": (_I, [_VP, _F, _F, _F, _I, _I, _I, _I, _VP, _VP, _VP, _I, _VP, _VP, _VP]),
     "udx_mbb_count": (_I, [_VP, C.POINTER(C.c_ulonglong)]),
     "udx_mbb_collisions": (_VP, [_VP]),
+    "udx_mbb_set_pair_capacity": (_I, [_VP, _I]),
     "udx_flu_set_rbc_bounce": (_I, [_VP, _VP]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
     "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
@@ -614,6 +615,9 @@ class Mbb:
         v = C.c_ulonglong()
         _ok(lib().udx_mbb_count(self.h, C.byref(v)))
         return v.value
+
+    def set_pair_capacity(self, cap):
+        _ok(lib().udx_mbb_set_pair_capacity(self.h, cap))
 
     def collisions(self, n):
         import torch

@@ -324,6 +324,10 @@ int udx_mbb_count(udx_mbb *b, unsigned long long *ncol);
 /* per particle of the last call: the triangle (mesh * nt + t) it bounced
  * off, -1 for none (device memory, n entries) */
 const int *udx_mbb_collisions(const udx_mbb *b);
+/* the list of (triangle, particle) pairs that pass the plane test and get
+ * the full crossing test (default max(2 maxp, 2^20)); when a call has more,
+ * the search runs again with the full test in place (same results, slower) */
+int udx_mbb_set_pair_capacity(udx_mbb *b, int cap);
 /* bounce-back inside udx_flu_run after the membrane advance
  * (objects_bounce, src/sim/objects/imp/bounce.h:236-248); NULL switches it
  * off.  Needs the membranes of udx_flu_set_rbc; the same template. */

@@ -80,6 +80,11 @@ def test_bounce_matches_oracle(ctx, seed):
     rc2, pp2 = dev(c["rc"]), dev(c["pp"])
     b.bounce(c["dt"], 1.0, 0.5, c["L"], nm, dev(rp), rc2, dev(c["starts"], torch.int32), dev(c["pp0"]), pp2)
     assert torch.equal(pp2, pp) and torch.equal(rc2, rc)
+    # a pair list too short for the call: the in-place search gives the same
+    b.set_pair_capacity(8)
+    rc3, pp3 = dev(c["rc"]), dev(c["pp"])
+    b.bounce(c["dt"], 1.0, 0.5, c["L"], nm, dev(rp), rc3, dev(c["starts"], torch.int32), dev(c["pp0"]), pp3)
+    assert torch.equal(pp3, pp) and torch.equal(rc3, rc) and b.count() == want_n
     b.close()
 
 
