@@ -42,6 +42,9 @@ def _lib():
             "orc_fsi_seeds": (None, [C.c_int, F32P]),
             "orc_rbc_adjoint": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, I32P, F64P, F32P, F32P,
                                        C.c_float, C.c_float, F64P, F64P, F64P, F64P]),
+            "orc_jul_edges": (C.c_int, [C.c_int, C.c_int, I32P, I32P]),
+            "orc_jul_theta": (C.c_double, [F64P, F64P, F64P, F64P]),
+            "orc_jul_forces": (None, [C.c_int, C.c_int, C.c_int, I32P, C.c_int, I32P, F32P, F32P, F32P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -261,3 +264,28 @@ def suspension_step(L, spp, opp, mesh, nc, dpd, seeds28, fsi_par, fsi_seed2, p, 
     from . import mbb as B
     sout, opp, _, _, ncol = B.bounce(dt, mass, rbc_mass, L, nc, mesh.tt, start[:, :3], opp, starts, srt, sout)
     return O.wrap(L, sout), opp, npair // 2, nfsi, ncol
+
+
+def jul_edges(mesh):
+    """(ne, 4) edges (i, j, k, l) in the oracle's numbering"""
+    tt = _i32(mesh.tt)
+    dih = np.zeros((3 * len(tt), 4), np.int32)
+    ne = _lib().orc_jul_edges(mesh.nv, len(tt), _ptr(tt, I32P), _ptr(dih, I32P))
+    return dih[:ne].copy()
+
+
+def jul_theta(a, b, c, d):
+    return _lib().orc_jul_theta(*[_ptr(_f64(x), F64P) for x in (a, b, c, d)])
+
+
+def jul_forces(mesh, pp, nc):
+    """Juelicher bending (f) and area-difference (fad) forces, 3 floats per
+    vertex each (src/rbc/force/bending/juelicher)"""
+    dih = jul_edges(mesh)
+    pp = _f32(pp).reshape(-1, 6)
+    f = np.zeros((len(pp), 3), np.float32)
+    fad = np.zeros((len(pp), 3), np.float32)
+    tt = _i32(mesh.tt)
+    _lib().orc_jul_forces(nc, mesh.nv, len(tt), _ptr(tt, I32P), len(dih), _ptr(dih, I32P), _ptr(pp, F32P),
+                          _ptr(f, F32P), _ptr(fad, F32P))
+    return f, fad

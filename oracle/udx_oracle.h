@@ -223,6 +223,13 @@ long  orc_mbb_bounce(float dt, float flu_mass, float rbc_mass, const int L[3], i
                      const float *rp, float *rc, const int *starts, int n, const float *pp0, float *pp, float *mm,
                      int *col_tri);
 
+/* ---- Juelicher bending: src/rbc/force/bending/juelicher (udx_oracle_jul.c) */
+/* edges as (i, j, k, l): 4 ints per edge; returns the edge count */
+int    orc_jul_edges(int nv, int nt, const int *tt, int *dih);
+double orc_jul_theta(const double a[3], const double b[3], const double c[3], const double d[3]);
+void   orc_jul_forces(int nc, int nv, int nt, const int *tt, int ne, const int *dih, const float *pp, float *f,
+                      float *fad);
+
 #ifdef __cplusplus
 }
 #endif
