@@ -586,6 +586,8 @@ struct udx_rbc {
     float *ll, *aa;              // device, stress free (or null)
     float l0, a0;
     int stress_free, stressful_set, bending;
+    int juelicher;               // Juelicher bending instead of Kantor's
+    JulTemplate *jt;             // its edges and scratch, built on first use
     double *av;                  // device, 2 * max_cells
     double *avd, *gpb, *gp;      // adjoint scratch: 4 * max_cells, 5 per vertex block, 5
     float *rd_tmp;               // redistribution scratch: 6 floats per vertex, 2 ints per cell
@@ -614,6 +616,7 @@ int udx_rbc_fin(udx_rbc *r) {
     dev_free(r->rd_tmp);
     dev_free(r->rd_fid);
     dev_free(r->rd_new);
+    jul_free(r->jt);
     delete r;
     return UDX_OK;
 }
@@ -719,6 +722,21 @@ int udx_rbc_set_stressfree(udx_rbc *r, const float *rr) {
 int udx_rbc_set_bending(udx_rbc *r, int kantor) {
     UDX_CHECK(r, UDX_EINVAL, "udx_rbc_set_bending: null handle");
     r->bending = kantor ? 1 : 0;
+    r->juelicher = 0;
+    return UDX_OK;
+}
+
+int udx_rbc_set_bending_type(udx_rbc *r, int type) {
+    UDX_CHECK(r, UDX_EINVAL, "udx_rbc_set_bending_type: null handle");
+    UDX_CHECK(type == UDX_BENDING_NONE || type == UDX_BENDING_KANTOR || type == UDX_BENDING_JUELICHER, UDX_EINVAL,
+              "udx_rbc_set_bending_type: unknown bending type %d", type);
+    if (type == UDX_BENDING_JUELICHER && !r->jt) {
+        std::vector<int> tt_h(3 * (size_t)r->nt);
+        UDX_HIP(hipMemcpy(tt_h.data(), r->tt, sizeof(int) * tt_h.size(), hipMemcpyDeviceToHost));
+        UDX_TRY(jul_build(r->ctx->stream, r->nv, r->nt, tt_h.data(), r->max_cells, &r->jt));
+    }
+    r->bending = type == UDX_BENDING_KANTOR ? 1 : 0;
+    r->juelicher = type == UDX_BENDING_JUELICHER ? 1 : 0;
     return UDX_OK;
 }
 
@@ -771,6 +789,8 @@ int udx_rbc_force_apply(udx_rbc *r, const udx_rbc_params *p, float dt, int nc, c
     hipLaunchKernelGGL(k_rbc_force, dim3(ceil_div(n, RBC_THREADS)), dim3(RBC_THREADS), 0, r->ctx->stream, K, pp,
                        r->av, ff);
     UDX_LAUNCHED();
+    // bending_apply (src/rbc/force/imp/forces.h:54) for the Juelicher model
+    if (r->juelicher) UDX_TRY(jul_apply(r->ctx->stream, r->jt, nc, pp, ff));
     return UDX_OK;
 }
 
@@ -781,6 +801,7 @@ int udx_rbc_force_adjoint(udx_rbc *r, const udx_rbc_params *p, int nc, const flo
               "udx_rbc_force_adjoint: call udx_rbc_set_stressful or udx_rbc_set_stressfree first");
     UDX_CHECK(nc >= 0 && nc <= r->max_cells, UDX_ECAP, "udx_rbc_force_adjoint: nc = %d above max_cells = %d", nc,
               r->max_cells);
+    UDX_CHECK(!r->juelicher, UDX_EINVAL, "udx_rbc_force_adjoint: the adjoint covers Kantor bending only");
     for (int k = 0; k < 5; k++) gp[k] = 0;
     if (nc == 0) return UDX_OK;
     hipStream_t s = r->ctx->stream;

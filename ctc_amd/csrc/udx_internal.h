@@ -188,6 +188,12 @@ int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float 
 // the start of the step and after the membrane advance (velocities updated),
 // pos0 the solvent positions at the start of the step; applies the periodic
 // wrap to every particle
+// Juelicher bending of one template (juelicher.hip): build from the
+// triangles, add the forces of nc cells to ff (3 floats per vertex)
+struct JulTemplate;
+int jul_build(hipStream_t s, int nv, int nt, const int *tt_host, int max_cells, JulTemplate **out);
+int jul_apply(hipStream_t s, const JulTemplate *j, int nc, const float *pp, float *ff);
+void jul_free(JulTemplate *j);
 udx_ctx *mbb_ctx(const udx_mbb *b);
 int mbb_nv(const udx_mbb *b);
 int mbb_step(hipStream_t s, udx_mbb *b, float dt, float flu_mass, float rbc_mass, const Box &box, int nm,

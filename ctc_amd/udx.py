@@ -143,6 +143,7 @@ _SIGS = {
     "udx_rbc_set_stressful": (_I, [_VP, _F]),
     "udx_rbc_set_stressfree": (_I, [_VP, C.POINTER(_F)]),
     "udx_rbc_set_bending": (_I, [_VP, _I]),
+    "udx_rbc_set_bending_type": (_I, [_VP, _I]),
     "udx_rbc_area_volume": (_I, [_VP, _I, _VP, _VP]),
     "udx_rbc_av": (_VP, [_VP]),
     "udx_rbc_force_apply": (_I, [_VP, C.POINTER(RbcParams), _F, _I, _VP, _VP]),
@@ -724,6 +725,10 @@ class Rbc:
 
     def set_bending(self, kantor=True):
         _ok(lib().udx_rbc_set_bending(self.h, 1 if kantor else 0))
+
+    def set_bending_type(self, kind):
+        """rbc_force_set_bending: one of none, kantor, juelicher"""
+        _ok(lib().udx_rbc_set_bending_type(self.h, {"none": 0, "kantor": 1, "juelicher": 2}[kind]))
 
     def area_volume(self, nc, pp):
         import torch

@@ -229,6 +229,12 @@ int udx_rbc_set_stressful(udx_rbc *r, float totArea);
 /* stress-free references from a shape (host, 3 * nv floats; src/rbc/force/imp/main.h:61-92) */
 int udx_rbc_set_stressfree(udx_rbc *r, const float *rr);
 int udx_rbc_set_bending(udx_rbc *r, int kantor);           /* 1: "kantor", 0: "none" */
+/* rbc_force_set_bending (src/rbc/force/imp/main.h:120-130): "none",
+ * "kantor" or "juelicher" (src/rbc/force/bending/juelicher: kb = 1 and
+ * H0 = -1/2 fixed as there, the params' kb unused; area-difference term
+ * included).  The adjoint covers Kantor bending only. */
+enum { UDX_BENDING_NONE = 0, UDX_BENDING_KANTOR = 1, UDX_BENDING_JUELICHER = 2 };
+int udx_rbc_set_bending_type(udx_rbc *r, int type);
 /* per cell [area, volume] in FP64 into av (device, 2 * nc; NULL: the
  * handle's own buffer, see udx_rbc_av) */
 int udx_rbc_area_volume(udx_rbc *r, int nc, const float *pp, double *av);

@@ -233,3 +233,41 @@ def test_fsi_trunk(ctx):
     f.apply(U.PairDpd(*par), opp_d, fo_d)
     assert np.array_equal(host(fo_d), fo_ref) and np.array_equal(host(fs_d), fs_ref)
     f.close()
+
+
+@pytest.mark.parametrize("nv,nc", [(498, 3), (642, 2)])
+def test_juelicher_bending(ctx, nv, nc):
+    """bending "juelicher": the membrane force without bending plus the
+    oracle's Juelicher terms (f + fad), accumulated as the kernel does.  The
+    device's double atan2 may differ from the host's by an ulp before the
+    float rounding; the bound is 2 float ulps of the largest term."""
+    say("juelicher", nv, nc)
+    m = mesh(nv)
+    rng = np.random.default_rng(nv + 7)
+    pp = R.place(m.rr, rotations(nc, nv + 1), (32, 32, 32))
+    pp[:, :3] += rng.normal(0, 0.02, (len(pp), 3)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, 0.5, (len(pp), 3)).astype(np.float32)
+    tmpl = np.zeros((m.nv, 6), np.float32)
+    tmpl[:, :3] = m.rr
+    A0, V0 = R.area_volume(m, tmpl, 1)
+    kw = dict(R.RBC_TEST, totArea=np.float32(A0 * 1.02), totVolume=np.float32(V0 * 0.97))
+    g = U.Rbc(ctx, m.tt, m.nv, nc)
+    g.set_stressful(kw["totArea"])
+    g.set_bending_type("none")
+    base = torch.zeros((nc * m.nv, 3), dtype=torch.float32, device="cuda")
+    g.force(U.rbc_params(**kw), 5e-4, nc, dev(pp), base)
+    g.set_bending_type("juelicher")
+    got = torch.zeros_like(base)
+    g.force(U.rbc_params(**kw), 5e-4, nc, dev(pp), got)
+    f, fad = R.jul_forces(m, pp, nc)
+    want = host(base) + (f + fad)
+    got = host(got)
+    scale = np.abs(f + fad).max()
+    exact = np.mean(got == want)
+    say("juelicher: %.4f of the components bit-identical" % exact)
+    assert np.abs(got - want).max() <= 2 * 2 ** -23 * (scale + np.abs(want).max())
+    # the adjoint refuses the Juelicher model
+    with pytest.raises(U.UdxError, match="Kantor"):
+        g.adjoint(U.rbc_params(**kw), nc, dev(pp), dev(np.zeros((nc * m.nv, 3))), torch.zeros_like(base))
+    g.set_bending_type("kantor")
+    g.close()
