@@ -13,6 +13,11 @@ class RbcPar(C.Structure):
                                           "kv", "totArea", "totVolume")]
 
 
+class DpdLj(C.Structure):
+    """DPD + Lennard-Jones pair (src/pair/type.h:22-25)"""
+    _fields_ = [(k, C.c_float) for k in ("a", "g", "s", "spow", "ljs", "lje")]
+
+
 # conf/rbc.test.cfg
 RBC_TEST = dict(gammaC=30.0, gammaT=90.0, kBT=0.0444, kb=17.78, phi=6.97, ks=14.8, x0=0.5, mpow=2.0, ka=2450.0,
                 kd=100.0, kv=3750.0)
@@ -43,6 +48,9 @@ def _lib():
             "orc_rbc_adjoint": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int, I32P, I32P, I32P, F64P, F32P, F32P,
                                        C.c_float, C.c_float, F64P, F64P, F64P, F64P]),
             "orc_jul_edges": (C.c_int, [C.c_int, C.c_int, I32P, I32P]),
+            "orc_cnt_pair": (C.c_int, [C.POINTER(DpdLj), F32P, F32P, C.c_float, F32P]),
+            "orc_cnt_images": (C.c_int, [I32P, C.c_int, F32P, F32P, I32P]),
+            "orc_cnt_forces": (C.c_long, [I32P, C.c_int, F32P, DpdLj, C.c_float, C.c_float, F32P]),
             "orc_jul_theta": (C.c_double, [F64P, F64P, F64P, F64P]),
             "orc_jul_forces": (None, [C.c_int, C.c_int, C.c_int, I32P, C.c_int, I32P, F32P, F32P, F32P]),
         }
@@ -289,3 +297,27 @@ def jul_forces(mesh, pp, nc):
     _lib().orc_jul_forces(nc, mesh.nv, len(tt), _ptr(tt, I32P), len(dih), _ptr(dih, I32P), _ptr(pp, F32P),
                           _ptr(f, F32P), _ptr(fad, F32P))
     return f, fad
+
+
+def cnt_pair(params, a, b, rnd):
+    f = np.zeros(3, np.float32)
+    p = DpdLj(*params)
+    ok = _lib().orc_cnt_pair(C.byref(p), _ptr(_f32(a), F32P), _ptr(_f32(b), F32P), np.float32(rnd), _ptr(f, F32P))
+    return f, bool(ok)
+
+
+def cnt_images(L, pp):
+    pp = _f32(pp).reshape(-1, 6)
+    img = np.zeros((26 * max(len(pp), 1), 6), np.float32)
+    iv = np.zeros(26 * max(len(pp), 1), np.int32)
+    n = _lib().orc_cnt_images(_ptr(_i32(L), I32P), len(pp), _ptr(pp, F32P), _ptr(img, F32P), _ptr(iv, I32P))
+    return img[:n].copy(), iv[:n].copy()
+
+
+def cnt_forces(L, pp, params, seed_bulk, seed_halo, ff=None):
+    """membrane contact forces (src/cnt) added to ff; returns (ff, pairs)"""
+    pp = _f32(pp).reshape(-1, 6)
+    ff = np.zeros((len(pp), 3), np.float32) if ff is None else _f32(ff).copy()
+    n = _lib().orc_cnt_forces(_ptr(_i32(L), I32P), len(pp), _ptr(pp, F32P), DpdLj(*params), np.float32(seed_bulk),
+                              np.float32(seed_halo), _ptr(ff, F32P))
+    return ff, int(n)

@@ -230,6 +230,13 @@ double orc_jul_theta(const double a[3], const double b[3], const double c[3], co
 void   orc_jul_forces(int nc, int nv, int nt, const int *tt, int ne, const int *dih, const float *pp, float *f,
                       float *fad);
 
+/* ---- membrane contact: src/cnt (udx_oracle_cnt.c) ------------------------ */
+typedef struct { float a, g, s, spow, ljs, lje; } orc_dpd_lj;
+int   orc_cnt_pair(const orc_dpd_lj *p, const float *a, const float *b, float rnd, float *f);
+int   orc_cnt_images(const int L[3], int n, const float *pp, float *img, int *img_v);
+long  orc_cnt_forces(const int L[3], int n, const float *pp, orc_dpd_lj p, float seed_bulk, float seed_halo,
+                     float *ff);
+
 #ifdef __cplusplus
 }
 #endif
