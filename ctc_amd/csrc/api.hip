@@ -147,6 +147,9 @@ struct udx_flu {
         udx_mbb *bb = nullptr;
         float *start_pp = nullptr;
         float4 *pos0 = nullptr;
+        // contact between membranes (udx_flu_set_rbc_contact)
+        udx_cnt *cnt = nullptr;
+        udx_pair_dpd_lj cnt_par{};
     } mb;
 };
 
@@ -163,10 +166,14 @@ static void flu_free_membranes(udx_flu *f, bool keep_bounce = false) {
     dev_free(f->mb.start_pp);
     udx_mbb *bb = f->mb.bb;
     float4 *pos0 = f->mb.pos0;
+    udx_cnt *cnt = f->mb.cnt;
+    const udx_pair_dpd_lj cnt_par = f->mb.cnt_par;
     f->mb = udx_flu::Membranes{};
     if (keep_bounce) {
         f->mb.bb = bb;
         f->mb.pos0 = pos0;
+        f->mb.cnt = cnt;
+        f->mb.cnt_par = cnt_par;
     } else {
         dev_free(pos0);
     }
@@ -545,6 +552,11 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
         UDX_HIP(hipMemsetAsync(m.fo, 0, 3 * sizeof(float) * (size_t)nvtx, s));
         UDX_TRY(fsi_forces(s, fsi_bufs(m.fsi), f->n, f->pos[b], f->vel[b], f->cl.starts, nvtx, m.pp, m.fp, fsi_seeds,
                            m.fo, (float *)f->ff, 4, fsi_pairs_dev(m.fsi), f->conf.mode == UDX_MODE_EXACT));
+        // contact (forces_cnt, cnt_halo: src/sim/objinter/imp/main.h:22-26,79)
+        if (m.cnt) {
+            const float sb = cnt_draw(m.cnt), sh = cnt_draw(m.cnt);
+            UDX_TRY(cnt_forces(s, m.cnt, m.cnt_par, nvtx, m.pp, sb, sh, m.fo));
+        }
     }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     // store_solvent (src/sim/imp/update.h:11-16): positions before the move
@@ -953,6 +965,21 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
         m.bb = nullptr;
         UDX_CHECK(false, UDX_EINVAL, "udx_flu_set_rbc: the bounce-back template differs; bounce-back switched off");
     }
+    return UDX_OK;
+}
+
+int udx_flu_set_rbc_contact(udx_flu *f, udx_cnt *c, const udx_pair_dpd_lj *p) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_rbc_contact: null handle");
+    udx_flu::Membranes &m = f->mb;
+    if (!c) {
+        m.cnt = nullptr;
+        return UDX_OK;
+    }
+    UDX_CHECK(p, UDX_EINVAL, "udx_flu_set_rbc_contact: null parameters");
+    UDX_CHECK(m.r, UDX_EINVAL, "udx_flu_set_rbc_contact: no membranes (udx_flu_set_rbc first)");
+    UDX_CHECK(cnt_ctx(c) == f->ctx, UDX_EINVAL, "udx_flu_set_rbc_contact: contact and solvent contexts differ");
+    m.cnt = c;
+    m.cnt_par = *p;
     return UDX_OK;
 }
 

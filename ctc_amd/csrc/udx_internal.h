@@ -194,6 +194,11 @@ struct JulTemplate;
 int jul_build(hipStream_t s, int nv, int nt, const int *tt_host, int max_cells, JulTemplate **out);
 int jul_apply(hipStream_t s, const JulTemplate *j, int nc, const float *pp, float *ff);
 void jul_free(JulTemplate *j);
+// membrane contact forces (cnt.hip) and the handle's seed draws
+udx_ctx *cnt_ctx(const udx_cnt *c);
+int cnt_forces(hipStream_t s, udx_cnt *c, const udx_pair_dpd_lj &pr, int n, const float *pp, float seed_bulk,
+               float seed_halo, float *ff);
+float cnt_draw(udx_cnt *c);
 udx_ctx *mbb_ctx(const udx_mbb *b);
 int mbb_nv(const udx_mbb *b);
 int mbb_step(hipStream_t s, udx_mbb *b, float dt, float flu_mass, float rbc_mass, const Box &box, int nm,

@@ -25,6 +25,11 @@ class PairDpd(C.Structure):
     _fields_ = [("a", C.c_float), ("g", C.c_float), ("s", C.c_float), ("spow", C.c_float)]
 
 
+class PairDpdLj(C.Structure):
+    """udx_pair_dpd_lj (PairDPDLJ, src/pair/type.h:22-25)"""
+    _fields_ = [(k, C.c_float) for k in ("a", "g", "s", "spow", "ljs", "lje")]
+
+
 class FluConf(C.Structure):
     _fields_ = [("L", C.c_int * 3), ("maxp", C.c_int), ("dpd", PairDpd), ("dt", C.c_float),
                 ("mass", C.c_float), ("accel", C.c_float * 3), ("mode", C.c_int)]
@@ -135,6 +140,12 @@ _SIGS = {
     "udx_mbb_collisions": (_VP, [_VP]),
     "udx_mbb_set_pair_capacity": (_I, [_VP, _I]),
     "udx_flu_set_rbc_bounce": (_I, [_VP, _VP]),
+    "udx_cnt_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
+    "udx_cnt_fin": (_I, [_VP]),
+    "udx_cnt_apply": (_I, [_VP, C.POINTER(PairDpdLj), _I, _VP, _VP]),
+    "udx_cnt_apply_seed": (_I, [_VP, C.POINTER(PairDpdLj), _I, _VP, _F, _F, _VP]),
+    "udx_cnt_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
+    "udx_flu_set_rbc_contact": (_I, [_VP, _VP, C.POINTER(PairDpdLj)]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
     "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
     "udx_rbc_fin": (_I, [_VP]),
@@ -436,6 +447,12 @@ class Flu:
         """the step clock (time of the next step), advanced by dt per step"""
         _ok(lib().udx_flu_set_time(self.h, float(t)))
 
+    def set_rbc_contact(self, cnt, params=None):
+        """membrane contact in the step (udx_flu_set_rbc_contact); None: off"""
+        self._cnt = cnt
+        _ok(lib().udx_flu_set_rbc_contact(self.h, cnt.h if cnt is not None else None,
+                                          C.byref(params) if params is not None else None))
+
     def set_rbc_bounce(self, mbb):
         """membrane bounce-back in the step (udx_flu_set_rbc_bounce); None: off"""
         self._mbb = mbb
@@ -588,6 +605,35 @@ class Wall:
     def close(self):
         if self.h:
             lib().udx_wall_fin(self.h)
+            self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- membrane contact
+class Cnt:
+    """cnt_* (src/cnt/imp.h): contact between membrane vertices"""
+
+    def __init__(self, ctx, L, max_n):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        _ok(lib().udx_cnt_ini(ctx.h, L[0], L[1], L[2], max_n, C.byref(self.h)))
+
+    def apply(self, params, pp, ff, seeds=None):
+        """pp: (n, 6) cuda tensor; ff: (n, 3) cuda tensor, added to; seeds:
+        (bulk, halo) or None to draw them from the handle's trunk"""
+        if seeds is None:
+            _ok(lib().udx_cnt_apply(self.h, C.byref(params), pp.shape[0], _ptr(pp), _ptr(ff)))
+        else:
+            _ok(lib().udx_cnt_apply_seed(self.h, C.byref(params), pp.shape[0], _ptr(pp), float(seeds[0]),
+                                         float(seeds[1]), _ptr(ff)))
+
+    def pairs(self, reset=False):
+        v = C.c_ulonglong()
+        _ok(lib().udx_cnt_pairs(self.h, C.byref(v), 1 if reset else 0))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().udx_cnt_fin(self.h)
             self.h = C.c_void_p()
 
 

@@ -37,6 +37,8 @@ float udx_rnd_get(udx_rnd *r);
 /* ---- DPD pair parameters: replaces pair_set_dpd / pair_compute_dpd_sigma /
  *      pair_get_view_dpd (src/pair/imp.h:11-35, grey single-colour view) ---- */
 typedef struct udx_pair_dpd { float a, g, s, spow; } udx_pair_dpd;
+/* DPD + Lennard-Jones (PairDPDLJ, src/pair/type.h:22-25): the membrane contact pair */
+typedef struct udx_pair_dpd_lj { float a, g, s, spow, ljs, lje; } udx_pair_dpd_lj;
 /* s = sqrt(2 kBT g / dt) (src/pair/imp/main.h:54-62) */
 void udx_pair_compute_dpd_sigma(float kBT, float dt, udx_pair_dpd *p);
 
@@ -300,6 +302,29 @@ int udx_fsi_pairs(udx_fsi *f, unsigned long long *npairs, int reset);
  * reference's step.  r = NULL detaches the membranes.  Not combined with walls. */
 int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *rp, const udx_pair_dpd *fp,
                     float mass, int substeps, int nc, float *pp);
+
+/* ---- membrane contact: replaces cnt_ini / cnt_fin / cnt_build_cells /
+ *      cnt_bulk / cnt_halo (src/cnt/imp.h:5-13) for one object type on one
+ *      periodic domain ----
+ * DPD plus the capped repulsive Lennard-Jones term between membrane vertices
+ * (pairs a > b whose cells, clamped into the domain, differ by at most one
+ * per axis; noise mean0var1ii(seed_bulk, a, b)) and between the periodic
+ * images of the vertices outside the domain and the vertices (noise
+ * mean0var1ii(seed_halo, image, b), the image list ordered by receive
+ * fragment, vertices ascending).  pp: n vertices (6 floats, device), ff: 3
+ * floats per vertex, added to.  udx_cnt_apply draws (seed_bulk, seed_halo)
+ * from the handle's trunk (7119, 187, 18278, 15674).  Each vertex sums its
+ * terms in a fixed order: no dependence on scheduling. */
+typedef struct udx_cnt udx_cnt;
+int udx_cnt_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int max_n, udx_cnt **c);
+int udx_cnt_fin(udx_cnt *c);
+int udx_cnt_apply(udx_cnt *c, const udx_pair_dpd_lj *p, int n, const float *pp, float *ff);
+int udx_cnt_apply_seed(udx_cnt *c, const udx_pair_dpd_lj *p, int n, const float *pp, float seed_bulk,
+                       float seed_halo, float *ff);
+int udx_cnt_pairs(udx_cnt *c, unsigned long long *npairs, int reset);
+/* contact between the membranes of udx_flu_set_rbc inside udx_flu_run (with
+ * the FSI force, src/sim/objinter/imp/main.h:42-80); NULL switches it off */
+int udx_flu_set_rbc_contact(udx_flu *f, udx_cnt *c, const udx_pair_dpd_lj *p);
 
 /* ---- membrane bounce-back: replaces mesh_bounce_ini / _fin /
  *      _find_collisions / _select_collisions / _bounce /

@@ -245,7 +245,8 @@ def redistribute(L, opp, nc, nv):
 
 
 def suspension_step(L, spp, opp, mesh, nc, dpd, seeds28, fsi_par, fsi_seed2, p, dt, mass=1.0, rbc_mass=0.5,
-                    substeps=1, stress_free=None, l0=0.0, a0=0.0, nthreads=1, bounce=False):
+                    substeps=1, stress_free=None, l0=0.0, a0=0.0, nthreads=1, bounce=False, cnt_par=None,
+                    cnt_seeds=None):
     """One step of solvent + membranes on one periodic domain in the order of
     src/sim/imp/run.h:8-68: solvent cell list, membrane redistribution,
     solvent DPD, FSI (+f vertices, -f solvent), solvent kick-drift + wrap,
@@ -261,6 +262,8 @@ def suspension_step(L, spp, opp, mesh, nc, dpd, seeds28, fsi_par, fsi_seed2, p, 
     start = opp.copy()
     ff, npair = O.dpd_forces(L, srt, starts, counts, dpd, seeds28[:27], nthreads=nthreads)
     fo, ff, nfsi = fsi_forces(L, srt, starts, opp, fsi_par, fsi_seed2, fs=ff)
+    if cnt_par is not None:   # contact with the FSI force (src/sim/objinter/imp/main.h:42-80)
+        fo, _ = cnt_forces(L, opp, cnt_par, cnt_seeds[0], cnt_seeds[1], ff=fo)
     sout = O.integrate(L, srt, ff, dt, mass, wrap=not bounce)
     sub_dt = np.float32(np.float32(dt) / np.float32(substeps))
     for _ in range(substeps):

@@ -65,10 +65,14 @@ int orc_cnt_pair(const orc_dpd_lj *p, const float *a, const float *b, float rnd,
     return 1;
 }
 
+/* floor(x) + L/2 clamped into the domain (vertices) or into [-2, L + 1]
+ * (image centres: a block that far out reaches no cell of the domain) */
 static int ccell(float x, int L, int clampit) {
-    int c = L / 2 + (int)floorf(x);
-    if (clampit) c = c < 0 ? 0 : (c > L - 1 ? L - 1 : c);
-    return c;
+    const float f = floorf(x);
+    const int lo = clampit ? 0 : -2, hi = clampit ? L - 1 : L + 1;
+    if (!(f >= (float)(lo - L / 2))) return lo;
+    if (!(f <= (float)(hi - L / 2))) return hi;
+    return L / 2 + (int)f;
 }
 
 /* counting sort of points by clamped cell, ascending index within a cell */

@@ -224,3 +224,72 @@ def test_no_solvent_crosses_membranes(ctx, bounce):
         assert bounced > 30 and changed == 0
     else:
         assert changed > 30
+
+
+def test_contact_matches_oracle(ctx):
+    """udx_cnt_apply_seed against the contact oracle on cells that touch and
+    straddle the periodic faces: forces added to a nonzero array bit for bit,
+    the same pair count"""
+    from test_oracle_cnt import PAR, contact_case
+    L = (16, 16, 16)
+    pp = contact_case(2, L)
+    rng = np.random.default_rng(3)
+    ff0 = rng.normal(0, 1, (len(pp), 3)).astype(np.float32)
+    sb, sh = np.float32(0.123), np.float32(0.456)
+    want, npair = R.cnt_forces(L, pp, PAR, sb, sh, ff=ff0)
+    c = U.Cnt(ctx, L, len(pp))
+    ffd = dev(ff0)
+    c.apply(U.PairDpdLj(*PAR), dev(pp), ffd, seeds=(sb, sh))
+    assert c.pairs() == npair > 1000
+    assert np.array_equal(bits(host(ffd)), bits(want))
+    # a second call (counters reset by the first) gives the same forces
+    ffd2 = dev(ff0)
+    c.apply(U.PairDpdLj(*PAR), dev(pp), ffd2, seeds=(sb, sh))
+    assert torch.equal(ffd2, ffd)
+    c.close()
+
+
+def test_suspension_step_with_contact(ctx):
+    """one exact-mode suspension step with membrane contact: vertices within
+    1e-6 of the oracle (membrane force ulps), and the contact moves them by
+    more than that"""
+    from test_oracle_cnt import PAR
+    L, kBT, dt = (16, 16, 16), 0.0444, 5e-3
+    m, opp, kw = suspension_setup(L, (2, 2, 2), 7)
+    nc = len(opp) // m.nv
+    # two cells pressed together so that contact pairs exist
+    opp[:m.nv, 0] += np.float32(2.2)
+    rng = np.random.default_rng(11)
+    n = 10 * L[0] * L[1] * L[2]
+    spp = np.zeros((n, 6), np.float32)
+    spp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    spp[:, 3:] = rng.normal(0, np.sqrt(kBT), (n, 3)).astype(np.float32)
+    dpd = (2.6666667, 8.0, O.dpd_sigma(kBT, 8.0, dt), 0.25)
+    seeds = O.seeds_table(1)
+    l0, a0 = m.stressful(kw["totArea"])
+    say("suspension step with contact: %d solvent, %d cells" % (n, nc))
+    flu = U.Flu(ctx, L, n, U.PairDpd(*dpd), dt, 1.0, exact=True)
+    flu.set_pp(dev(spp))
+    r = U.Rbc(ctx, m.tt, m.nv, nc)
+    r.set_stressful(kw["totArea"])
+    r.set_bending(True)
+    fsi = U.Fsi(ctx, L, nc * m.nv)
+    cnt = U.Cnt(ctx, L, nc * m.nv)
+    vert = dev(opp)
+    flu.set_rbc(r, fsi, U.rbc_params(**kw), U.PairDpd(*dpd), 0.5, 1, nc, vert)
+    flu.set_rbc_contact(cnt, U.PairDpdLj(*PAR))
+    flu.run(1, seeds)
+    gv, gpairs = host(vert), cnt.pairs()
+    flu.set_rbc(None)
+    flu.close()
+    r.close()
+    cs = O.kiss_fill(7119, 187, 18278, 15674, 2)
+    args = (L, spp, opp, m, nc, dpd, seeds[0], dpd, R.fsi_seeds(1)[0], R.par(**kw), dt)
+    _, rv, _, _ = R.suspension_step(*args, l0=l0, a0=a0, nthreads=16, cnt_par=PAR, cnt_seeds=cs)
+    _, rv0, _, _ = R.suspension_step(*args, l0=l0, a0=a0, nthreads=16)
+    say("contact pairs %d; vertex shift by contact %.3g" % (gpairs, np.abs(rv[:, :3] - rv0[:, :3]).max()))
+    assert gpairs > 0
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-6 * L[0]
+    assert np.abs(gv[:, 3:] - rv[:, 3:]).max() <= 1e-5 * max(np.abs(rv[:, 3:]).max(), 1.0)
+    assert np.abs(rv[:, 3:] - rv0[:, 3:]).max() > 1e-3
+    cnt.close()
