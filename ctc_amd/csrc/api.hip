@@ -111,6 +111,7 @@ struct udx_wall {
 
 static const SdfField *wall_sdf(const udx_wall *w) { return &w->sdf->f; }
 static const WallSet *wall_set(const udx_wall *w) { return &w->w; }
+static unsigned long long *wall_pairs_dev(udx_wall *w) { return w->pairs_dev; }
 
 struct udx_flu {
     udx_ctx *ctx;
@@ -150,11 +151,18 @@ struct udx_flu {
         // contact between membranes (udx_flu_set_rbc_contact)
         udx_cnt *cnt = nullptr;
         udx_pair_dpd_lj cnt_par{};
+        // walls: the vertices as float4 streams for the wall DPD, their
+        // count on the device, the wall repulsion coefficient (0: none)
+        float4 *vpos = nullptr, *vvel = nullptr;
+        int *nvtx_dev = nullptr;
+        float wall_lambda = 0.f;
     } mb;
 };
 
-// FSI seeds of step k: after the NSEED x seed_cap solvent table
+// FSI seeds of step k: after the NSEED x seed_cap solvent table; then the
+// seeds of the membranes' wall DPD, one per step
 static const float *flu_fsi_seeds(const udx_flu *f, int k) { return f->seeds_dev + NSEED * f->seed_cap + 2 * k; }
+static const float *flu_mwall_seed(const udx_flu *f, int k) { return f->seeds_dev + (NSEED + 2) * f->seed_cap + k; }
 
 // keep_bounce: a new membrane capacity keeps the bounce-back set-up
 static void flu_free_membranes(udx_flu *f, bool keep_bounce = false) {
@@ -164,6 +172,10 @@ static void flu_free_membranes(udx_flu *f, bool keep_bounce = false) {
     dev_free(f->mb.cfid);
     dev_free(f->mb.cnew);
     dev_free(f->mb.start_pp);
+    dev_free(f->mb.vpos);
+    dev_free(f->mb.vvel);
+    dev_free(f->mb.nvtx_dev);
+    const float wall_lambda = f->mb.wall_lambda;
     udx_mbb *bb = f->mb.bb;
     float4 *pos0 = f->mb.pos0;
     udx_cnt *cnt = f->mb.cnt;
@@ -174,6 +186,7 @@ static void flu_free_membranes(udx_flu *f, bool keep_bounce = false) {
         f->mb.pos0 = pos0;
         f->mb.cnt = cnt;
         f->mb.cnt_par = cnt_par;
+        f->mb.wall_lambda = wall_lambda;
     } else {
         dev_free(pos0);
     }
@@ -475,18 +488,24 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
         if (f->seeds_host) (void)hipHostFree(f->seeds_host);
         f->seeds_host = nullptr;
         f->seed_cap = 0;
-        UDX_TRY(dev_alloc(&f->seeds_dev, (NSEED + 2) * (size_t)nsteps));
-        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, (NSEED + 2) * sizeof(float) * (size_t)nsteps));
+        UDX_TRY(dev_alloc(&f->seeds_dev, (NSEED + 3) * (size_t)nsteps));
+        UDX_HIP(hipHostMalloc((void **)&f->seeds_host, (NSEED + 3) * sizeof(float) * (size_t)nsteps));
         f->seed_cap = nsteps;
     } else {
         // the staging buffer may still feed an earlier copy
         UDX_HIP(hipStreamSynchronize(s));
     }
+    // the wall trunk draws for the solvent, then for the membranes
+    // (wall_interact, src/sim/walls/imp/main.h:47-61): with a seed table the
+    // solvent's comes from it and the membranes' from the wall trunk
+    float *mw = f->seeds_host + (NSEED + 2) * f->seed_cap;
+    const bool mwall = f->wall && f->mb.r;
     for (int k = 0; k < nsteps; k++) {
         if (seeds_host)
             memcpy(f->seeds_host + NSEED * k, seeds_host + NSEED * k, NSEED * sizeof(float));
         else
             f->trunks.next(f->seeds_host + NSEED * k);
+        mw[k] = mwall ? udx_rnd_get(&f->trunks.t[27]) : 0.f;
     }
     // the FSI trunk draws twice per step, bulk then halo (src/fsi/imp/bulk.h:8, src/fsi/imp/halo.h:7)
     float *fsi = f->seeds_host + NSEED * f->seed_cap;
@@ -498,13 +517,15 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
                            hipMemcpyHostToDevice, s));
     UDX_HIP(hipMemcpyAsync(f->seeds_dev + NSEED * f->seed_cap, fsi, 2 * sizeof(float) * (size_t)nsteps,
                            hipMemcpyHostToDevice, s));
+    UDX_HIP(hipMemcpyAsync(f->seeds_dev + (NSEED + 2) * f->seed_cap, mw, sizeof(float) * (size_t)nsteps,
+                           hipMemcpyHostToDevice, s));
     return UDX_OK;
 }
 
 // one step of src/sim/imp/run.h:8-68: solvent cell list + reorder, membrane
 // redistribution, DPD forces, walls, FSI, body force + solvent kick-drift +
 // bounce + periodic wrap, membrane sub-steps (FSI + internal forces, kick-drift)
-static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) {
+static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, const float *mwall_seed) {
     hipStream_t s = f->ctx->stream;
     const int nc = f->box.ncells();
     const int a = f->cur, b = 1 - f->cur;
@@ -548,8 +569,16 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
                             f->conf.mode == UDX_MODE_EXACT, wv));
     }
     if (m.r && nvtx > 0) {
-        // FSI: +f on the vertices, -f on the solvent (src/sim/objinter/imp/main.h:42-91)
         UDX_HIP(hipMemsetAsync(m.fo, 0, 3 * sizeof(float) * (size_t)nvtx, s));
+        if (f->wall) {
+            // the vertices' wall DPD with the FSI pair, then the wall
+            // repulsion (forces_wall, src/sim/imp/force.h:37-59)
+            UDX_TRY(aos_to_f4(s, nvtx, m.pp, m.vpos, m.vvel));
+            UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), nvtx, m.nvtx_dev, m.vpos, m.vvel, m.fp, mwall_seed, m.fo,
+                                3, wall_pairs_dev(f->wall), f->conf.mode == UDX_MODE_EXACT, wv));
+            if (m.wall_lambda > 0) UDX_TRY(wall_repulse_aos(s, sdf, m.wall_lambda, nvtx, m.pp, m.fo));
+        }
+        // FSI: +f on the vertices, -f on the solvent (src/sim/objinter/imp/main.h:42-91)
         UDX_TRY(fsi_forces(s, fsi_bufs(m.fsi), f->n, f->pos[b], f->vel[b], f->cl.starts, nvtx, m.pp, m.fp, fsi_seeds,
                            m.fo, (float *)f->ff, 4, fsi_pairs_dev(m.fsi), f->conf.mode == UDX_MODE_EXACT));
         // contact (forces_cnt, cnt_halo: src/sim/objinter/imp/main.h:22-26,79)
@@ -575,6 +604,8 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds) 
             UDX_TRY(move_aos(s, nvtx, sub_dt, m.mass, m.ffast, m.pp));
         }
     }
+    // the vertices' wall bounce-back (bounce_wall, src/sim/imp/update.h:41-53)
+    if (f->wall && m.r && nvtx > 0) UDX_TRY(wall_bounce_aos(s, sdf, f->conf.dt, nvtx, m.pp, wv));
     // objects_bounce (src/sim/objects/imp/bounce.h:236-248); also the wrap
     // the move left out
     if (bounce)
@@ -592,7 +623,7 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
     UDX_TRY(flu_seed_table(f, nsteps, seeds_host));
     float acc[4] = {0, 0, 0, 0};
     for (int k = 0; k < nsteps; k++) {
-        UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k, flu_fsi_seeds(f, k)));
+        UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k, flu_fsi_seeds(f, k), flu_mwall_seed(f, k)));
         if (f->timing) {
             UDX_HIP(hipEventSynchronize(f->ev[4]));
             for (int c = 0; c < 4; c++) {
@@ -870,7 +901,6 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
     UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_wall: null handle");
     UDX_CHECK(!w || (w->sdf->f.L[0] == f->box.L[0] && w->sdf->f.L[1] == f->box.L[1] && w->sdf->f.L[2] == f->box.L[2]),
               UDX_EINVAL, "udx_flu_set_wall: wall and solvent boxes differ");
-    UDX_CHECK(!w || !f->mb.r, UDX_EINVAL, "udx_flu_set_wall: membranes together with walls are not supported");
     f->wall = w;
     return UDX_OK;
 }
@@ -929,7 +959,6 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
     UDX_CHECK(fsi && rp && fp && (nc == 0 || pp), UDX_EINVAL, "udx_flu_set_rbc: null argument");
     UDX_CHECK(rbc_ctx(r) == f->ctx && fsi_ctx(fsi) == f->ctx, UDX_EINVAL,
               "udx_flu_set_rbc: membranes, FSI and solvent must share one context");
-    UDX_CHECK(!f->wall, UDX_EINVAL, "udx_flu_set_rbc: membranes together with walls are not supported");
     UDX_CHECK(mass > 0 && substeps >= 1, UDX_EINVAL, "udx_flu_set_rbc: bad mass or substeps");
     const int nv = rbc_nv(r);
     UDX_CHECK(nc >= 0 && nc <= rbc_max_cells(r), UDX_ECAP, "udx_flu_set_rbc: %d cells above the membrane capacity %d",
@@ -946,6 +975,8 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
         flu_free_membranes(f, true);
         UDX_TRY(dev_alloc(&m.fo, 3 * (size_t)n));
         UDX_TRY(dev_alloc(&m.start_pp, 6 * (size_t)(n > 0 ? n : 1)));
+        UDX_TRY(dev_alloc(&m.vpos, (size_t)(n > 0 ? n : 1)));
+        UDX_TRY(dev_alloc(&m.vvel, (size_t)(n > 0 ? n : 1)));
         UDX_TRY(dev_alloc(&m.ffast, 3 * (size_t)n));
         UDX_TRY(dev_alloc(&m.tmp, 6 * (size_t)n));
         UDX_TRY(dev_alloc(&m.cfid, (size_t)nc));
@@ -961,10 +992,22 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
     m.nc = nc;
     m.nv = nv;
     m.pp = pp;
+    if (!m.nvtx_dev) UDX_TRY(dev_alloc(&m.nvtx_dev, 1));
+    {
+        const int nvtx = nc * nv;
+        UDX_HIP(hipStreamSynchronize(f->ctx->stream));
+        UDX_HIP(hipMemcpy(m.nvtx_dev, &nvtx, sizeof nvtx, hipMemcpyHostToDevice));
+    }
     if (m.bb && mbb_nv(m.bb) != nv) {
         m.bb = nullptr;
         UDX_CHECK(false, UDX_EINVAL, "udx_flu_set_rbc: the bounce-back template differs; bounce-back switched off");
     }
+    return UDX_OK;
+}
+
+int udx_flu_set_rbc_repulsion(udx_flu *f, float lambda) {
+    UDX_CHECK(f && lambda >= 0, UDX_EINVAL, "udx_flu_set_rbc_repulsion: bad arguments");
+    f->mb.wall_lambda = lambda;
     return UDX_OK;
 }
 

@@ -146,6 +146,7 @@ _SIGS = {
     "udx_cnt_apply_seed": (_I, [_VP, C.POINTER(PairDpdLj), _I, _VP, _F, _F, _VP]),
     "udx_cnt_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
     "udx_flu_set_rbc_contact": (_I, [_VP, _VP, C.POINTER(PairDpdLj)]),
+    "udx_flu_set_rbc_repulsion": (_I, [_VP, _F]),
     "udx_wall_repulse": (_I, [_VP, _F, _I, _VP, _VP]),
     "udx_rbc_ini": (_I, [_VP, _I, _I, C.POINTER(_I), _I, C.POINTER(_VP)]),
     "udx_rbc_fin": (_I, [_VP]),
@@ -446,6 +447,10 @@ class Flu:
     def set_time(self, t):
         """the step clock (time of the next step), advanced by dt per step"""
         _ok(lib().udx_flu_set_time(self.h, float(t)))
+
+    def set_rbc_repulsion(self, lam):
+        """wall repulsion of the membrane vertices (0: none)"""
+        _ok(lib().udx_flu_set_rbc_repulsion(self.h, float(lam)))
 
     def set_rbc_contact(self, cnt, params=None):
         """membrane contact in the step (udx_flu_set_rbc_contact); None: off"""

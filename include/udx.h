@@ -325,6 +325,14 @@ int udx_cnt_pairs(udx_cnt *c, unsigned long long *npairs, int reset);
 /* contact between the membranes of udx_flu_set_rbc inside udx_flu_run (with
  * the FSI force, src/sim/objinter/imp/main.h:42-80); NULL switches it off */
 int udx_flu_set_rbc_contact(udx_flu *f, udx_cnt *c, const udx_pair_dpd_lj *p);
+/* walls and membranes together (udx_flu_set_wall and udx_flu_set_rbc): the
+ * vertices get the wall DPD with the FSI pair (seeded by the wall trunk after
+ * the solvent's draw of each step; with a seed table the solvent's wall seed
+ * is its column 27 and the membranes' the handle's next wall-trunk draw; pairs
+ * counted by udx_wall_pairs), the wall repulsion with coefficient lambda
+ * (wall_repulse, src/wall/force/imp.h:17-19; 0, the default, for "none") and
+ * the wall bounce-back after the membrane advance */
+int udx_flu_set_rbc_repulsion(udx_flu *f, float lambda);
 
 /* ---- membrane bounce-back: replaces mesh_bounce_ini / _fin /
  *      _find_collisions / _select_collisions / _bounce /

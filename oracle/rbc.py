@@ -324,3 +324,41 @@ def cnt_forces(L, pp, params, seed_bulk, seed_halo, ff=None):
     n = _lib().orc_cnt_forces(_ptr(_i32(L), I32P), len(pp), _ptr(pp, F32P), DpdLj(*params), np.float32(seed_bulk),
                               np.float32(seed_halo), _ptr(ff, F32P))
     return ff, int(n)
+
+
+def suspension_wall_step(L, spp, opp, mesh, nc, dpd, seeds28, fsi_par, fsi_seed2, mwall_seed, p, dt, sdf, wpp, wstarts,
+                         lam=0.0, mass=1.0, rbc_mass=0.5, substeps=1, l0=0.0, a0=0.0, nthreads=1,
+                         accel=(0.0, 0.0, 0.0), bounce=False):
+    """One step of solvent + membranes between walls, in the order of
+    src/sim/imp/run.h:8-68 and src/sim/imp/force.h:122-141: solvent cell list,
+    membrane redistribution, solvent DPD, wall DPD of the solvent (seed
+    seeds28[27]) and of the vertices (FSI pair, seed mwall_seed), wall
+    repulsion of the vertices (lam > 0), FSI, solvent kick-drift + wall
+    bounce-back + wrap, membrane sub-steps, the vertices' wall bounce-back.
+    Returns (cell-ordered solvent, vertices, solvent pairs, FSI pairs,
+    solvent-wall pairs, vertex-wall pairs).  With bounce, the solvent moves
+    without the wrap, bounces back off the membranes after the vertices' wall
+    bounce-back (src/sim/imp/run.h:54-56) and is wrapped after it; the number
+    of bounced particles is returned seventh."""
+    from . import oracle as O
+    from . import wall as W
+    starts, counts, srt, _ = O.clist_build(L, spp)
+    opp, _ = redistribute(L, opp, nc, mesh.nv)
+    start = opp.copy()
+    ff, npair = O.dpd_forces(L, srt, starts, counts, dpd, seeds28[:27], nthreads=nthreads)
+    ff, nw = W.wall_forces(L, srt, sdf, wpp, wstarts, dpd, seeds28[27], ff)
+    fo, nwm = W.wall_forces(L, opp, sdf, wpp, wstarts, fsi_par, mwall_seed)
+    if lam > 0:
+        fo = sdf.repulse(opp, lam, fo)
+    fo, ff, nfsi = fsi_forces(L, srt, starts, opp, fsi_par, fsi_seed2, fo=fo, fs=ff)
+    sout = W.integrate_wall(L, srt, ff, dt, sdf, mass, accel, wrap=not bounce)
+    sub_dt = np.float32(np.float32(dt) / np.float32(substeps))
+    for _ in range(substeps):
+        F = forces(p, mesh, opp, nc, area_volume(mesh, opp, nc), l0=l0, a0=a0)
+        opp = O.move(opp, fo + F.astype(np.float32), sub_dt, rbc_mass)
+    opp = sdf.bounce(opp, dt)
+    if not bounce:
+        return sout, opp, npair // 2, nfsi, nw, nwm
+    from . import mbb as B
+    sout, opp, _, _, ncol = B.bounce(dt, mass, rbc_mass, L, nc, mesh.tt, start[:, :3], opp, starts, srt, sout)
+    return O.wrap(L, sout), opp, npair // 2, nfsi, nw, nwm, ncol

@@ -149,6 +149,9 @@ void  orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp, const orc_wv
 /* kick-drift (+ body force), bounce-back when q != NULL, periodic wrap */
 void  orc_integrate_wall(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
                          float *pp, const orc_sdf *q, const orc_wvel *wv);
+/* the same with the wrap optional (the membrane bounce-back wraps after it) */
+void  orc_integrate_wall_ex(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
+                            float *pp, const orc_sdf *q, const orc_wvel *wv, int wrap);
 void  orc_wall_repulse(const orc_sdf *q, float lambda, int n, const float *pp, float *ff);
 
 /* ---- RBC membrane (udx_oracle_rbc.c) ------------------------------------ */

@@ -405,8 +405,8 @@ void orc_wall_bounce(const orc_sdf *q, float dt, int n, float *pp, const orc_wve
 
 /* kick-drift with the constant body force, bounce-back, periodic wrap:
  * the solvent update of a wall-bounded step (src/sim/imp/run.h:43-54) */
-void orc_integrate_wall(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
-                        float *pp, const orc_sdf *q, const orc_wvel *wv) {
+void orc_integrate_wall_ex(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
+                           float *pp, const orc_sdf *q, const orc_wvel *wv, int wrap) {
     for (int i = 0; i < n; i++) {
         float *r = pp + 6 * i, *v = r + 3;
         const float *f = ff + 3 * i;
@@ -419,12 +419,18 @@ void orc_integrate_wall(const int L[3], int n, float dt, float mass, const float
             float s = orc_sdf_eval(q, r[0], r[1], r[2]);
             if (s >= 0) bounce_1p(q, wv, dt, s, r, v);
         }
+        if (!wrap) continue;
         for (int k = 0; k < 3; k++) {
             float Lk = (float)L[k], h = 0.5f * Lk;
             if (r[k] < -h) r[k] = r[k] + Lk;
             if (r[k] >= h) r[k] = r[k] - Lk;
         }
     }
+}
+
+void orc_integrate_wall(const int L[3], int n, float dt, float mass, const float accel[3], const float *ff,
+                        float *pp, const orc_sdf *q, const orc_wvel *wv) {
+    orc_integrate_wall_ex(L, n, dt, mass, accel, ff, pp, q, wv, 1);
 }
 
 /* repulsion of object particles (src/wall/force/dev/main.h:52-91) */

@@ -194,18 +194,18 @@ def wvel_at(wv, r):
     return out
 
 
-def integrate_wall(L, pp, ff, dt, sdf=None, mass=1.0, accel=(0.0, 0.0, 0.0), wv=None):
-    """Kick-drift, bounce-back (when sdf is given), periodic wrap."""
+def integrate_wall(L, pp, ff, dt, sdf=None, mass=1.0, accel=(0.0, 0.0, 0.0), wv=None, wrap=True):
+    """Kick-drift, bounce-back (when sdf is given), periodic wrap (optional)."""
     L_ = _lib()
-    if not hasattr(L_.orc_integrate_wall, "_udx_decl"):
-        L_.orc_integrate_wall.restype = None
-        L_.orc_integrate_wall.argtypes = [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P,
-                                          C.POINTER(SdfStruct), C.POINTER(Wvel)]
-        L_.orc_integrate_wall._udx_decl = True
+    if not hasattr(L_.orc_integrate_wall_ex, "_udx_decl"):
+        L_.orc_integrate_wall_ex.restype = None
+        L_.orc_integrate_wall_ex.argtypes = [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P,
+                                             C.POINTER(SdfStruct), C.POINTER(Wvel), C.c_int]
+        L_.orc_integrate_wall_ex._udx_decl = True
     pp = _f32(pp).reshape(-1, 6).copy()
-    L_.orc_integrate_wall(_ptr(_i32(L), I32P), len(pp), np.float32(dt), np.float32(mass), _ptr(_f32(accel), F32P),
-                          _ptr(_f32(ff), F32P), _ptr(pp, F32P), C.byref(sdf.st) if sdf is not None else None,
-                          _wv(wv))
+    L_.orc_integrate_wall_ex(_ptr(_i32(L), I32P), len(pp), np.float32(dt), np.float32(mass),
+                             _ptr(_f32(accel), F32P), _ptr(_f32(ff), F32P), _ptr(pp, F32P),
+                             C.byref(sdf.st) if sdf is not None else None, _wv(wv), 1 if wrap else 0)
     return pp
 
 

@@ -169,3 +169,90 @@ def test_distribute_matches_oracle(ctx, nc):
     got = host(d)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     r.close()
+
+
+@pytest.mark.parametrize("lam,bounce", [(0.0, False), (2.0, False), (0.0, True)])
+def test_suspension_step_between_walls(ctx, lam, bounce):
+    """solvent + membranes in a plane channel (|z| < 9 of a 16 x 16 x 24 box),
+    cells close to a wall: one exact-mode step from the internal seed trunks
+    (the wall trunk drawing for the solvent, then for the vertices) against
+    the oracle: the solvent bit for bit, the vertices within 1e-6, the same
+    solvent-wall, vertex-wall and FSI pair counts; with the membrane
+    bounce-back also the same bounced particles (hotter solvent, longer step)"""
+    from oracle import wall as W
+    from membranes import mesh as tmesh, rotation
+    L, h, kBT = (16, 16, 24), 9.0, 0.0444
+    dt, vsig = (5e-3, 1.0) if bounce else (2e-3, np.sqrt(kBT))
+    m = tmesh(498)
+    rng = np.random.default_rng(5)
+    # the first cell 0.4 from the lower wall, the second 0.45 from the upper
+    # one and across the x boundary, the third in the middle
+    centres = [(-3.0, -3.0, None, 0.4), (7.5, 3.0, None, -0.45), (0.0, 4.0, 0.0, 0.0)]
+    opp = np.zeros((len(centres) * m.nv, 6), np.float32)
+    for k, (cx, cy, cz, gap) in enumerate(centres):
+        r = m.rr.astype(np.float64) @ rotation(rng).T
+        if cz is None:
+            cz = -h + gap - r[:, 2].min() if gap > 0 else h + gap - r[:, 2].max()
+        opp[k * m.nv:(k + 1) * m.nv, :3] = (r + np.array([cx, cy, cz])).astype(np.float32)
+    opp[:, 3:] = rng.normal(0, 0.05, (len(opp), 3)).astype(np.float32)
+    nc = len(centres)
+    ext, N = np.array(L, np.float32), (48, 48, 48)
+    D = W.plane_channel_sdf(L, 48, h)
+    q = U.Sdf(ctx, L, ext=ext, N=N, D=D)
+    ref = W.Sdf(L, ext, N, D.ravel())
+    n0 = 10 * L[0] * L[1] * L[2]
+    pp = np.zeros((n0, 6), np.float32)
+    pp[:, :3] = (rng.uniform(-0.5, 0.5, (n0, 3)) * np.array(L)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, vsig, (n0, 3)).astype(np.float32)
+    ppd = dev(pp)
+    w = U.Wall(ctx, q, ppd, exact=True)
+    solvent = host(ppd)[:w.n_solvent].copy()
+    keep, frozen = ref.freeze(pp)
+    wst, _, wpp, _ = O.clist_build(W.margin_box(L), W.wall_images(L, frozen))
+    tmpl = np.zeros((m.nv, 6), np.float32)
+    tmpl[:, :3] = m.rr
+    A0, V0 = R.area_volume(m, tmpl, 1)
+    kw = dict(R.RBC_TEST, kb=40.0, totArea=np.float32(A0), totVolume=np.float32(V0))
+    l0, a0 = m.stressful(kw["totArea"])
+    dpd = (2.6666667, 8.0, O.dpd_sigma(kBT, 8.0, dt), 0.25)
+    say("suspension between walls: %d solvent, %d cells, lambda %g, bounce %d" % (len(solvent), nc, lam, bounce))
+    flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*dpd), dt, 1.0, exact=True)
+    sd = dev(solvent)
+    flu.set_pp(sd)
+    flu.set_wall(w)
+    r = U.Rbc(ctx, m.tt, m.nv, nc)
+    r.set_stressful(kw["totArea"])
+    r.set_bending(True)
+    fsi = U.Fsi(ctx, L, nc * m.nv)
+    vert = dev(opp)
+    flu.set_rbc(r, fsi, U.rbc_params(**kw), U.PairDpd(*dpd), 0.5, 1, nc, vert)
+    flu.set_rbc_repulsion(lam)
+    if bounce:
+        bb = U.Mbb(ctx, m.nv, m.tt, nc, len(solvent))
+        flu.set_rbc_bounce(bb)
+    w.pairs(reset=True)
+    flu.run(1)
+    out = torch.zeros_like(sd)
+    nout = flu.get_pp(out)
+    gs, gv = host(out)[:nout], host(vert)
+    gp, gw, gwm, gf = flu.pairs(), flu.wall_pairs(), w.pairs(), fsi.pairs()
+    gn = bb.count() if bounce else 0
+    flu.set_rbc(None)
+    flu.close()
+    r.close()
+    wseeds = O.kiss_fill(42, 42, 42, 42, 2)
+    seeds28 = np.concatenate([O.seeds_table(1)[0], [wseeds[0]]]).astype(np.float32)
+    out = R.suspension_wall_step(L, solvent, opp, m, nc, dpd, seeds28, dpd, R.fsi_seeds(1)[0], np.float32(wseeds[1]),
+                                 R.par(**kw), dt, ref, wpp, wst, lam=lam, l0=l0, a0=a0, nthreads=16, bounce=bounce)
+    rs, rv, rp, rf, rw, rwm = out[:6]
+    rn = out[6] if bounce else 0
+    say("pairs gpu", gp, gw, gwm, gf, gn, "oracle", rp, rw, rwm, rf, rn)
+    assert (gp, gw, gwm, gf, gn) == (rp, rw, rwm, rf, rn) and rwm > 0 and (rn > 0 or not bounce)
+    assert np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-6 * L[0]
+    assert np.abs(gv[:, 3:] - rv[:, 3:]).max() <= 1e-5 * max(np.abs(rv[:, 3:]).max(), 1.0)
+    assert np.all(ref.eval(gv[:, :3]) < 0)
+    if bounce:
+        bb.close()
+    w.close()
+    q.close()
