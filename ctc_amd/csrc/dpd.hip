@@ -45,6 +45,54 @@ struct DpdK {
     unsigned long long *pairs;
 };
 
+// What the force pass reads of a box, read once at kernel entry.  The
+// arrays are typed as global memory: pointers read from a table in memory
+// (the batched kernel) are otherwise generic, and generic (flat) loads count
+// against the LDS wait counter as well, so every LDS read of the staged
+// candidate loops would also wait for the outstanding particle loads.  The
+// scalars are copied so that the loops do not read them from the table again
+// (a read inside a conditional block cannot be hoisted out of the loop).
+#define UDX_GLOBAL __attribute__((address_space(1)))
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f3v __attribute__((ext_vector_type(3)));   // 16 B in memory; loads read 12
+
+template <class T>
+__device__ __forceinline__ UDX_GLOBAL T *gptr(T *p) {
+    return (UDX_GLOBAL T *)p;
+}
+
+struct DpdP {
+    const UDX_GLOBAL f4v *pos, *vel;
+    const UDX_GLOBAL int *starts, *fss;
+    const UDX_GLOBAL float *seeds;
+    int Lx, Ly, Lz, nmax, periodic, i0;
+    DpdPar par;
+    // x, y, z of a particle (w is not used)
+    __device__ __forceinline__ float4 p(int j) const {
+        const f3v v = *(const UDX_GLOBAL f3v *)(pos + j);
+        return make_float4(v.x, v.y, v.z, 0.f);
+    }
+    __device__ __forceinline__ float4 v(int j) const {
+        const f3v u = *(const UDX_GLOBAL f3v *)(vel + j);
+        return make_float4(u.x, u.y, u.z, 0.f);
+    }
+};
+
+__device__ __forceinline__ DpdP dpd_view(const DpdK &A) {
+    return DpdP{(const UDX_GLOBAL f4v *)A.pos,
+                (const UDX_GLOBAL f4v *)A.vel,
+                gptr(A.starts),
+                gptr(A.fss),
+                gptr(A.seeds),
+                A.Lx,
+                A.Ly,
+                A.Lz,
+                A.nmax,
+                A.periodic,
+                A.i0,
+                A.par};
+}
+
 // x-runs of row (dy, dz) for a particle in column cx: seg 0 and seg 1 in
 // ascending dx order (dx = -1, 0, +1 across the wrap)
 __device__ __forceinline__ bool xrun(int sg, int cx, int Lx, int periodic, int *x0, int *x1, int *wx) {
@@ -77,14 +125,14 @@ struct PairIn {
     float dx, dy, dz, r2, dvx, dvy, dvz, x;
 };
 
-__device__ __forceinline__ PairIn pair_in(const DpdK &A, unsigned ent, int i, const float4 &pi, const float4 &vi,
-                                          int cx, int cy, int cz, int istart, float seed_bulk) {
+__device__ __forceinline__ PairIn pair_in(const DpdK &A, const DpdP &P, unsigned ent, int i, const float4 &pi,
+                                          const float4 &vi, int cx, int cy, int cz, int istart, float seed_bulk) {
     const int j = (int)(ent & 0x7FFFFFFu);
     const unsigned w = ent >> 27;
-    const float4 pj = A.pos[j], vj = A.vel[j];
+    const float4 pj = P.p(j), vj = P.v(j);
     float ax = pi.x, ay = pi.y, az = pi.z;
     // bulk pair: mean0var1ii(seed, max, min) (src/fluforces/bulk/dev/main.h:32-38)
-    int u = (i > j ? i : j) - A.i0, v = (i > j ? j : i) - A.i0;
+    int u = (i > j ? i : j) - P.i0, v = (i > j ? j : i) - P.i0;
     float seed = seed_bulk;
     if (w != W_NONE) {
         // pair across the periodic boundary: the local particle is moved into
@@ -92,24 +140,24 @@ __device__ __forceinline__ PairIn pair_in(const DpdK &A, unsigned ent, int i, co
         // noise is mean0var1uu(seed_d, ...) of fragment d = w with positions
         // in the packed lists (src/fluforces/halo/dev/main.h:7-12,
         // src/exch/flu/dev/pack.h:2-40)
-        const int Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
+        const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
         const int wx = (int)(w % 3u) - 1, wy = (int)((w / 3u) % 3u) - 1, wz = (int)(w / 9u) - 1;
         ax = pi.x - (float)(wx * Lx);
         ay = pi.y - (float)(wy * Ly);
         az = pi.z - (float)(wz * Lz);
         const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
-        const int aid = A.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] - A.fss[A.foff[fid]] + (i - istart);
+        const int aid = P.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] - P.fss[A.foff[fid]] + (i - istart);
         int jx = 0, jy = 0, jz = 0;
         cell_coord(pj.x, Lx, &jx);
         cell_coord(pj.y, Ly, &jy);
         cell_coord(pj.z, Lz, &jz);
-        const int jstart = A.starts[jx + Lx * (jy + Ly * jz)];
-        const int bid = A.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] - A.fss[A.foff[anti]] +
+        const int jstart = P.starts[jx + Lx * (jy + Ly * jz)];
+        const int bid = P.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] - P.fss[A.foff[anti]] +
                         (j - jstart);
         const bool mask = fid < anti;
         u = mask ? aid : bid;
         v = mask ? bid : aid;
-        seed = A.seeds[fid];
+        seed = P.seeds[fid];
     }
     PairIn o;
     o.dx = ax - pj.x;
@@ -140,22 +188,23 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     __shared__ unsigned long long lpairs[NT / 64];
     constexpr int DPD_THREADS = NT, QCAP = QC;
     const int tid = threadIdx.x;
-    const int Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
-    const int n = A.starts[Lx * Ly * Lz];
+    const DpdP P = dpd_view(A);
+    const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
+    const int n = P.starts[Lx * Ly * Lz];
     const int i = first + tid;
-    const bool active = i < n && i - A.i0 < A.nmax;
+    const bool active = i < n && i - P.i0 < P.nmax;
 
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
     int cx = 0, cy = 0, cz = 0, istart = 0;
     if (active) {
-        pi = A.pos[i];
-        vi = A.vel[i];
+        pi = P.p(i);
+        vi = P.v(i);
         cell_coord(pi.x, Lx, &cx);
         cell_coord(pi.y, Ly, &cy);
         cell_coord(pi.z, Lz, &cz);
-        istart = A.starts[cx + Lx * (cy + Ly * cz)];
+        istart = P.starts[cx + Lx * (cy + Ly * cz)];
     }
-    const float seed_bulk = A.seeds[26];
+    const float seed_bulk = P.seeds[26];
     float fx = 0.f, fy = 0.f, fz = 0.f;
     int npair = 0, fill = 0, head = 0;
     unsigned *qlane = q + tid;
@@ -169,7 +218,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     // would run the expensive pair body with most lanes idle.
     // An idle slot evaluates a discarded pair with an in-bounds particle
     // (i itself, or the box's first particle for a lane past the end).
-    const unsigned idle = (unsigned)(active ? i : A.i0) | (W_NONE << 27);
+    const unsigned idle = (unsigned)(active ? i : P.i0) | (W_NONE << 27);
     auto ring = [&](int k) {
         const int s = head + k;
         return s >= QCAP ? s - QCAP : s;
@@ -178,12 +227,12 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         const bool va = fill > 0, vb = fill > 1;
         const unsigned ea = va ? qlane[ring(0) * DPD_THREADS] : idle;
         const unsigned eb = vb ? qlane[ring(1) * DPD_THREADS] : idle;
-        PairIn a = pair_in(A, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
-        PairIn b = pair_in(A, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+        PairIn a = pair_in(A, P, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+        PairIn b = pair_in(A, P, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
         float xa = a.x, xb = b.x;
         logistic18x2(&xa, &xb);
-        const float3 f1 = dpd_pair<WR, EXACT>(A.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
-        const float3 f2 = dpd_pair<WR, EXACT>(A.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+        const float3 f1 = dpd_pair<WR, EXACT>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+        const float3 f2 = dpd_pair<WR, EXACT>(P.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
         fx += va ? f1.x : 0.f;
         fy += va ? f1.y : 0.f;
         fz += va ? f1.z : 0.f;
@@ -231,19 +280,19 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
         int yy = cy + dy, zz = cz + dz;
         const int wy = yy < 0 ? -1 : (yy >= Ly ? 1 : 0), wz = zz < 0 ? -1 : (zz >= Lz ? 1 : 0);
-        const bool rowok = active && (A.periodic || (wy | wz) == 0);
+        const bool rowok = active && (P.periodic || (wy | wz) == 0);
         yy -= wy * Ly;
         zz -= wz * Lz;
         const int base = Lx * (yy + Ly * zz);
         int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
         if (rowok) {
-            if (xrun(0, cx, Lx, A.periodic, &x0, &x1, &wx0)) {
-                j0 = A.starts[base + x0];
-                n0 = A.starts[base + x1 + 1] - j0;
+            if (xrun(0, cx, Lx, P.periodic, &x0, &x1, &wx0)) {
+                j0 = P.starts[base + x0];
+                n0 = P.starts[base + x1 + 1] - j0;
             }
-            if (xrun(1, cx, Lx, A.periodic, &x0, &x1, &wx1)) {
-                j1 = A.starts[base + x0];
-                n1 = A.starts[base + x1 + 1] - j1;
+            if (xrun(1, cx, Lx, P.periodic, &x0, &x1, &wx1)) {
+                j1 = P.starts[base + x0];
+                n1 = P.starts[base + x1 + 1] - j1;
             }
         }
         const bool simple = !rowok || ((wy | wz | wx0) == 0 && n1 == 0);
@@ -276,7 +325,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         const bool lds_row = fits && all_simple;
         if (lds_row) {
             __builtin_amdgcn_wave_barrier();
-            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = A.pos[lo + k];
+            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = P.p(lo + k);
             __builtin_amdgcn_wave_barrier();
             const float4 *sp = sb + (rowok ? j0 - lo : 0);
             for (int t = 0; wave_any(t < n0); t += 4) {
@@ -302,12 +351,13 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             // shared by the lanes of one y row) still fit the staging buffer:
             // staged as above, each candidate taking its run's shift
             __builtin_amdgcn_wave_barrier();
-            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = A.pos[lo + k];
+            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = P.p(lo + k);
             __builtin_amdgcn_wave_barrier();
             const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
             const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
             const unsigned w0 = wcode(wx0, wy, wz), w1 = wcode(wx1, wy, wz);
-            const int len = n0 + n1, o0 = j0 - lo, o1 = j1 - lo;
+            const int len = n0 + n1;
+            const int o0 = j0 - lo, o1 = j1 - lo;
             for (int t = 0; wave_any(t < len); t += 4) {
                 make_room(4);
 #pragma unroll
@@ -320,7 +370,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
                     const float ax = s1 ? ax1 : ax0;
                     const float ddx = ax - c.x, ddy = ay - c.y, ddz = az - c.z;
                     const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));
-                    const int j = lo + idx;
+                    const int j = s1 ? j1 + (tt - n0) : j0 + tt;
                     push(ok && j != i && r2 < 1.0f, j, s1 ? w1 : w0);
                 }
             }
@@ -336,8 +386,8 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
                 const bool oka = ta < len, okb = tb < len;
                 const int ja = ta >= n0 ? j1 + (ta - n0) : j0 + ta;
                 const int jb = tb >= n0 ? j1 + (tb - n0) : j0 + tb;
-                const float4 pa = A.pos[oka ? ja : 0];
-                const float4 pb = A.pos[okb ? jb : 0];
+                const float4 pa = P.p(oka ? ja : 0);
+                const float4 pb = P.p(okb ? jb : 0);
                 const float axa = ta >= n0 ? ax1 : ax0, axb = tb >= n0 ? ax1 : ax0;
                 const float dxa = axa - pa.x, dya = ay - pa.y, dza = az - pa.z;
                 const float dxb = axb - pb.x, dyb = ay - pb.y, dzb = az - pb.z;
@@ -352,11 +402,12 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
 
     if (active) {
         if (A.ff_stride == 4) {
-            float4 *f4 = (float4 *)A.ff + i;
-            float4 o = A.accumulate ? *f4 : make_float4(0.f, 0.f, 0.f, 0.f);
-            *f4 = make_float4(o.x + fx, o.y + fy, o.z + fz, 0.f);
+            UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
+            f4v o = {0.f, 0.f, 0.f, 0.f};
+            if (A.accumulate) o = *f4;
+            *f4 = f4v{o.x + fx, o.y + fy, o.z + fz, 0.f};
         } else {
-            float *f3 = A.ff + 3 * (long)i;
+            UDX_GLOBAL float *f3 = gptr(A.ff) + 3 * (long)i;
             if (A.accumulate) {
                 f3[0] += fx; f3[1] += fy; f3[2] += fz;
             } else {
