@@ -34,6 +34,17 @@ __device__ __forceinline__ int find_box(const int *off, int nbox, int i) {
     return lo;
 }
 
+// The box of particle i: one search per wave from its first active lane's
+// index (wave-uniform, so it runs on scalar loads), then each lane steps over
+// the box starts between that particle and its own (none or one for boxes of
+// more than 64 particles).  Active lanes hold consecutive indices from the
+// first active one upward.
+__device__ __forceinline__ int find_box_wave(const int *off, int nbox, int i) {
+    int b = find_box(off, nbox, __builtin_amdgcn_readfirstlane(i));
+    while (b + 1 < nbox && off[b + 1] <= i) b++;
+    return b;
+}
+
 // cell (global index) and provisional slot of each particle; consecutive
 // lanes in one cell take their slots with one atomic (as k_clist_count)
 __global__ __launch_bounds__(256) void k_ens_count(int n, int nbox, const int *poff, const int *coff,
@@ -43,7 +54,7 @@ __global__ __launch_bounds__(256) void k_ens_count(int n, int nbox, const int *p
     const int lane = threadIdx.x & 63;
     int cid = -1;
     if (i < n) {
-        const int b = find_box(poff, nbox, i);
+        const int b = find_box_wave(poff, nbox, i);
         const int4 L = boxL[b];
         const float4 r = pos[i];
         int cx, cy, cz;
@@ -98,7 +109,7 @@ __global__ __launch_bounds__(256) void k_ens_move(int n, int nbox, const int *po
                                                   const float4 *ff, float4 *pos, float4 *vel) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const MoveBox m = mb[find_box(poff, nbox, i)];
+    const MoveBox m = mb[find_box_wave(poff, nbox, i)];
     float4 p = pos[i], v = vel[i];
     const float4 f = ff[i];
     kick_drift(m.dt, m.mass, m.ax, f.x, &p.x, &v.x);
