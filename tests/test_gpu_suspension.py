@@ -171,14 +171,17 @@ def test_distribute_matches_oracle(ctx, nc):
     r.close()
 
 
-@pytest.mark.parametrize("lam,bounce", [(0.0, False), (2.0, False), (0.0, True)])
-def test_suspension_step_between_walls(ctx, lam, bounce):
+@pytest.mark.parametrize("lam,bounce,nsteps", [(0.0, False, 1), (2.0, False, 1), (0.0, True, 1), (2.0, False, 3)])
+def test_suspension_step_between_walls(ctx, lam, bounce, nsteps):
     """solvent + membranes in a plane channel (|z| < 9 of a 16 x 16 x 24 box),
     cells close to a wall: one exact-mode step from the internal seed trunks
     (the wall trunk drawing for the solvent, then for the vertices) against
     the oracle: the solvent bit for bit, the vertices within 1e-6, the same
     solvent-wall, vertex-wall and FSI pair counts; with the membrane
-    bounce-back also the same bounced particles (hotter solvent, longer step)"""
+    bounce-back also the same bounced particles (hotter solvent, longer step).
+    Three steps: the wall trunk's draws alternate solvent, vertices, solvent,
+    ... over the steps; positions within 1e-5 of the box, velocities within
+    1e-5 of the largest."""
     from oracle import wall as W
     from membranes import mesh as tmesh, rotation
     L, h, kBT = (16, 16, 24), 9.0, 0.0444
@@ -231,7 +234,7 @@ def test_suspension_step_between_walls(ctx, lam, bounce):
         bb = U.Mbb(ctx, m.nv, m.tt, nc, len(solvent))
         flu.set_rbc_bounce(bb)
     w.pairs(reset=True)
-    flu.run(1)
+    flu.run(nsteps)
     out = torch.zeros_like(sd)
     nout = flu.get_pp(out)
     gs, gv = host(out)[:nout], host(vert)
@@ -240,16 +243,26 @@ def test_suspension_step_between_walls(ctx, lam, bounce):
     flu.set_rbc(None)
     flu.close()
     r.close()
-    wseeds = O.kiss_fill(42, 42, 42, 42, 2)
-    seeds28 = np.concatenate([O.seeds_table(1)[0], [wseeds[0]]]).astype(np.float32)
-    out = R.suspension_wall_step(L, solvent, opp, m, nc, dpd, seeds28, dpd, R.fsi_seeds(1)[0], np.float32(wseeds[1]),
-                                 R.par(**kw), dt, ref, wpp, wst, lam=lam, l0=l0, a0=a0, nthreads=16, bounce=bounce)
-    rs, rv, rp, rf, rw, rwm = out[:6]
-    rn = out[6] if bounce else 0
+    wseeds = O.kiss_fill(42, 42, 42, 42, 2 * nsteps)
+    table, fseeds = O.seeds_table(nsteps), R.fsi_seeds(nsteps)
+    rs, rv, rp, rf, rw, rwm, rn = solvent, opp, 0, 0, 0, 0, 0
+    for k in range(nsteps):
+        seeds28 = np.concatenate([table[k], [wseeds[2 * k]]]).astype(np.float32)
+        out = R.suspension_wall_step(L, rs, rv, m, nc, dpd, seeds28, dpd, fseeds[k], np.float32(wseeds[2 * k + 1]),
+                                     R.par(**kw), dt, ref, wpp, wst, lam=lam, l0=l0, a0=a0, nthreads=16,
+                                     bounce=bounce)
+        rs, rv = out[0], out[1]
+        rp, rf, rw, rwm = rp + out[2], rf + out[3], rw + out[4], rwm + out[5]
+        rn += out[6] if bounce else 0
     say("pairs gpu", gp, gw, gwm, gf, gn, "oracle", rp, rw, rwm, rf, rn)
     assert (gp, gw, gwm, gf, gn) == (rp, rw, rwm, rf, rn) and rwm > 0 and (rn > 0 or not bounce)
-    assert np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
-    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-6 * L[0]
+    if nsteps == 1:
+        assert np.array_equal(gs.view(np.uint32), rs.view(np.uint32))
+        assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-6 * L[0]
+    else:
+        assert np.abs(gs[:, :3] - rs[:, :3]).max() <= 1e-5 * L[0]
+        assert np.abs(gs[:, 3:] - rs[:, 3:]).max() <= 1e-5 * np.abs(rs[:, 3:]).max()
+        assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-5 * L[0]
     assert np.abs(gv[:, 3:] - rv[:, 3:]).max() <= 1e-5 * max(np.abs(rv[:, 3:]).max(), 1.0)
     assert np.all(ref.eval(gv[:, :3]) < 0)
     if bounce:
