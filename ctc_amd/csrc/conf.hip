@@ -546,4 +546,18 @@ int udx_wall_vel_set_conf(const udx_conf *c, udx_wall_vel *v) {
     return UDX_OK;
 }
 
+int udx_wall_repulse_set_conf(const udx_conf *c, const char *ns, float *lambda) {
+    UDX_CHECK(c && ns && lambda, UDX_EINVAL, "udx_wall_repulse_set_conf: null argument");
+    const char *grp;
+    UDX_TRY(udx_conf_lookup_string(c, (std::string(ns) + ".repulsion").c_str(), &grp));
+    if (std::string(grp) == "none") {
+        *lambda = 0.f;
+        return UDX_OK;
+    }
+    float l;
+    UDX_TRY(udx_conf_lookup_float(c, (std::string(grp) + ".lambda").c_str(), &l));
+    *lambda = l;
+    return UDX_OK;
+}
+
 }  // extern "C"

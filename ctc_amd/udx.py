@@ -125,6 +125,7 @@ _SIGS = {
     "udx_wall_bounce_wall": (_I, [_VP, _F, _I, _VP]),
     "udx_flu_set_time": (_I, [_VP, _F]),
     "udx_wall_vel_set_conf": (_I, [_VP, C.POINTER(WallVel)]),
+    "udx_wall_repulse_set_conf": (_I, [_VP, C.c_char_p, C.POINTER(_F)]),
     "udx_restart_write_pp": (_I, [C.c_char_p, C.c_char_p, _I, C.c_long, _VP]),
     "udx_restart_read_pp": (_I, [C.c_char_p, C.c_char_p, _I, _I, C.POINTER(_I), _VP]),
     "udx_restart_write_ii": (_I, [C.c_char_p, C.c_char_p, _I, C.c_long, _VP]),
@@ -928,6 +929,12 @@ class Conf:
         v = WallVel()
         _ok(lib().udx_wall_vel_set_conf(self.h, C.byref(v)))
         return v
+
+    def wall_repulse(self, ns):
+        """the wall repulsion coefficient of object type ns (0 for "none")"""
+        lam = _F()
+        _ok(lib().udx_wall_repulse_set_conf(self.h, ns.encode(), C.byref(lam)))
+        return lam.value
 
     def close(self):
         if self.h:

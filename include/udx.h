@@ -409,6 +409,12 @@ int udx_rbc_params_set_conf(const udx_conf *c, const char *ns, udx_rbc_params *p
 /* wvel.type ("constant", "shear", "shear sin", "hele shaw") and its keys
  * (wvel_set_conf, src/wall/wvel/conf.cpp:10-47) */
 int udx_wall_vel_set_conf(const udx_conf *c, udx_wall_vel *v);
+/* the wall repulsion coefficient of an object type for
+ * udx_flu_set_rbc_repulsion (ini_repulsion_params,
+ * src/sim/objects/imp/ini.h:69-75, and wall_repulse_prm_ini_conf,
+ * src/wall/imp/repulse.h:8-12): <ns>.repulsion is "none" (lambda = 0) or the
+ * name of a group whose key lambda is read */
+int udx_wall_repulse_set_conf(const udx_conf *c, const char *ns, float *lambda);
 
 /* ---- restart files: replaces restart_write_pp / restart_read_pp /
  *      restart_write_ii / restart_read_ii (src/io/restart/imp.h:14-32),

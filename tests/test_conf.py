@@ -125,3 +125,15 @@ def test_wall_vel_conf():
         U.Conf(args='wvel = { type = "shear sin" gdot = 2.0 vdir = 1 gdir = 0 half = 0 }').wall_vel()
     with pytest.raises(U.UdxError, match="wrong type"):
         U.Conf(args='wvel = { type = "shear" gdot = 1 vdir = 0 gdir = 2 half = 0 }').wall_vel()
+
+
+def test_wall_repulse_conf():
+    """<obj>.repulsion names "none" or a group with lambda
+    (src/sim/objects/imp/ini.h:69-75, src/wall/imp/repulse.h:8-12)"""
+    assert U.Conf(args='rbc = { repulsion = "none" }').wall_repulse("rbc") == 0.0
+    c = U.Conf(args='rbc = { repulsion = "wrep" } wrep = { lambda = 2.5 }')
+    assert c.wall_repulse("rbc") == 2.5
+    with pytest.raises(U.UdxError, match="could not find"):
+        U.Conf(args='rbc = { repulsion = "wrep" }').wall_repulse("rbc")
+    with pytest.raises(U.UdxError, match="could not find"):
+        U.Conf(args='rbc = { }').wall_repulse("rbc")
