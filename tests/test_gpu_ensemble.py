@@ -155,3 +155,31 @@ def test_tile_variants_identical_and_autotune(ctx):
     say("autotune", best, ms)
     assert 0 <= best < 6 and all(t > 0 for t in ms)
     e.close()
+
+
+def test_tiny_boxes_equal_single_boxes(ctx):
+    """boxes of 27 to 48 particles (3^3 to 3x4x4 cells at density 1): one wave
+    of 64 particles spans three or more boxes, so the cell-count and move
+    kernels step over several box starts per lane"""
+    rng = np.random.default_rng(8)
+    boxes, pps = [], []
+    for b in range(11):
+        L = (3, 3 + b % 2, 3 + (b // 2) % 2)
+        n = L[0] * L[1] * L[2]
+        pp = np.zeros((n, 6), np.float32)
+        pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+        pp[:, 3:] = rng.normal(0, 0.5, (n, 3)).astype(np.float32)
+        par = (25.0, 4.5, O.dpd_sigma(0.25, 4.5, 1e-3), 0.25)
+        boxes.append(dict(L=L, n=n, params=U.PairDpd(*par), dt=1e-3, mass=1.0, accel=(0.05 * b, 0.0, 0.0), par=par))
+        pps.append(pp)
+    nsteps = 5
+    seeds = U.ens_trunk_seeds(len(boxes), nsteps)
+    say("tiny boxes", [b["n"] for b in boxes])
+    got, pairs = run_ensemble(ctx, boxes, pps, nsteps, seeds, True)
+    off = np.cumsum([0] + [b["n"] for b in boxes])
+    total = 0
+    for b in range(len(boxes)):
+        want, p = run_single(ctx, boxes[b], pps[b], seeds[:, b], True)
+        total += p
+        assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), want.view(np.uint32)), "box %d" % b
+    assert pairs == total > 0
