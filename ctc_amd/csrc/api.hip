@@ -140,6 +140,7 @@ struct udx_flu {
         udx_pair_dpd fp;
         float mass = 0;
         int substeps = 0, nc = 0, nv = 0, cap = 0;
+        int ccap = 0;                                      // cells the per-cell arrays (cfid, cnew) hold
         float *pp = nullptr;                               // the caller's vertices (AoS), advanced in place
         float *fo = nullptr, *ffast = nullptr, *tmp = nullptr;   // FSI force, sub-step force, redistribution staging
         int *cfid = nullptr, *cnew = nullptr;
@@ -243,7 +244,7 @@ int udx_clist_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_clist **ou
               Lx, Ly, Lz, maxp);
     udx_clist *c = new udx_clist;
     c->ctx = ctx;
-    int s = clist_alloc(&c->c, make_box(Lx, Ly, Lz), maxp);
+    int s = clist_alloc(&c->c, make_box(Lx, Ly, Lz), maxp, SOLVENT_SUB);
     if (s) {
         clist_free(&c->c);
         delete c;
@@ -421,7 +422,7 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
         if (!s) s = dev_alloc(&f->vel[b], conf->maxp);
     }
     if (!s) s = dev_alloc(&f->ff, conf->maxp);
-    if (!s) s = clist_alloc(&f->cl, f->box, conf->maxp);
+    if (!s) s = clist_alloc(&f->cl, f->box, conf->maxp, SOLVENT_SUB);
     if (!s) s = frag_alloc(&f->frag, f->box);
     if (!s) s = dev_alloc(&f->pairs_dev, 2);
     if (!s && hipMemset(f->pairs_dev, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
@@ -499,7 +500,9 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
     // (wall_interact, src/sim/walls/imp/main.h:47-61): with a seed table the
     // solvent's comes from it and the membranes' from the wall trunk
     float *mw = f->seeds_host + (NSEED + 2) * f->seed_cap;
-    const bool mwall = f->wall && f->mb.r;
+    // the reference skips wall_force and its draw when there are no vertices
+    // (src/sim/walls/imp/main.h:54-58, src/wall/force/imp/main.h:12)
+    const bool mwall = f->wall && f->mb.r && (long)f->mb.nc * f->mb.nv > 0;
     for (int k = 0; k < nsteps; k++) {
         if (seeds_host)
             memcpy(f->seeds_host + NSEED * k, seeds_host + NSEED * k, NSEED * sizeof(float));
@@ -545,6 +548,8 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     d.pos = f->pos[b];
     d.vel = f->vel[b];
     d.starts = f->cl.starts;
+    d.bstarts = f->cl.bstarts;
+    d.sub = f->cl.sub;
     d.frag = &f->frag;
     d.box = f->box;
     d.nmax = f->n;
@@ -586,6 +591,11 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
             const float sb = cnt_draw(m.cnt), sh = cnt_draw(m.cnt);
             UDX_TRY(cnt_forces(s, m.cnt, m.cnt_par, nvtx, m.pp, sb, sh, m.fo));
         }
+    } else if (m.r && m.cnt) {
+        // the object interactions run for every object type, also one with
+        // no cells, and draw their seeds (src/sim/objinter/imp/main.h:42-51)
+        (void)cnt_draw(m.cnt);
+        (void)cnt_draw(m.cnt);
     }
     if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
     // store_solvent (src/sim/imp/update.h:11-16): positions before the move
@@ -970,7 +980,10 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
               "udx_flu_set_rbc: FSI and solvent boxes differ");
     udx_flu::Membranes &m = f->mb;
     const int n = nc * nv;
-    if (n > m.cap) {
+    // vertex arrays hold m.cap entries and the per-cell ones m.ccap: a new
+    // mesh with fewer vertices per cell can bring more cells in the same
+    // vertex capacity
+    if (n > m.cap || nc > m.ccap) {
         UDX_HIP(hipStreamSynchronize(f->ctx->stream));
         flu_free_membranes(f, true);
         UDX_TRY(dev_alloc(&m.fo, 3 * (size_t)n));
@@ -979,9 +992,10 @@ int udx_flu_set_rbc(udx_flu *f, udx_rbc *r, udx_fsi *fsi, const udx_rbc_params *
         UDX_TRY(dev_alloc(&m.vvel, (size_t)(n > 0 ? n : 1)));
         UDX_TRY(dev_alloc(&m.ffast, 3 * (size_t)n));
         UDX_TRY(dev_alloc(&m.tmp, 6 * (size_t)n));
-        UDX_TRY(dev_alloc(&m.cfid, (size_t)nc));
-        UDX_TRY(dev_alloc(&m.cnew, (size_t)nc));
+        UDX_TRY(dev_alloc(&m.cfid, (size_t)(nc > 0 ? nc : 1)));
+        UDX_TRY(dev_alloc(&m.cnew, (size_t)(nc > 0 ? nc : 1)));
         m.cap = n;
+        m.ccap = nc;
     }
     m.r = r;
     m.fsi = fsi;

@@ -3,8 +3,8 @@
 // The reference assigns each particle a slot with atomicAdd (so its in-cell
 // order depends on thread timing, src/clist/dev/main.h:44-68) and builds the
 // start table with a three-kernel uchar scan (src/algo/scan/dev.h).  Here the
-// in-cell order is defined: ties keep the input order (a stable counting
-// sort).  Atomics still hand out provisional slots; a per-cell insertion
+// in-cell order is defined: by x slab (each cell cut into `sub` slabs along
+// x), ties keeping the input order (a stable counting sort over slabs).  Atomics still hand out provisional slots; a per-cell insertion
 // sort of the (few) slot owners makes the result independent of timing.
 // Start/count tables come from a wavefront scan (64-lane shuffles, one LDS
 // exchange per block).
@@ -108,20 +108,20 @@ struct PosAos {
     }
 };
 
-// Cell of each particle and a provisional slot in it.  Consecutive lanes
-// with the same cell (the usual case: the input is the previous step's
+// Slab of each particle and a provisional slot in it.  Consecutive lanes
+// with the same slab (the usual case: the input is the previous step's
 // cell-ordered array) form a segment that takes its slots with one atomic.
 template <typename Pos>
-__global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int Ly, int Lz, int *counts,
+__global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int Ly, int Lz, int sub, int *counts,
                                                      int2 *cellslot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     int cid = -1;
     if (i < n) {
         const float3 r = pos(i);
-        int cx, cy, cz;
-        if (cell_coord(r.x, Lx, &cx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz))
-            cid = cx + Lx * (cy + Ly * cz);
+        int bx, cy, cz;
+        if (slab_coord(r.x, Lx, sub, &bx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz))
+            cid = bx + Lx * sub * (cy + Ly * cz);
     }
     const int prev = __shfl_up(cid, 1, 64);
     const bool head = lane == 0 || prev != cid;
@@ -134,6 +134,16 @@ __global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int
     if (head && cid >= 0) base = atomicAdd(counts + cid, next - lane);
     base = __shfl(base, hlane, 64);
     if (i < n) cellslot[i] = make_int2(cid, base + (lane - hlane));
+}
+
+// cell tables from the slab starts: starts[c] = bstarts[c sub]
+__global__ __launch_bounds__(256) void k_cell_tables(int ncells, int sub, const int *bstarts, int *starts,
+                                                     int *counts) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncells) return;
+    const int s0 = bstarts[c * sub];
+    starts[c] = s0;
+    if (c < ncells) counts[c] = bstarts[(c + 1) * sub] - s0;
 }
 
 __global__ __launch_bounds__(256) void k_clist_scatter(int n, const int2 *cellslot, const int *starts,
@@ -225,20 +235,35 @@ int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch) 
     return UDX_OK;
 }
 
-int clist_alloc(ClistBufs *c, const Box &b, int maxp) {
+int clist_alloc(ClistBufs *c, const Box &b, int maxp, int sub) {
     *c = ClistBufs{};
     c->box = b;
     c->maxp = maxp;
-    int nc = b.ncells();
+    c->sub = sub;
+    UDX_CHECK(sub == 1 || sub == 2 || sub == 4 || sub == 8, UDX_EINVAL, "clist: %d slabs per cell (1, 2, 4 or 8)", sub);
+    const int nc = b.ncells();
+    UDX_CHECK((long)nc * sub < (1L << 30), UDX_ECAP, "clist: %d cells x %d slabs are too many", nc, sub);
     UDX_TRY(dev_alloc(&c->counts, nc));
     UDX_TRY(dev_alloc(&c->starts, nc + 1));
+    if (sub > 1) {
+        UDX_TRY(dev_alloc(&c->bcounts, (size_t)nc * sub));
+        UDX_TRY(dev_alloc(&c->bstarts, (size_t)nc * sub + 1));
+    } else {
+        c->bcounts = c->counts;
+        c->bstarts = c->starts;
+    }
     UDX_TRY(dev_alloc(&c->cellslot, maxp));
     UDX_TRY(dev_alloc(&c->order, maxp));
-    UDX_TRY(dev_alloc(&c->scan_tmp, scan_scratch_size(nc)));
+    UDX_TRY(dev_alloc(&c->scan_tmp, scan_scratch_size(nc * sub)));
     return UDX_OK;
 }
 
 void clist_free(ClistBufs *c) {
+    if (c->sub > 1) {
+        dev_free(c->bcounts);
+        dev_free(c->bstarts);
+    }
+    c->bcounts = c->bstarts = nullptr;
     dev_free(c->counts);
     dev_free(c->starts);
     dev_free(c->cellslot);
@@ -250,33 +275,36 @@ template <typename Pos>
 static int clist_sort_impl(hipStream_t s, ClistBufs *c, int n, Pos pos) {
     UDX_CHECK(n >= 0 && n <= c->maxp, UDX_ECAP, "clist: %d particles exceed capacity %d", n, c->maxp);
     const Box &b = c->box;
-    int nc = b.ncells();
-    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c->counts, nc, 0);
-    UDX_LAUNCHED();
+    UDX_TRY(clist_clear(s, c));
     if (n > 0) {
         hipLaunchKernelGGL(k_clist_count<Pos>, dim3(ceil_div(n, 256)), dim3(256), 0, s, pos, n, b.L[0],
-                           b.L[1], b.L[2], c->counts, c->cellslot);
+                           b.L[1], b.L[2], c->sub, c->bcounts, c->cellslot);
         UDX_LAUNCHED();
     }
     return clist_finish(s, c, n);
 }
 
 int clist_clear(hipStream_t s, ClistBufs *c) {
-    const int nc = c->box.ncells();
-    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c->counts, nc, 0);
+    const int nb = c->box.ncells() * c->sub;
+    hipLaunchKernelGGL(k_fill_int, dim3(ceil_div(nb, 256)), dim3(256), 0, s, c->bcounts, nb, 0);
     UDX_LAUNCHED();
     return UDX_OK;
 }
 
 int clist_finish(hipStream_t s, ClistBufs *c, int n) {
-    const int nc = c->box.ncells();
-    UDX_TRY(scan_exclusive(s, c->counts, nc, c->starts, c->scan_tmp));
+    const int nc = c->box.ncells(), nb = nc * c->sub;
+    UDX_TRY(scan_exclusive(s, c->bcounts, nb, c->bstarts, c->scan_tmp));
     if (n > 0) {
         hipLaunchKernelGGL(k_clist_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c->cellslot,
-                           c->starts, c->order);
+                           c->bstarts, c->order);
         UDX_LAUNCHED();
-        hipLaunchKernelGGL(k_clist_cellsort, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, c->starts,
+        hipLaunchKernelGGL(k_clist_cellsort, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, c->bstarts,
                            c->order);
+        UDX_LAUNCHED();
+    }
+    if (c->sub > 1) {
+        hipLaunchKernelGGL(k_cell_tables, dim3(ceil_div(nc + 1, 256)), dim3(256), 0, s, nc, c->sub, c->bstarts,
+                           c->starts, c->counts);
         UDX_LAUNCHED();
     }
     return UDX_OK;

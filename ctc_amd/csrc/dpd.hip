@@ -35,6 +35,8 @@ constexpr int DRAIN_STEPS = 2;  // pair steps per lane when a queue is full
 struct DpdK {
     const float4 *pos, *vel;
     const int *starts, *fss;
+    const int *bstarts;       // x-slab starts (sub per cell) or null: whole cells
+    int sub;
     int foff[27];
     int Lx, Ly, Lz, nmax, periodic;
     int i0;                   // first particle of the box; noise indices are relative to it
@@ -63,9 +65,9 @@ __device__ __forceinline__ UDX_GLOBAL T *gptr(T *p) {
 
 struct DpdP {
     const UDX_GLOBAL f4v *pos, *vel;
-    const UDX_GLOBAL int *starts, *fss;
+    const UDX_GLOBAL int *starts, *fss, *bstarts;
     const UDX_GLOBAL float *seeds;
-    int Lx, Ly, Lz, nmax, periodic, i0;
+    int Lx, Ly, Lz, nmax, periodic, i0, sub;
     DpdPar par;
     // x, y, z of a particle (w is not used)
     __device__ __forceinline__ float4 p(int j) const {
@@ -83,6 +85,7 @@ __device__ __forceinline__ DpdP dpd_view(const DpdK &A) {
                 (const UDX_GLOBAL f4v *)A.vel,
                 gptr(A.starts),
                 gptr(A.fss),
+                gptr(A.bstarts),
                 gptr(A.seeds),
                 A.Lx,
                 A.Ly,
@@ -90,6 +93,7 @@ __device__ __forceinline__ DpdP dpd_view(const DpdK &A) {
                 A.nmax,
                 A.periodic,
                 A.i0,
+                A.sub,
                 A.par};
 }
 
@@ -179,6 +183,50 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * per + (x < rem ? x : rem) + k;
 }
 
+// Candidate window along x.  With x-slab starts a row run [x0, x1] of cells
+// (wrap wx) is cut to the slabs that can hold a particle within the cutoff:
+// a candidate at y-z distance >= d from the particle has |dx| < sqrt(1 - d^2).
+// The bound is conservative for the float pair test: the y-z gaps g are
+// lowered and the half-width raised by WIN_MARGIN (above the rounding of
+// the particle's periodic shift for boxes up to 2048 cells) and 1 + 1e-5
+// stands for the cutoff, so every candidate the test would accept lies in
+// the window.  hw < 0: no particle of the row is within the cutoff.
+constexpr float WIN_MARGIN = 1.0f / 4096.0f;
+
+__device__ __forceinline__ float row_halfwidth(int dy, int dz, float fy, float fz) {
+    float gy = dy == 0 ? 0.f : (dy < 0 ? fy : 1.f - fy);
+    float gz = dz == 0 ? 0.f : (dz < 0 ? fz : 1.f - fz);
+    gy = fmaxf(gy - WIN_MARGIN, 0.f);
+    gz = fmaxf(gz - WIN_MARGIN, 0.f);
+    const float w2 = 1.00001f - gy * gy - gz * gz;
+    return w2 < 0.f ? -1.f : __builtin_sqrtf(w2) + WIN_MARGIN;
+}
+
+// candidates of the run [x0, x1] of the row starting at cell `base`, for the
+// particle at x = ax in the run's frame: index range [*j, *j + *n)
+__device__ __forceinline__ void run_range(const DpdP &P, int base, int x0, int x1, float ax, float hw, int *j,
+                                          int *n) {
+    if (!P.bstarts) {
+        *j = P.starts[base + x0];
+        *n = P.starts[base + x1 + 1] - *j;
+        return;
+    }
+    const int S = P.sub;
+    const double h = 0.5 * (double)P.Lx;
+    double lo = __builtin_floor(((double)ax - (double)hw + h) * (double)S);
+    double hi = __builtin_floor(((double)ax + (double)hw + h) * (double)S);
+    lo = fmax(lo, (double)(x0 * S));
+    hi = fmin(hi, (double)(x1 * S + S - 1));
+    if (!(hw >= 0.f) || lo > hi) {
+        *j = 0;
+        *n = 0;
+        return;
+    }
+    const int sb = base * S;
+    *j = P.bstarts[sb + (int)lo];
+    *n = P.bstarts[sb + (int)hi + 1] - *j;
+}
+
 // The force pass of one block of NT consecutive particles starting at i
 // = first + threadIdx.x; QC queued pairs per lane before a drain.
 template <int WR, bool EXACT, int NT, int QC, int SC>
@@ -196,6 +244,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
 
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
     int cx = 0, cy = 0, cz = 0, istart = 0;
+    float ry = 0.f, rz = 0.f;   // position inside the cell along y and z
     if (active) {
         pi = P.p(i);
         vi = P.v(i);
@@ -203,6 +252,8 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         cell_coord(pi.y, Ly, &cy);
         cell_coord(pi.z, Lz, &cz);
         istart = P.starts[cx + Lx * (cy + Ly * cz)];
+        ry = (float)((double)pi.y + 0.5 * (double)Ly - (double)cy);
+        rz = (float)((double)pi.z + 0.5 * (double)Lz - (double)cz);
     }
     const float seed_bulk = P.seeds[26];
     float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -286,40 +337,28 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         const int base = Lx * (yy + Ly * zz);
         int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
         if (rowok) {
-            if (xrun(0, cx, Lx, P.periodic, &x0, &x1, &wx0)) {
-                j0 = P.starts[base + x0];
-                n0 = P.starts[base + x1 + 1] - j0;
-            }
-            if (xrun(1, cx, Lx, P.periodic, &x0, &x1, &wx1)) {
-                j1 = P.starts[base + x0];
-                n1 = P.starts[base + x1 + 1] - j1;
-            }
+            const float hw = row_halfwidth(dy, dz, ry, rz);
+            if (xrun(0, cx, Lx, P.periodic, &x0, &x1, &wx0))
+                run_range(P, base, x0, x1, pi.x - (float)(wx0 * Lx), hw, &j0, &n0);
+            if (xrun(1, cx, Lx, P.periodic, &x0, &x1, &wx1))
+                run_range(P, base, x0, x1, pi.x - (float)(wx1 * Lx), hw, &j1, &n1);
         }
         const bool simple = !rowok || ((wy | wz | wx0) == 0 && n1 == 0);
         const bool all_simple = !wave_any(!simple);
+        // the union of the wave's candidate ranges of this row
         int lo = 0x7fffffff, hi = 0;
-        if (all_simple) {
-            // unwrapped runs of consecutive cell-ordered particles grow with
-            // the lane: the first and the last lane with a row bound them
-            const unsigned long long act = __ballot(rowok);
-            if (act) {
-                lo = __shfl(j0, __ffsll(act) - 1, 64);
-                hi = __shfl(j0 + n0, 63 - __clzll(act), 64);
-            }
-        } else {
-            if (rowok) {
-                lo = j0;
-                hi = j0 + n0;
-                if (n1 > 0) {
-                    lo = min(lo, j1);
-                    hi = max(hi, j1 + n1);
-                }
-            }
+        if (n0 > 0) {
+            lo = j0;
+            hi = j0 + n0;
+        }
+        if (n1 > 0) {
+            lo = min(lo, j1);
+            hi = max(hi, j1 + n1);
+        }
 #pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) {
-                lo = min(lo, __shfl_xor(lo, d, 64));
-                hi = max(hi, __shfl_xor(hi, d, 64));
-            }
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
         }
         const bool fits = hi > lo && hi - lo <= SC;
         const bool lds_row = fits && all_simple;
@@ -327,7 +366,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             __builtin_amdgcn_wave_barrier();
             for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = P.p(lo + k);
             __builtin_amdgcn_wave_barrier();
-            const float4 *sp = sb + (rowok ? j0 - lo : 0);
+            const float4 *sp = sb + (n0 > 0 ? j0 - lo : 0);
             for (int t = 0; wave_any(t < n0); t += 4) {
                 make_room(4);
                 const float4 c0 = sp[t], c1 = sp[t + 1], c2 = sp[t + 2], c3 = sp[t + 3];
@@ -482,6 +521,8 @@ DpdK make_k(const DpdLaunch &a) {
     k.pos = a.pos;
     k.vel = a.vel;
     k.starts = a.starts;
+    k.bstarts = a.bstarts;
+    k.sub = a.sub;
     k.fss = a.frag->fss;
     for (int f = 0; f < 27; f++) k.foff[f] = a.frag->off[f];
     k.Lx = a.box.L[0];
@@ -543,6 +584,8 @@ int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *star
 // mode UDX_MODE_EXACT selects correctly rounded sqrt/division in the pair force
 int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
     const bool exact = mode == UDX_MODE_EXACT;
+    UDX_CHECK(!a.bstarts || (a.box.L[0] <= 2048 && a.box.L[1] <= 2048 && a.box.L[2] <= 2048), UDX_EINVAL,
+              "dpd: x-slab windows need box sides of at most 2048 cells");
     UDX_CHECK(!a.periodic || (a.box.L[0] >= 3 && a.box.L[1] >= 3 && a.box.L[2] >= 3), UDX_EINVAL,
               "dpd: a periodic box needs at least 3 cells per side (got %d %d %d)", a.box.L[0], a.box.L[1],
               a.box.L[2]);
@@ -599,6 +642,8 @@ int dpd_batch_build(DpdBatch *b, const DpdLaunch *boxes, int nbox) {
         UDX_CHECK(!a.periodic || (a.box.L[0] >= 3 && a.box.L[1] >= 3 && a.box.L[2] >= 3), UDX_EINVAL,
                   "dpd: box %d: a periodic box needs at least 3 cells per side", k);
         UDX_CHECK(a.par.spow == boxes[0].par.spow, UDX_EINVAL, "dpd: the boxes of a batch share spow");
+        UDX_CHECK(!a.bstarts || (a.box.L[0] <= 2048 && a.box.L[1] <= 2048 && a.box.L[2] <= 2048), UDX_EINVAL,
+                  "dpd: box %d: x-slab windows need box sides of at most 2048 cells", k);
         ks[k] = make_k(a);
         for (int w = 0; w < 2; w++) bb[w][k + 1] = bb[w][k] + ceil_div(a.nmax, w == 0 ? 128 : 256);
     }

@@ -45,8 +45,9 @@ __device__ __forceinline__ int find_box_wave(const int *off, int nbox, int i) {
     return b;
 }
 
-// cell (global index) and provisional slot of each particle; consecutive
-// lanes in one cell take their slots with one atomic (as k_clist_count)
+// x slab (global index: cell * SOLVENT_SUB + slab) and provisional slot of
+// each particle; consecutive lanes in one slab take their slots with one
+// atomic (as k_clist_count)
 __global__ __launch_bounds__(256) void k_ens_count(int n, int nbox, const int *poff, const int *coff,
                                                    const int4 *boxL, const float4 *pos, int *counts,
                                                    int2 *cellslot) {
@@ -57,9 +58,9 @@ __global__ __launch_bounds__(256) void k_ens_count(int n, int nbox, const int *p
         const int b = find_box_wave(poff, nbox, i);
         const int4 L = boxL[b];
         const float4 r = pos[i];
-        int cx, cy, cz;
-        if (cell_coord(r.x, L.x, &cx) && cell_coord(r.y, L.y, &cy) && cell_coord(r.z, L.z, &cz))
-            cid = coff[b] + cx + L.x * (cy + L.y * cz);
+        int bx, cy, cz;
+        if (slab_coord(r.x, L.x, SOLVENT_SUB, &bx) && cell_coord(r.y, L.y, &cy) && cell_coord(r.z, L.z, &cz))
+            cid = coff[b] * SOLVENT_SUB + bx + L.x * SOLVENT_SUB * (cy + L.y * cz);
     }
     const int prev = __shfl_up(cid, 1, 64);
     const bool head = lane == 0 || prev != cid;
@@ -179,6 +180,8 @@ static int ens_build_batches(udx_ens *e) {
             a.pos = e->pos[par];
             a.vel = e->vel[par];
             a.starts = e->cl.starts + e->coff_h[b];
+            a.bstarts = e->cl.bstarts + (size_t)e->coff_h[b] * SOLVENT_SUB;
+            a.sub = SOLVENT_SUB;
             a.frag = &e->frag[b];
             a.box = Box{{e->box[b].L[0], e->box[b].L[1], e->box[b].L[2]}};
             a.nmax = e->box[b].n;
@@ -205,7 +208,7 @@ static int ens_step(udx_ens *e, int cur) {
     UDX_LAUNCHED();
     UDX_TRY(clist_clear(s, &e->cl));
     hipLaunchKernelGGL(k_ens_count, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, e->nbox, e->poff, e->coff,
-                       e->boxL, e->pos[a], e->cl.counts, e->cl.cellslot);
+                       e->boxL, e->pos[a], e->cl.bcounts, e->cl.cellslot);
     UDX_LAUNCHED();
     UDX_TRY(clist_finish(s, &e->cl, n));
     UDX_TRY(gather_f4(s, &e->cl, n, e->pos[a], e->vel[a], e->pos[b], e->vel[b]));
@@ -370,7 +373,7 @@ int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **out)
         if (!st) st = dev_alloc(&e->vel[k], (size_t)e->ntot);
     }
     if (!st) st = dev_alloc(&e->ff, (size_t)e->ntot);
-    if (!st) st = clist_alloc(&e->cl, Box{{e->ctot, 1, 1}}, e->ntot > 0 ? e->ntot : 1);
+    if (!st) st = clist_alloc(&e->cl, Box{{e->ctot, 1, 1}}, e->ntot > 0 ? e->ntot : 1, SOLVENT_SUB);
     if (!st) st = dev_alloc(&e->fcnt, (size_t)e->ftot);
     if (!st) st = dev_alloc(&e->fss, (size_t)e->ftot + 1);
     if (!st) st = dev_alloc(&e->fscan, (size_t)scan_scratch_size(e->ftot));

@@ -35,161 +35,165 @@ struct MPart {
     float r[3], v[3];
 };
 
-// ---- roots (src/mesh_bounce/dev/roots.h) ---------------------------------
-__device__ __forceinline__ bool valid_time(double dt, double t) { return t >= 0 && t <= dt; }
-
-__device__ bool quadratic(double a, double b, double c, double *h0, double *h1) {
-    const int sgnb = b > 0 ? 1 : -1;
-    const double D = b * b - 4 * a * c;
-    if (D < 0) return false;
-    *h0 = (-b - sgnb * sqrt(D)) / (2 * a);
-    *h1 = c / (a * *h0);
-    if (*h0 > *h1) {
-        const double t = *h0;
-        *h0 = *h1;
-        *h1 = t;
-    }
-    return true;
+// ---- crossing time ---------------------------------------------------------
+// The crossing test of src/mesh_bounce/dev/intersection.h:30-127, derived
+// here.  Over the step every vertex moves linearly, X(t) = X0 + t Xv, and so
+// does the particle, p(t) = p0 + t pv.  With the edges e1(t) = B(t) - A(t)
+// and e2(t) = C(t) - A(t) the plane normal N(t) = e1 x e2 is the quadratic
+//   N(t) = N0 + t N1 + t^2 N2,
+//   N0 = e1 x e2,  N1 = e1 x e2v + e1v x e2,  N2 = e1v x e2v,
+// and the particle's offset from A is g(t) = g0 + t gv.  The signed volume
+// phi(t) = g . N is the cubic k0 + k1 t + k2 t^2 + k3 t^3 with
+//   k0 = g0.N0,  k1 = gv.N0 + g0.N1,  k2 = gv.N1 + g0.N2,  k3 = gv.N2,
+// and the particle passes through the plane at the first root of phi in
+// [0, dt] where phi changes sign.  [0, dt] is cut at the turning points of
+// phi (roots of phi' by the cancellation-free quadratic formula); phi is
+// monotone on each piece, the first piece whose ends differ in sign holds
+// the root, and a Newton iteration kept inside that bracket (bisection when
+// a step leaves it) converges to it.  The coefficients are used as they are
+// (the reference drops a leading coefficient below 1e-8 and solves the
+// lower degree); a root where phi touches zero without a sign change is not
+// a crossing.
+__device__ __forceinline__ double cub_eval(const double k[4], double t) {
+    return ((k[3] * t + k[2]) * t + k[1]) * t + k[0];
+}
+__device__ __forceinline__ double cub_slope(const double k[4], double t) {
+    return (3 * k[3] * t + 2 * k[2]) * t + k[1];
 }
 
-// real roots of x^3 + a x^2 + b x + c in ascending order (GSL's method)
-__device__ int cubic(double a, double b, double c, double *x0, double *x1, double *x2) {
-    const double q = a * a - 3 * b;
-    const double r = 2 * a * a * a - 9 * a * b + 27 * c;
-    const double Q = q / 9;
-    const double R = r / 54;
-    const double Q3 = Q * Q * Q;
-    const double R2 = R * R;
-    const double CR2 = 729 * r * r;
-    const double CQ3 = 2916 * q * q * q;
-    if (R == 0 && Q == 0) {
-        *x0 = *x1 = *x2 = -a / 3;
-        return 3;
-    } else if (CR2 == CQ3) {
-        const double sqrtQ = sqrt(Q);
-        if (R > 0) {
-            *x0 = -2 * sqrtQ - a / 3;
-            *x1 = sqrtQ - a / 3;
-            *x2 = sqrtQ - a / 3;
-        } else {
-            *x0 = -sqrtQ - a / 3;
-            *x1 = -sqrtQ - a / 3;
-            *x2 = 2 * sqrtQ - a / 3;
-        }
-        return 3;
-    } else if (R2 < Q3) {
-        const double pi = 3.14159265358979323846;
-        const double sgnR = R >= 0 ? 1 : -1;
-        const double ratio = sgnR * sqrt(R2 / Q3);
-        const double theta = acos(ratio);
-        const double norm = -2 * sqrt(Q);
-        double t;
-        *x0 = norm * cos(theta / 3) - a / 3;
-        *x1 = norm * cos((theta + 2.0 * pi) / 3) - a / 3;
-        *x2 = norm * cos((theta - 2.0 * pi) / 3) - a / 3;
-        if (*x0 > *x1) { t = *x0; *x0 = *x1; *x1 = t; }
-        if (*x1 > *x2) {
-            t = *x1; *x1 = *x2; *x2 = t;
-            if (*x0 > *x1) { t = *x0; *x0 = *x1; *x1 = t; }
-        }
-        return 3;
+// turning points of the cubic strictly inside (0, T), ascending
+__device__ int cub_turning(const double k[4], double T, double out[2]) {
+    const double qa = 3 * k[3], qb = 2 * k[2], qc = k[1];
+    double r[2];
+    int m = 0, n = 0;
+    if (qa == 0) {
+        if (qb != 0) r[m++] = -qc / qb;
     } else {
-        const double sgnR = R >= 0 ? 1 : -1;
-        const double A = -sgnR * pow(fabs(R) + sqrt(R2 - Q3), 1.0 / 3.0);
-        const double B = Q / A;
-        *x0 = A + B - a / 3;
-        return 1;
+        const double disc = qb * qb - 4 * qa * qc;
+        if (disc > 0) {
+            const double w = -0.5 * (qb + (qb >= 0 ? sqrt(disc) : -sqrt(disc)));
+            r[m++] = w / qa;
+            if (w != 0) r[m++] = qc / w;
+        }
     }
+    if (m == 2 && r[1] < r[0]) {
+        const double t = r[0];
+        r[0] = r[1];
+        r[1] = t;
+    }
+    for (int i = 0; i < m; i++)
+        if (r[i] > 0 && r[i] < T) out[n++] = r[i];
+    return n;
 }
 
-// the first root of a t^3 + b t^2 + c t + d in [0, dt]
-// (src/mesh_bounce/dev/intersection.h:1-28)
-__device__ bool cubic_root(double dt, double a, double b, double c, double d, double *h) {
-    const double eps = 1e-8;
-    double h1, h2, h3;
-    if (fabs(a) > eps) {
-        b /= a;
-        c /= a;
-        d /= a;
-        const int nsol = cubic(b, c, d, &h1, &h2, &h3);
-        if (valid_time(dt, h1)) { *h = h1; return true; }
-        if (nsol > 1 && valid_time(dt, h2)) { *h = h2; return true; }
-        if (nsol > 2 && valid_time(dt, h3)) { *h = h3; return true; }
-    } else if (fabs(b) > eps) {
-        if (!quadratic(b, c, d, &h1, &h2)) return false;
-        if (valid_time(dt, h1)) { *h = h1; return true; }
-        if (valid_time(dt, h2)) { *h = h2; return true; }
-    } else if (fabs(c) > eps) {
-        h1 = -d / c;
-        if (valid_time(dt, h1)) { *h = h1; return true; }
+// the root in (lo, hi) of a cubic monotone there; flo = value at lo (the
+// value at hi has the other sign)
+__device__ double cub_bracketed(const double k[4], double lo, double hi, double flo) {
+    double t = 0.5 * (lo + hi);
+    for (int it = 0; it < 200; it++) {
+        const double f = cub_eval(k, t);
+        if (f == 0) return t;
+        if ((f < 0) == (flo < 0)) {
+            lo = t;
+            flo = f;
+        } else {
+            hi = t;
+        }
+        const double d = cub_slope(k, t);
+        double tn = d != 0 ? t - f / d : lo;
+        if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+        if (tn == t || !(hi - lo > 4e-16 * hi)) return tn;
+        t = tn;
+    }
+    return t;
+}
+
+// first sign-changing root of the cubic in [0, T]
+__device__ bool cub_first_root(const double k[4], double T, double *h) {
+    double s[4];
+    int m = 0;
+    s[m++] = 0;
+    m += cub_turning(k, T, s + 1);
+    s[m++] = T;
+    double fprev = cub_eval(k, s[0]);
+    if (fprev == 0) {
+        *h = s[0];
+        return true;
+    }
+    for (int i = 1; i < m; i++) {
+        const double f = cub_eval(k, s[i]);
+        if (f == 0) {
+            *h = s[i];
+            return true;
+        }
+        if ((f < 0) != (fprev < 0)) {
+            *h = cub_bracketed(k, s[i - 1], s[i], fprev);
+            return true;
+        }
+        fprev = f;
     }
     return false;
 }
 
-__device__ __forceinline__ double ddot(const double a[3], const double b[3]) {
+__device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-__device__ __forceinline__ void dcross(const double a[3], const double b[3], double c[3]) {
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3]) {
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// Fedosov's moving-triangle crossing (src/mesh_bounce/dev/intersection.h:30-127)
-__device__ int intersect(float dt_f, const MPart &A, const MPart &B, const MPart &C, const MPart &p0, float out[4]) {
-    const double dt = dt_f;
-    double a1[3], a2[3], at1[3], at2[3], n0[3], nt[3], ntt[3], dr0[3];
+// Triangle A B C and a particle P over [0, dt]: BB_* and (h, u, v, s) =
+// crossing time, barycentric coordinates of the crossing point, the side of
+// the plane the particle starts on (+1: phi(0) > 0)
+__device__ int intersect(float dt_f, const MPart &A, const MPart &B, const MPart &C, const MPart &P, float out[4]) {
+    const double T = dt_f;
+    double e1[3], e2[3], e1v[3], e2v[3], g0[3], gv[3];
     for (int c = 0; c < 3; c++) {
-        a1[c] = (double)(B.r[c] - A.r[c]);
-        a2[c] = (double)(C.r[c] - A.r[c]);
-        at1[c] = (double)(B.v[c] - A.v[c]);
-        at2[c] = (double)(C.v[c] - A.v[c]);
+        e1[c] = (double)(B.r[c] - A.r[c]);
+        e2[c] = (double)(C.r[c] - A.r[c]);
+        e1v[c] = (double)(B.v[c] - A.v[c]);
+        e2v[c] = (double)(C.v[c] - A.v[c]);
+        g0[c] = (double)(P.r[c] - A.r[c]);
+        gv[c] = (double)(P.v[c] - A.v[c]);
     }
-    dcross(a1, a2, n0);
-    dcross(at1, at2, ntt);
-    nt[0] = a1[1] * at2[2] - a1[2] * at2[1] + at1[1] * a2[2] - at1[2] * a2[1];
-    nt[1] = a1[2] * at2[0] - a1[0] * at2[2] + at1[2] * a2[0] - at1[0] * a2[2];
-    nt[2] = a1[0] * at2[1] - a1[1] * at2[0] + at1[0] * a2[1] - at1[1] * a2[0];
-    for (int c = 0; c < 3; c++) dr0[c] = (double)(p0.r[c] - A.r[c]);
-    {
-        double r1[3], s11[3], n1[3], dr1[3];
-        for (int c = 0; c < 3; c++) {
-            r1[c] = (double)p0.r[c] + dt * (double)p0.v[c];
-            s11[c] = (double)A.r[c] + dt * (double)A.v[c];
-            n1[c] = n0[c] + dt * (nt[c] + dt * ntt[c]);
-        }
-        for (int c = 0; c < 3; c++) dr1[c] = r1[c] - s11[c];
-        const double b0 = ddot(dr0, n0), b1 = ddot(dr1, n1);
-        out[3] = b0 > 0 ? 1.f : -1.f;
-        if (b0 * b1 > 0) return BB_NOCROSS;
-    }
-    double hl;
-    {
-        double dv[3];
-        for (int c = 0; c < 3; c++) dv[c] = (double)(p0.v[c] - A.v[c]);
-        const double a = ddot(ntt, dv);
-        const double b = ddot(ntt, dr0) + ddot(nt, dv);
-        const double c = ddot(nt, dr0) + ddot(n0, dv);
-        const double d = ddot(n0, dr0);
-        if (!cubic_root(dt, a, b, c, d, &hl)) return BB_HFAIL;
-    }
-    double rwl[3], g[3], a1_[3], a2_[3];
+    double N0[3], N1[3], N2[3], xa[3], xb[3];
+    cross3(e1, e2, N0);
+    cross3(e1, e2v, xa);
+    cross3(e1v, e2, xb);
+    for (int c = 0; c < 3; c++) N1[c] = xa[c] + xb[c];
+    cross3(e1v, e2v, N2);
+    // sides of the plane at the start and at the end of the step
+    double gT[3], NT[3];
     for (int c = 0; c < 3; c++) {
-        rwl[c] = (double)p0.r[c] + hl * (double)p0.v[c];
-        g[c] = rwl[c] - (double)A.r[c] - hl * (double)A.v[c];
-        a1_[c] = a1[c] + hl * at1[c];
-        a2_[c] = a2[c] + hl * at2[c];
+        gT[c] = g0[c] + T * gv[c];
+        NT[c] = N0[c] + T * (N1[c] + T * N2[c]);
     }
-    const double ga1 = ddot(g, a1_), ga2 = ddot(g, a2_);
-    const double a11 = ddot(a1_, a1_), a12 = ddot(a1_, a2_), a22 = ddot(a2_, a2_);
-    const double fac = 1.0 / (a11 * a22 - a12 * a12);
-    const double ul = (ga1 * a22 - ga2 * a12) * fac;
-    const double vl = (ga2 * a11 - ga1 * a12) * fac;
-    if ((ul < 0) || (vl < 0) || (ul + vl > 1)) return BB_WTRIANGLE;
-    out[0] = (float)hl;
-    out[1] = (float)ul;
-    out[2] = (float)vl;
+    const double phi0 = dot3(g0, N0), phiT = dot3(gT, NT);
+    out[3] = phi0 > 0 ? 1.f : -1.f;
+    if (phi0 * phiT > 0) return BB_NOCROSS;
+    const double k[4] = {phi0, dot3(gv, N0) + dot3(g0, N1), dot3(gv, N1) + dot3(g0, N2), dot3(gv, N2)};
+    double h;
+    if (!cub_first_root(k, T, &h)) return BB_HFAIL;
+    // the crossing point on the triangle at time h: g = u e1 + v e2 in the
+    // least-squares sense (normal equations of the two edge directions)
+    double g[3], f1[3], f2[3];
+    for (int c = 0; c < 3; c++) {
+        g[c] = g0[c] + h * gv[c];
+        f1[c] = e1[c] + h * e1v[c];
+        f2[c] = e2[c] + h * e2v[c];
+    }
+    const double m11 = dot3(f1, f1), m12 = dot3(f1, f2), m22 = dot3(f2, f2);
+    const double r1 = dot3(g, f1), r2 = dot3(g, f2);
+    const double inv = 1.0 / (m11 * m22 - m12 * m12);
+    const double u = (r1 * m22 - r2 * m12) * inv;
+    const double v = (r2 * m11 - r1 * m12) * inv;
+    if (u < 0 || v < 0 || u + v > 1) return BB_WTRIANGLE;
+    out[0] = (float)h;
+    out[1] = (float)u;
+    out[2] = (float)v;
     return BB_SUCCESS;
 }
 
@@ -444,7 +448,7 @@ __global__ __launch_bounds__(MBB_THREADS) void k_mbb_find(MeshView m, int max_gi
     if (!wave_any(active)) return;   // whole waves only: the queue below is wave-wide
     MPart A, B, C;
     const double ddt = dt;
-    double n0[3] = {0, 0, 0}, n1[3] = {0, 0, 0}, s11[3] = {0, 0, 0};
+    double N0[3] = {0, 0, 0}, NT[3] = {0, 0, 0};
     int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
     if (active) {
         fetch_tri(m, dt, gid, &A, &B, &C);
@@ -462,24 +466,21 @@ __global__ __launch_bounds__(MBB_THREADS) void k_mbb_find(MeshView m, int max_gi
             lo[c] = cell_of(L[c], mn - tol);
             hi[c] = cell_of(L[c], mx + tol);
         }
-        // the plane terms that depend on the triangle only (the operations
-        // of intersect(), so the same decisions)
-        double a1[3], a2[3], at1[3], at2[3], nt[3], ntt[3];
+        // the plane normal at the start and at the end of the step, with the
+        // operations of intersect(), so the same decisions
+        double e1[3], e2[3], e1v[3], e2v[3], N1[3], N2[3], xa[3], xb[3];
         for (int c = 0; c < 3; c++) {
-            a1[c] = (double)(B.r[c] - A.r[c]);
-            a2[c] = (double)(C.r[c] - A.r[c]);
-            at1[c] = (double)(B.v[c] - A.v[c]);
-            at2[c] = (double)(C.v[c] - A.v[c]);
+            e1[c] = (double)(B.r[c] - A.r[c]);
+            e2[c] = (double)(C.r[c] - A.r[c]);
+            e1v[c] = (double)(B.v[c] - A.v[c]);
+            e2v[c] = (double)(C.v[c] - A.v[c]);
         }
-        dcross(a1, a2, n0);
-        dcross(at1, at2, ntt);
-        nt[0] = a1[1] * at2[2] - a1[2] * at2[1] + at1[1] * a2[2] - at1[2] * a2[1];
-        nt[1] = a1[2] * at2[0] - a1[0] * at2[2] + at1[2] * a2[0] - at1[0] * a2[2];
-        nt[2] = a1[0] * at2[1] - a1[1] * at2[0] + at1[0] * a2[1] - at1[1] * a2[0];
-        for (int c = 0; c < 3; c++) {
-            s11[c] = (double)A.r[c] + ddt * (double)A.v[c];
-            n1[c] = n0[c] + ddt * (nt[c] + ddt * ntt[c]);
-        }
+        cross3(e1, e2, N0);
+        cross3(e1, e2v, xa);
+        cross3(e1v, e2, xb);
+        for (int c = 0; c < 3; c++) N1[c] = xa[c] + xb[c];
+        cross3(e1v, e2v, N2);
+        for (int c = 0; c < 3; c++) NT[c] = N0[c] + ddt * (N1[c] + ddt * N2[c]);
     }
     const int ny = active ? hi[1] - lo[1] + 1 : 0, nrows = active ? ny * (hi[2] - lo[2] + 1) : 0;
     int fill = 0;   // the same in every lane of the wave
@@ -520,12 +521,12 @@ __global__ __launch_bounds__(MBB_THREADS) void k_mbb_find(MeshView m, int max_gi
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 if (c0 + u >= len) continue;
-                double dr0[3], dr1[3];
+                double g0[3], gT[3];
                 for (int c = 0; c < 3; c++) {
-                    dr0[c] = (double)(p[u].r[c] - A.r[c]);
-                    dr1[c] = ((double)p[u].r[c] + ddt * (double)p[u].v[c]) - s11[c];
+                    g0[c] = (double)(p[u].r[c] - A.r[c]);
+                    gT[c] = g0[c] + ddt * (double)(p[u].v[c] - A.v[c]);
                 }
-                if (!(ddot(dr0, n0) * ddot(dr1, n1) > 0)) pass |= 1u << u;   // not BB_NOCROSS
+                if (!(dot3(g0, N0) * dot3(gT, NT) > 0)) pass |= 1u << u;   // not BB_NOCROSS
             }
             unsigned long long bu[4];
             int tot = 0;

@@ -115,28 +115,49 @@ __device__ __forceinline__ bool cell_coord(float r, int L, int *c) {
     return true;
 }
 
+// x slab floor((r + L/2) sub), exact in double (sub a power of two); false
+// outside [0, L sub).  The slab's cell is slab / sub.
+__device__ __forceinline__ bool slab_coord(float r, int L, int sub, int *c) {
+    double v = __builtin_floor(((double)r + 0.5 * (double)L) * (double)sub);
+    if (!(v >= 0.0 && v < (double)L * (double)sub)) return false;
+    *c = (int)v;
+    return true;
+}
+
 // ---------------------------------------------------------------- DPD pair
+// Correctly rounded square root of a positive normal float (or 0): the
+// hardware root (within 1 ulp) moved to a neighbour when an fma residual
+// shows that neighbour is nearer.  The compiler's IEEE square root does the
+// same after scaling denormal inputs, which the pair force never has.  Every
+// x in [2^-24, 1] is checked on the host against each value a 1-ulp root
+// can give (the oracle's orc_verify_sqrt_rn).
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1), su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, x), ru = __builtin_fmaf(-su, s, x);
+    const float t = rd <= 0.f ? sd : s;
+    return ru > 0.f ? su : t;
+}
+
 // w_R = (1 - r)^spow for the exponents the kernels specialise
 enum WrKind { WR_QUARTER = 0, WR_HALF = 1, WR_ONE = 2, WR_POW = 3 };
 
 template <int WR, bool EXACT>
 __device__ __forceinline__ float wr_pow(float rm, float spow) {
     if (WR == WR_ONE) return rm;
-    if (WR == WR_HALF) return EXACT ? __builtin_sqrtf(rm) : __builtin_amdgcn_sqrtf(rm);
-    if (WR == WR_QUARTER)
-        return EXACT ? __builtin_sqrtf(__builtin_sqrtf(rm))
-                     : __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(rm));
+    if (WR == WR_HALF) return sqrt_rn(rm);
+    if (WR == WR_QUARTER) return sqrt_rn(sqrt_rn(rm));
     return __builtin_powf(rm, spow);
 }
 
 struct DpdPar { float a, g, s, spow; };
 
 // DPD force on a from b given dr = ra - rb, dv = va - vb and r2 = |dr|^2 < 1
-// (src/pair/dev.h:12-47,110-139).  r is always the correctly rounded square
-// root: w_R = (1 - r)^spow has an unbounded slope at the cutoff, so 1 - r
-// must not carry approximation error.  EXACT also rounds 1/r and the
-// w_R roots correctly (bitwise equal to the CPU restatement); otherwise the
-// hardware reciprocal and square root (1 ulp, relative error only) are used.
+// (src/pair/dev.h:12-47,110-139).  r, 1/r and the w_R roots are correctly
+// rounded, so the force equals the CPU restatement bit for bit (the
+// reference's fast-math r2 rsqrtf(r2) is 1e-3 wrong in w_R near the cutoff,
+// where w_R = (1 - r)^spow has an unbounded slope).  The template argument
+// EXACT is kept for the generic exponent (powf).
 template <int WR, bool EXACT>
 __device__ __forceinline__ float3 dpd_pair(const DpdPar &p, float dx, float dy, float dz, float r2,
                                            float dvx, float dvy, float dvz, float rnd) {
@@ -144,8 +165,8 @@ __device__ __forceinline__ float3 dpd_pair(const DpdPar &p, float dx, float dy, 
     if (r2 < 1e-6f) {
         ex = ey = ez = r = 0.0f;
     } else {
-        r = __builtin_sqrtf(r2);
-        float invr = EXACT ? 1.0f / r : __builtin_amdgcn_rcpf(r);
+        r = sqrt_rn(r2);
+        const float invr = 1.0f / r;
         ex = dx * invr; ey = dy * invr; ez = dz * invr;
     }
     float ev = __builtin_fmaf(ez, dvz, __builtin_fmaf(ex, dvx, ey * dvy));

@@ -69,23 +69,34 @@ int scan_scratch_size(int n);
 int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch);
 
 // ------------------------------------------------------------------ cell list
+// Particles are sorted by cell and, inside a cell, by x slab: every cell is
+// cut into `sub` slabs of width 1/sub along x (slab id = cell * sub +
+// floor((x + L/2) sub) mod sub), ties keeping the input order.  The slab
+// starts let a force kernel read only the part of a row of cells that lies
+// within the cutoff along x.
 struct ClistBufs {
     Box box;
     int maxp;
+    int sub;         // x slabs per cell (1: plain cells)
     int *counts;     // ncells
     int *starts;     // ncells + 1 (starts[ncells] = number kept)
-    int2 *cellslot;  // maxp: (cell, provisional slot)
+    int *bcounts;    // ncells * sub (the same array as counts when sub == 1)
+    int *bstarts;    // ncells * sub + 1 (the same array as starts when sub == 1)
+    int2 *cellslot;  // maxp: (slab, provisional slot)
     int *order;      // maxp: input index of each output slot
     int *scan_tmp;
 };
-int clist_alloc(ClistBufs *c, const Box &b, int maxp);
+// x slabs per cell of the solvent's cell lists
+constexpr int SOLVENT_SUB = 4;
+int clist_alloc(ClistBufs *c, const Box &b, int maxp, int sub = 1);
 void clist_free(ClistBufs *c);
 // pos4 (input, n particles): counting sort into cell order; order[] and
 // starts[] filled; the number kept is starts[ncells] on the device
 int clist_sort(hipStream_t s, ClistBufs *c, int n, const float4 *pos4);
 int clist_sort_aos(hipStream_t s, ClistBufs *c, int n, const float *pp);
-// the two halves of a sort for callers that fill counts[] and cellslot[]
-// themselves: zero the counts; scan, scatter and the stable in-cell order
+// the two halves of a sort for callers that fill bcounts[] and cellslot[]
+// themselves: zero the slab counts; scan, scatter, the stable in-slab order
+// and the cell tables
 int clist_clear(hipStream_t s, ClistBufs *c);
 int clist_finish(hipStream_t s, ClistBufs *c, int n);
 // out[k] = in[order[k]] for k < starts[ncells]; n bounds the launch
@@ -111,6 +122,8 @@ int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *star
 struct DpdLaunch {
     const float4 *pos, *vel;
     const int *starts;      // ncells + 1, starts[ncells] = n
+    const int *bstarts = nullptr;   // x-slab starts (ncells * sub + 1) of a slab-sorted list, or null
+    int sub = 1;
     const FragTables *frag;
     Box box;
     int nmax;               // launch bound (>= particles)

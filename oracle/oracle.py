@@ -70,6 +70,8 @@ def _declare(L):
         "orc_frag_tables": (None, [I32P, I32P, I32P, I32P]),
         "orc_gen_random": (None, [C.c_int] * 5 + [F32P]),
         "orc_clist_build": (C.c_int, [I32P, C.c_int, F32P, I32P, I32P, F32P, I32P]),
+        "orc_clist_build_sub": (C.c_int, [I32P, C.c_int, C.c_int, F32P, I32P, I32P, I32P, F32P, I32P]),
+        "orc_slab_of": (C.c_int, [C.c_int, C.c_int, C.c_float]),
         "orc_dpd_sigma": (C.c_float, [C.c_float] * 3),
         "orc_pair_force": (C.c_int, [_p(Dpd), F32P, F32P, C.c_float, F32P]),
         "orc_dpd_forces": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
@@ -175,18 +177,29 @@ def gen_random(X, Y, Z, n, roundtrip=True):
     return pp
 
 
-def clist_build(L, pp):
+# x slabs per cell of the solvent's cell lists (ORC_SOLVENT_SUB)
+SOLVENT_SUB = 4
+
+
+def clist_build(L, pp, sub=SOLVENT_SUB, slabs=False):
+    """Counting sort by cell, then x slab (sub per cell), then input index.
+    Returns (starts, counts, cell-ordered pp, order) and, with slabs=True,
+    the slab starts (ncells * sub + 1) as a fifth item.  Solvent cell lists
+    use sub = SOLVENT_SUB; the wall particles' margin grid and the FSI
+    instances use sub = 1."""
     L = _L(L)
     pp = _f32(pp).reshape(-1, 6)
     n = pp.shape[0]
     nc = int(L[0] * L[1] * L[2])
     starts = np.empty(nc + 1, np.int32)
     counts = np.empty(nc, np.int32)
+    bstarts = np.empty(nc * sub + 1, np.int32)
     out = np.empty_like(pp)
     order = np.empty(max(n, 1), np.int32)
-    nout = lib().orc_clist_build(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(starts, I32P),
-                                 _ptr(counts, I32P), _ptr(out, F32P), _ptr(order, I32P))
-    return starts, counts, out[:nout].copy(), order[:nout].copy()
+    nout = lib().orc_clist_build_sub(_ptr(L, I32P), sub, n, _ptr(pp, F32P), _ptr(starts, I32P),
+                                     _ptr(counts, I32P), _ptr(bstarts, I32P), _ptr(out, F32P), _ptr(order, I32P))
+    res = (starts, counts, out[:nout].copy(), order[:nout].copy())
+    return res + (bstarts,) if slabs else res
 
 
 def dpd_sigma(kBT, g, dt):

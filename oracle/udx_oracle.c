@@ -283,29 +283,58 @@ int orc_cell_of(const int L[3], const float r[3], int c[3]) {
     return c[0] + L[0] * (c[1] + L[1] * c[2]);
 }
 
-int orc_clist_build(const int L[3], int n, const float *pp_in, int *starts, int *counts,
-                    float *pp_out, int *order) {
-    int nc = L[0] * L[1] * L[2];
-    int *cid = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    memset(counts, 0, sizeof(int) * (size_t)nc);
+int orc_slab_of(int L, int sub, float x) {
+    double v = floor(((double)x + 0.5 * (double)L) * (double)sub);
+    if (!(v >= 0.0 && v < (double)L * sub)) return -1;
+    return (int)v;
+}
+
+/* Counting sort by slab id (cell * sub + slab): the order of the build's
+ * cell lists.  The reference's in-cell order comes from atomicAdd
+ * (src/clist/dev/main.h:64-65) and is not reproducible; here it is defined
+ * by the x slab, then by input index. */
+int orc_clist_build_sub(const int L[3], int sub, int n, const float *pp_in, int *starts, int *counts,
+                        int *bstarts, float *pp_out, int *order) {
+    int nc = L[0] * L[1] * L[2], nb = nc * sub;
+    int *bid = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *bcnt = (int *)calloc((size_t)nb + 1, sizeof(int));
+    int *bst = (int *)malloc(sizeof(int) * ((size_t)nb + 1));
     for (int i = 0; i < n; i++) {
         int c[3];
-        cid[i] = orc_cell_of(L, pp_in + 6 * i, c);
-        if (cid[i] >= 0) counts[cid[i]]++;
+        const float *r = pp_in + 6 * i;
+        int cid = orc_cell_of(L, r, c);
+        bid[i] = -1;
+        if (cid < 0) continue;
+        int sx = orc_slab_of(L[0], sub, r[0]);
+        bid[i] = cid * sub + (sx - c[0] * sub);
+        bcnt[bid[i]]++;
     }
     int acc = 0;
-    for (int c = 0; c < nc; c++) { starts[c] = acc; acc += counts[c]; }
+    for (int b = 0; b < nb; b++) { bst[b] = acc; acc += bcnt[b]; }
+    bst[nb] = acc;
+    for (int c = 0; c < nc; c++) {
+        starts[c] = bst[c * sub];
+        counts[c] = bst[(c + 1) * sub] - bst[c * sub];
+    }
     starts[nc] = acc;
-    int *fill = (int *)calloc((size_t)nc, sizeof(int));
+    if (bstarts) memcpy(bstarts, bst, sizeof(int) * ((size_t)nb + 1));
+    int *fill = (int *)calloc((size_t)nb, sizeof(int));
     for (int i = 0; i < n; i++) {
-        if (cid[i] < 0) continue;
-        int k = starts[cid[i]] + fill[cid[i]]++;
+        if (bid[i] < 0) continue;
+        int k = bst[bid[i]] + fill[bid[i]]++;
         if (order) order[k] = i;
         if (pp_out) memcpy(pp_out + 6 * k, pp_in + 6 * i, 6 * sizeof(float));
     }
     free(fill);
-    free(cid);
+    free(bst);
+    free(bcnt);
+    free(bid);
     return acc;
+}
+
+int orc_clist_build(const int L[3], int n, const float *pp_in, int *starts, int *counts,
+                    float *pp_out, int *order) {
+    return orc_clist_build_sub(L, 1, n, pp_in, starts, counts, NULL, pp_out, order);
 }
 
 /* ======================================================================== */
@@ -514,10 +543,41 @@ void orc_diag(int n, const float *pp, double out[5]) {
 long orc_flu_step(const int L[3], int n, float *pp, float *scratch_pp, int *starts, int *counts,
                   float *ff, orc_dpd p, const float seeds[27], float dt, float mass,
                   const float accel[3], int nthreads) {
-    int nout = orc_clist_build(L, n, pp, starts, counts, scratch_pp, NULL);
+    int nout = orc_clist_build_sub(L, ORC_SOLVENT_SUB, n, pp, starts, counts, NULL, scratch_pp, NULL);
     memcpy(pp, scratch_pp, sizeof(float) * 6 * (size_t)nout);
     memset(ff, 0, sizeof(float) * 3 * (size_t)nout);
     long np = orc_dpd_forces(L, nout, pp, starts, counts, p, seeds, 1, nthreads, ff);
     orc_integrate(L, nout, dt, mass, accel, ff, pp);
     return np / 2;
+}
+
+/* ======================================================================== */
+/* Exhaustive check of the kernels' correctly rounded square root           */
+/* ======================================================================== */
+static float f_bits(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t bits_f(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+static float dev_sqrt_fix(float x, float s) {
+    const float sd = f_bits(bits_f(s) - 1u), su = f_bits(bits_f(s) + 1u);
+    const float rd = fmaf(-sd, s, x), ru = fmaf(-su, s, x);
+    const float t = rd <= 0.f ? sd : s;
+    return ru > 0.f ? su : t;
+}
+
+long orc_verify_sqrt_rn(float lo, float hi, int nthreads, int correct) {
+    const uint32_t a = bits_f(lo), b = bits_f(hi);
+    long bad = 0;
+    (void)nthreads;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+    #pragma omp parallel for schedule(static, 1 << 16) reduction(+:bad) num_threads(nthreads)
+#endif
+    for (long u = a; u <= (long)b; u++) {
+        const float x = f_bits((uint32_t)u), want = sqrtf(x);
+        for (int d = -1; d <= 1; d++) {
+            const float s = f_bits(bits_f(want) + (uint32_t)d);
+            if ((correct ? dev_sqrt_fix(x, s) : s) != want) bad++;
+        }
+    }
+    return bad;
 }

@@ -65,7 +65,17 @@ void orc_gen_random(int X, int Y, int Z, int n, int roundtrip, float *pp);
  * [-L/2, L/2) are dropped.  starts has ncells+1 entries.  Returns nout. */
 int orc_clist_build(const int L[3], int n, const float *pp_in, int *starts, int *counts,
                     float *pp_out, int *order);
+/* The same with every cell cut into `sub` slabs along x (sub = 1, 2, 4 or
+ * 8): particles are ordered by cell, then by slab
+ * floor((x + L/2) sub) mod sub, ties keep the input order.  bstarts (may be
+ * NULL) receives the slab starts, ncells * sub + 1 entries, slab s of cell c
+ * at c * sub + s.  The solvent's cell lists use sub = SOLVENT_SUB. */
+#define ORC_SOLVENT_SUB 4
+int orc_clist_build_sub(const int L[3], int sub, int n, const float *pp_in, int *starts, int *counts,
+                        int *bstarts, float *pp_out, int *order);
 int orc_cell_of(const int L[3], const float r[3], int c[3]);
+/* slab index floor((x + L/2) sub) in [0, L sub), exact in double; -1 outside */
+int orc_slab_of(int L, int sub, float x);
 
 /* ---- DPD pair force: src/pair/dev.h:12-47,110-139 ------------------------ */
 typedef struct { float a, g, s, spow; } orc_dpd;
@@ -239,6 +249,18 @@ int   orc_cnt_pair(const orc_dpd_lj *p, const float *a, const float *b, float rn
 int   orc_cnt_images(const int L[3], int n, const float *pp, float *img, int *img_v);
 long  orc_cnt_forces(const int L[3], int n, const float *pp, orc_dpd_lj p, float seed_bulk, float seed_halo,
                      float *ff);
+
+/* ---- correctly rounded square root from the 1-ulp hardware root --------
+ * The GPU kernels take the hardware square root s (within 1 ulp of the
+ * exact value) and correct it with two fma residuals:
+ *   s- = s - 1 ulp, s+ = s + 1 ulp;
+ *   t = (x - s- s <= 0) ? s- : s;  result = (x - s+ s > 0) ? s+ : t
+ * This checks every float x in [lo, hi] with every value the hardware may
+ * give (the correctly rounded root and its two neighbours) against the
+ * host's correctly rounded sqrtf.  Returns the number of failures.
+ * correct = 0 checks the uncorrected hardware values instead (a control:
+ * two failures per float). */
+long orc_verify_sqrt_rn(float lo, float hi, int nthreads, int correct);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,8 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
 # Production-kernel force tolerance.  The kernel uses the hardware reciprocal
 # and square roots (about 1 ulp each) where the oracle rounds correctly;
 # summation order is identical.  Per particle:
-#   |f_gpu - f_oracle| <= FTOL * max(|f_oracle|, 0.1 * F_rms)
+#   |f_gpu - f_oracle| <= FTOL * max(|f_oracle|, 1e-3 * F_rms)
+# (the per-particle metric of the contract, F_rms the RMS force magnitude)
 FTOL = 1e-5
 
 
@@ -173,7 +174,7 @@ def test_forces_fast_mode_tolerance(ctx, case):
     assert np_gpu == np_ref // 2           # cutoff decisions are exact arithmetic
     mag = np.linalg.norm(f_ref.astype(np.float64), axis=1)
     frms = np.sqrt((mag ** 2).mean())
-    err = np.linalg.norm(f_gpu.astype(np.float64) - f_ref, axis=1) / np.maximum(mag, 0.1 * frms)
+    err = np.linalg.norm(f_gpu.astype(np.float64) - f_ref, axis=1) / np.maximum(mag, 1e-3 * frms)
     say("  max rel err", err.max(), "F_rms", frms)
     assert err.max() <= FTOL
 

@@ -183,3 +183,65 @@ def test_tiny_boxes_equal_single_boxes(ctx):
         total += p
         assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), want.view(np.uint32)), "box %d" % b
     assert pairs == total > 0
+
+
+def c5_boxes(seed=2024, nbox=64, dt=1e-3):
+    """the C5 workload: 64 periodic boxes, edges uniform in {8..48}, density
+    in {3, 5, 10}, gamma in [4, 10], kBT in [0.04, 1], a = 25"""
+    rng = np.random.default_rng(seed)
+    boxes, pps = [], []
+    for b in range(nbox):
+        L = tuple(int(x) for x in rng.integers(8, 49, 3))
+        rho = int(rng.choice([3, 5, 10]))
+        g, kBT = float(rng.uniform(4, 10)), float(rng.uniform(0.04, 1.0))
+        n = rho * L[0] * L[1] * L[2]
+        pp = np.zeros((n, 6), np.float32)
+        pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+        pp[:, 3:] = rng.normal(0, np.sqrt(kBT), (n, 3)).astype(np.float32)
+        par = (25.0, g, O.dpd_sigma(kBT, g, dt), 0.25)
+        boxes.append(dict(L=L, n=n, params=U.PairDpd(*par), dt=dt, mass=1.0, accel=(0.0, 0.0, 0.0), par=par))
+        pps.append(pp)
+    return boxes, pps
+
+
+def test_c5_workload_batched_equals_per_box(ctx):
+    """C5 at its size (64 boxes, about 1.4e7 particles): 4 production steps
+    of the batched ensemble equal 4 steps of each box on its own, bit for
+    bit, for every tile variant; equal pair counts."""
+    boxes, pps = c5_boxes()
+    nbox, nsteps = len(boxes), 4
+    seeds = U.ens_trunk_seeds(nbox, nsteps)
+    ntot = sum(b["n"] for b in boxes)
+    say("C5: %d boxes, %d particles, edges %d..%d" % (nbox, ntot, min(min(b["L"]) for b in boxes),
+                                                      max(max(b["L"]) for b in boxes)))
+    off = np.cumsum([0] + [b["n"] for b in boxes])
+    single, total = [], 0
+    for b in range(nbox):
+        want, p = run_single(ctx, boxes[b], pps[b], seeds[:, b], False)
+        single.append(want)
+        total += p
+    for v in range(6):
+        say("C5: tile variant %d" % v)
+        got, pairs = run_ensemble(ctx, boxes, pps, nsteps, seeds, False, graph=(v % 2 == 0), variant=v)
+        assert pairs == total, "variant %d" % v
+        for b in range(nbox):
+            assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), single[b].view(np.uint32)), \
+                "variant %d box %d" % (v, b)
+
+
+def test_c5_workload_exact_equals_oracle(ctx):
+    """C5 in exact mode, 2 steps: a box with an edge of at least 40 and a box
+    with an odd edge equal the oracle bit for bit."""
+    boxes, pps = c5_boxes()
+    nbox, nsteps = len(boxes), 2
+    seeds = U.ens_trunk_seeds(nbox, nsteps)
+    big = next(b for b in range(nbox) if max(boxes[b]["L"]) >= 40)
+    odd = next(b for b in range(nbox) if b != big and any(x % 2 for x in boxes[b]["L"]))
+    say("C5 exact: boxes %d %s and %d %s" % (big, boxes[big]["L"], odd, boxes[odd]["L"]))
+    got, _ = run_ensemble(ctx, boxes, pps, nsteps, seeds, True, graph=True)
+    off = np.cumsum([0] + [b["n"] for b in boxes])
+    for b in (big, odd):
+        ref = pps[b].copy()
+        for k in range(nsteps):
+            ref, _ = O.flu_step(boxes[b]["L"], ref, boxes[b]["par"], seeds[k, b], boxes[b]["dt"], nthreads=16)
+        assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), ref.view(np.uint32)), "box %d" % b

@@ -6,8 +6,10 @@
 - C2 (wall-bounded 96^3, density 10, plane channel |z| - 40 from a 128^3
   field): the frozen walls of 8,847,360 particles and one exact-mode step
   (cell list, solvent DPD, wall DPD, body force, kick-drift, bounce, wrap)
-  equal the oracle bit for bit; then 20 production-mode steps leave no
-  solvent particle inside the walls.
+  equal the oracle bit for bit (state and forces); the production force
+  kernel on the same state is within the contract metric
+  |df| / max(|f|, 1e-3 F_rms) <= 1e-5 per particle with equal pair counts;
+  then 20 production-mode steps leave no solvent particle inside the walls.
 - C3 (64^3 suspension, 567 cells of 498 vertices): one exact-mode step of
   solvent + membranes + FSI; the solvent equals the oracle bit for bit and the
   vertices agree within 1e-6 (the membrane force's pow/sqrt ulps).
@@ -109,7 +111,7 @@ def test_c2_full_size_step_bit_exact(ctx):
     ref = W.Sdf(L, ext, N, D.ravel())
     keep, frozen = ref.freeze(pp)
     allw = W.wall_images(L, frozen)
-    wst, _, wpp, _ = O.clist_build(W.margin_box(L), allw)
+    wst, _, wpp, _ = O.clist_build(W.margin_box(L), allw, sub=1)
     assert np.array_equal(bits(solvent), bits(keep))
     assert np.array_equal(bits(host(w.particles())), bits(wpp))
     assert np.array_equal(host(w.starts()), wst)
@@ -118,20 +120,38 @@ def test_c2_full_size_step_bit_exact(ctx):
     accel = (0.01, 0.0, 0.0)
     seeds = O.seeds_table(1)
     seeds28 = np.concatenate([seeds, O.kiss_fill(42, 42, 42, 42, 1)[:, None]], axis=1).astype(np.float32)
-    flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*params), dt, 1.0, accel, exact=True)
     sd = dev(solvent)
-    flu.set_pp(sd)
-    flu.set_wall(w)
-    flu.run(1, seeds28)
-    out = torch.zeros_like(sd)
-    m = flu.get_pp(out)
-    got = host(out)[:m]
-    gp, gw = flu.pairs(), flu.wall_pairs()
-    flu.close()
-    want, npair, nw = W.flu_wall_step(L, keep, params, seeds28[0], dt, ref, wpp, wst, accel=accel,
-                                      nthreads=THREADS)
-    assert (gp, gw) == (npair, nw) and nw > 0
+    gpu = {}
+    for mode in (U.MODE_EXACT, U.MODE_FAST):
+        flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*params), dt, 1.0, accel, mode=mode)
+        flu.set_pp(sd)
+        flu.set_wall(w)
+        flu.run(1, seeds28)
+        out = torch.zeros_like(sd)
+        m = flu.get_pp(out)
+        ff = torch.zeros((len(solvent), 3), dtype=torch.float32, device="cuda")
+        flu.get_ff(ff)
+        gpu[mode] = (host(out)[:m], host(ff)[:m], flu.pairs(), flu.wall_pairs())
+        flu.close()
+    # the oracle step, keeping its forces (solvent DPD + wall DPD)
+    starts, counts, srt, _ = O.clist_build(L, keep)
+    ff, npair = O.dpd_forces(L, srt, starts, counts, params, seeds28[0, :27], nthreads=THREADS)
+    ff, nw = W.wall_forces(L, srt, ref, wpp, wst, params, seeds28[0, 27], ff)
+    want = W.integrate_wall(L, srt, ff, dt, ref, 1.0, accel)
+    got, gff, gp, gw = gpu[U.MODE_EXACT]
+    assert (gp, gw) == (npair // 2, nw) and nw > 0
+    assert np.array_equal(bits(gff), bits(ff))
     assert np.array_equal(bits(got), bits(want))
+    # the production kernel (the one the benchmark times) on the same state:
+    # per particle |df| / max(|f|, 1e-3 F_rms) <= 1e-5, equal pair counts
+    _, fff, fp, fw = gpu[U.MODE_FAST]
+    assert (fp, fw) == (npair // 2, nw)
+    mag = np.linalg.norm(ff.astype(np.float64), axis=1)
+    frms = np.sqrt((mag ** 2).mean())
+    err = np.linalg.norm(fff.astype(np.float64) - ff, axis=1) / np.maximum(mag, 1e-3 * frms)
+    say("C2 production kernel: max rel err %.3g (F_rms %.4g, min |f| %.3g, %d particles)"
+        % (err.max(), frms, mag.min(), len(mag)))
+    assert err.max() <= 1e-5
     say("C2: 20 production steps")
     flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*params), dt, 1.0, accel)
     flu.set_pp(sd)

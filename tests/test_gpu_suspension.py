@@ -211,7 +211,7 @@ def test_suspension_step_between_walls(ctx, lam, bounce, nsteps):
     w = U.Wall(ctx, q, ppd, exact=True)
     solvent = host(ppd)[:w.n_solvent].copy()
     keep, frozen = ref.freeze(pp)
-    wst, _, wpp, _ = O.clist_build(W.margin_box(L), W.wall_images(L, frozen))
+    wst, _, wpp, _ = O.clist_build(W.margin_box(L), W.wall_images(L, frozen), sub=1)
     tmpl = np.zeros((m.nv, 6), np.float32)
     tmpl[:, :3] = m.rr
     A0, V0 = R.area_volume(m, tmpl, 1)
@@ -267,5 +267,109 @@ def test_suspension_step_between_walls(ctx, lam, bounce, nsteps):
     assert np.all(ref.eval(gv[:, :3]) < 0)
     if bounce:
         bb.close()
+    w.close()
+    q.close()
+
+
+def _octahedron(scale=1.5):
+    rr = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)], np.float32) * scale
+    tt = np.array([(0, 2, 4), (2, 1, 4), (1, 3, 4), (3, 0, 4), (2, 0, 5), (1, 2, 5), (3, 1, 5), (0, 3, 5)], np.int32)
+    return R.Mesh(rr, tt)
+
+
+def test_switching_to_a_mesh_with_more_cells(ctx):
+    """udx_flu_set_rbc with a 498-vertex mesh (2 cells), then with an
+    octahedron (6 vertices, 100 cells: fewer vertices in all, more cells than
+    the per-cell arrays held): the per-cell arrays are reallocated, and one
+    exact step equals the same step of a solvent that only ever had the
+    octahedra."""
+    L = (16, 16, 16)
+    rng = np.random.default_rng(21)
+    n = 10 * 16 ** 3
+    spp = np.zeros((n, 6), np.float32)
+    spp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    spp[:, 3:] = rng.normal(0, 0.21, (n, 3)).astype(np.float32)
+    big = R.Mesh(*R.read_off(os.path.join(GOLD, "cells", "498.off")))
+    octa = _octahedron()
+    nc = 100
+    T = []
+    for _ in range(nc):
+        A = np.eye(4)
+        A[:3, 3] = rng.uniform(-0.5, 0.5, 3) * np.array(L)
+        T.append(A)
+    opp = R.place(octa.rr, T, (0, 0, 0))
+    opp[:, 3:] = rng.normal(0, 0.05, (len(opp), 3)).astype(np.float32)
+    bpp = R.place(big.rr, [np.eye(4), np.eye(4)], (0, 0, 0))
+    tmpl = np.zeros((octa.nv, 6), np.float32)
+    tmpl[:, :3] = octa.rr
+    A0, V0 = R.area_volume(octa, tmpl, 1)
+    kw = dict(R.RBC_TEST, kb=40.0, totArea=np.float32(A0), totVolume=np.float32(V0))
+    dpd = (2.5, 8.0, O.dpd_sigma(KBT, 8.0, DT), 0.25)
+    fsi_par = U.PairDpd(2.6666667, 8.0, O.dpd_sigma(KBT, 8.0, DT), 0.25)
+    seeds = O.seeds_table(1)
+    outs = []
+    for switch in (True, False):
+        say("switch meshes: %s" % switch)
+        flu = U.Flu(ctx, L, n, U.PairDpd(*dpd), DT, 1.0, exact=True)
+        flu.set_pp(dev(spp))
+        rb = U.Rbc(ctx, big.tt, big.nv, 2)
+        ro = U.Rbc(ctx, octa.tt, octa.nv, nc)
+        ro.set_stressful(kw["totArea"])
+        fsi = U.Fsi(ctx, L, 2 * big.nv)
+        bvert = dev(bpp)
+        vert = dev(opp)
+        if switch:
+            flu.set_rbc(rb, fsi, U.rbc_params(**kw), fsi_par, 0.5, 1, 2, bvert)
+        flu.set_rbc(ro, fsi, U.rbc_params(**kw), fsi_par, 0.5, 1, nc, vert)
+        flu.run(1, seeds)
+        sol = torch.zeros((n, 6), dtype=torch.float32, device="cuda")
+        flu.get_pp(sol)
+        outs.append((host(sol), host(vert)))
+        flu.set_rbc(None)
+        flu.close()
+        rb.close()
+        ro.close()
+    assert np.array_equal(outs[0][0].view(np.uint32), outs[1][0].view(np.uint32))
+    assert np.array_equal(outs[0][1].view(np.uint32), outs[1][1].view(np.uint32))
+
+
+def test_zero_cells_leave_the_wall_trunk_alone(ctx):
+    """With a wall and a membrane handle that holds no cells, the reference
+    skips the vertices' wall force and its seed draw
+    (src/sim/walls/imp/main.h:54-58): three steps from the internal trunks
+    equal three steps of the same solvent with no membranes, bit for bit."""
+    from oracle import wall as W
+    L, h = (16, 16, 24), 9.0
+    rng = np.random.default_rng(8)
+    ext, N = np.array(L, np.float32), (48, 48, 48)
+    q = U.Sdf(ctx, L, ext=ext, N=N, D=W.plane_channel_sdf(L, 48, h))
+    n0 = 10 * L[0] * L[1] * L[2]
+    pp = np.zeros((n0, 6), np.float32)
+    pp[:, :3] = (rng.uniform(-0.5, 0.5, (n0, 3)) * np.array(L)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, 0.21, (n0, 3)).astype(np.float32)
+    ppd = dev(pp)
+    w = U.Wall(ctx, q, ppd, exact=True)
+    solvent = host(ppd)[:w.n_solvent].copy()
+    m = _octahedron()
+    dpd = (2.6666667, 8.0, O.dpd_sigma(KBT, 8.0, DT), 0.25)
+    outs = []
+    for with_rbc in (True, False):
+        say("wall, zero cells: membranes attached %s" % with_rbc)
+        flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*dpd), DT, 1.0, exact=True)
+        sd = dev(solvent)
+        flu.set_pp(sd)
+        flu.set_wall(w)
+        r = U.Rbc(ctx, m.tt, m.nv, 4)
+        fsi = U.Fsi(ctx, L, 4 * m.nv)
+        if with_rbc:
+            flu.set_rbc(r, fsi, U.rbc_params(**R.RBC_TEST), U.PairDpd(*dpd), 0.5, 1, 0, None)
+        flu.run(3)
+        out = torch.zeros_like(sd)
+        nout = flu.get_pp(out)
+        outs.append(host(out)[:nout])
+        flu.set_rbc(None)
+        flu.close()
+        r.close()
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
     w.close()
     q.close()

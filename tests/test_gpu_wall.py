@@ -140,7 +140,7 @@ class Setup:
         self.keep, frozen = self.ref.freeze(pp)
         allw = W.wall_images(L, frozen)
         self.nfrozen = len(frozen)
-        self.wstarts, _, self.wpp, _ = O.clist_build(W.margin_box(L), allw)
+        self.wstarts, _, self.wpp, _ = O.clist_build(W.margin_box(L), allw, sub=1)
 
     def close(self):
         self.w.close()

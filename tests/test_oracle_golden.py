@@ -44,20 +44,30 @@ def test_clist_golden(name, gen):
     assert np.all(np.floor(r[:, 0]) == cx) and np.all(np.floor(r[:, 1]) == cy) and np.all(np.floor(r[:, 2]) == cz)
 
 
-def test_clist_stable_order_and_brute_force():
+@pytest.mark.parametrize("sub", [1, 2, 4, 8])
+def test_clist_stable_order_and_brute_force(sub):
+    """order = (cell, x slab of width 1/sub, input index); cell and slab
+    starts against brute-force counts."""
     rng = np.random.default_rng(0)
     L = (5, 7, 3)
     pp = np.zeros((500, 6), np.float32)
     pp[:, :3] = rng.uniform(-4, 4, (500, 3))   # some outside the box
-    starts, counts, out, order = O.clist_build(L, pp)
+    pp[:7, 0] = [-2.5, -2.25, 2.25, 0.0, 0.125, -0.375, 2.4999998]   # on slab faces
+    starts, counts, out, order, bstarts = O.clist_build(L, pp, sub=sub, slabs=True)
     h = np.array(L) / 2.0
     r = pp[:, :3].astype(np.float64)
     inside = np.all((r >= -h) & (r < h), axis=1)
     c = np.floor(r + h).astype(np.int64)
     cid = c[:, 0] + L[0] * (c[:, 1] + L[1] * c[:, 2])
-    want = sorted(np.nonzero(inside)[0], key=lambda i: (cid[i], i))
+    slab = np.floor((r[:, 0] + h[0]) * sub).astype(np.int64) - c[:, 0] * sub
+    assert np.all((slab[inside] >= 0) & (slab[inside] < sub))
+    want = sorted(np.nonzero(inside)[0], key=lambda i: (cid[i], slab[i], i))
     assert list(order) == want
     assert np.array_equal(out, pp[want])
+    bid = (cid * sub + slab)[inside]
+    assert np.array_equal(np.diff(bstarts), np.bincount(bid, minlength=np.prod(L) * sub))
+    assert np.array_equal(starts, bstarts[::sub])
+    assert np.array_equal(counts, np.bincount(cid[inside], minlength=np.prod(L)))
     assert starts[-1] == inside.sum()
 
 

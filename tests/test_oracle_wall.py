@@ -110,7 +110,7 @@ def wall_setup(L=(16, 12, 24), h=8.0, rho=4, seed=1):
     s, pp = channel(L, h, rho, seed)
     keep, wall = s.freeze(pp)
     allw = W.wall_images(L, wall)
-    ws, wc, wsorted, _ = O.clist_build(W.margin_box(L), allw)
+    ws, wc, wsorted, _ = O.clist_build(W.margin_box(L), allw, sub=1)
     starts, counts, srt, _ = O.clist_build(L, keep)
     return s, srt, wsorted, ws
 
