@@ -24,8 +24,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DPD pair-interactions/s & steps/s on 1 MI355X; batched vs per-box launches"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+# vector FP32 peak without packed (two-per-lane) instructions, which the
+# library is built without: 256 CUs x 4 SIMDs x 16 lanes x 2 flop x 2.4 GHz
+FP32_PEAK_TFLOPS = 78.6
+FP32_PACKED_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_dpd_gather_96cube_wall.md")
+# the sources the force kernel is built from; the committed traffic figure
+# is used only while their hash matches the one it was measured with
+KERNEL_SOURCES = [os.path.join(ROOT, "ctc_amd", "csrc", f) for f in ("dpd.hip", "udx_dev.h", "clist.hip")]
 
 
 def parse():
@@ -38,8 +44,9 @@ def parse():
     ap.add_argument("--half-width", type=float, default=40.0, help="channel half-width (0: periodic, no walls)")
     ap.add_argument("--sdf-grid", type=int, default=128)
     ap.add_argument("--equil", type=int, default=200, help="periodic equilibration steps before the freeze")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps for the CPU baseline (0: skip)")
-    ap.add_argument("--profile-steps", type=int, default=20, help="steps timed per kernel phase")
+    ap.add_argument("--cpu-steps", type=int, default=4,
+                    help="timed steps of the CPU baseline after one warm-up step (0: skip)")
+    ap.add_argument("--c1-steps", type=int, default=1000, help="steps per timed run of the 16^3 leg (0: skip)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="MiB copied by one plain device copy before the workload (calibrates HBM counters)")
     ap.add_argument("--ens-boxes", type=int, default=64, help="boxes of the batched-ensemble leg (0: skip)")
@@ -59,16 +66,30 @@ def plane_channel(L, N, h):
     return np.ascontiguousarray(np.broadcast_to(s[:, None, None], (N, N, N)))
 
 
+def kernel_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic():
     """HBM bytes per k_dpd launch (FETCH_SIZE + WRITE_SIZE, copy-kernel
-    calibrated) from the committed rocprofv3 counter summary."""
+    calibrated) from the committed rocprofv3 counter summary, if it was
+    measured on the current kernel sources (its kernel_sources_sha16 line);
+    None when it is stale or missing."""
+    val = stamp = None
     try:
         for line in open(TRAFFIC_FILE):
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+                val = float(line.split(":")[1])
+            if line.startswith("kernel_sources_sha16:"):
+                stamp = line.split(":")[1].strip()
     except OSError:
-        pass
-    return None
+        return None
+    return val if stamp == kernel_hash() else None
 
 
 def main():
@@ -135,20 +156,18 @@ def main():
     flu.run(args.warmup)
     torch.cuda.synchronize()
 
-    # per-phase kernel times: HIP events on the library's stream, one step at a time
-    flu.timing(True)
-    flu.run(args.profile_steps)
-    t_phase = flu.timing(False)          # ms: clist, solvent DPD, wall DPD, integrate, sum
-    torch.cuda.synchronize()
-
-    # timed region
+    # timed region; per-phase kernel times from HIP events recorded on the
+    # library's stream around each phase of every timed step (read after the
+    # run: no host synchronisation inside it)
     flu.pairs(reset=True)
     w0 = flu.wall_pairs() if walls else 0
+    flu.timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     flu.run(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    t_phase = flu.timing(False)          # ms: clist, solvent DPD, wall DPD, integrate, sum
     pairs = flu.pairs()
     wpairs = (flu.wall_pairs() - w0) if walls else 0
     sec = t1 - t0
@@ -208,7 +227,9 @@ def main():
             "kernel": "k_dpd",
             "achieved": flops / t_force / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": flops / t_force / 1e12 / FP32_PEAK_TFLOPS, "flop_per_launch": flops,
+            "frac_of_packed_peak": flops / t_force / 1e12 / FP32_PACKED_PEAK_TFLOPS,
         },
+        "kernel_sources_sha16": kernel_hash(),
     }
 
     if args.cpu_steps > 0:
@@ -227,6 +248,8 @@ def main():
         sdf.close()
     del pp
     torch.cuda.empty_cache()
+    if args.c1_steps > 0:
+        out["c1"] = c1_leg(ctx, args.c1_steps, args.cpu_steps > 0)
     if args.ens_boxes > 0:
         out["ensemble"] = ensemble_leg(ctx, args.ens_boxes, args.ens_steps)
     if args.adj_cells > 0:
@@ -543,36 +566,108 @@ def host(t):
     return t.cpu().numpy()
 
 
+def cpu_threads():
+    """host threads of the CPU baseline: the cores this process may run on,
+    at most 16 (a GPU box's CPU share per GPU)"""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
 def cpu_baseline(pp, L, params, dt, fx, nsteps, wdata, sdf_grid):
-    """The oracle (CPU restatement: clist, OpenMP force gather, wall DPD,
-    integrate, bounce-back) timed on the same state and walls."""
+    """The reference's formula on the host cores: the oracle's cell list, the
+    reference's half-shell traversal with reactions (one-sided wrapped
+    pairs), wall DPD, kick-drift and bounce-back, timed on the same state and
+    walls after one warm-up step."""
     import numpy as np
 
     import oracle as O
     from oracle import wall as W
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    seeds = O.seeds_table(nsteps)
-    seeds = np.concatenate([seeds, O.kiss_fill(42, 42, 42, 42, nsteps)[:, None]], axis=1)
+    cores = cpu_threads()
+    seeds = O.seeds_table(nsteps + 1)
+    seeds = np.concatenate([seeds, O.kiss_fill(42, 42, 42, 42, nsteps + 1)[:, None]], axis=1)
     q = wpp = wst = None
     if wdata is not None:
         D, wpp, wst = wdata
         q = W.Sdf(L, np.array(L, np.float32), (sdf_grid,) * 3, D.ravel())
-    t0 = time.perf_counter()
-    pairs = 0
-    for k in range(nsteps):
+    pairs, t0 = 0, None
+    for k in range(nsteps + 1):
+        if k == 1:
+            pairs, t0 = 0, time.perf_counter()
         pp, npair, nw = W.flu_wall_step(L, pp, params, seeds[k], dt, q, wpp, wst, accel=(fx, 0.0, 0.0),
-                                        nthreads=cores)
+                                        nthreads=cores, half=True)
         pairs += npair + nw
     sec = time.perf_counter() - t0
     return {"value": pairs / sec, "unit": "pair-interactions/s", "steps_per_s": nsteps / sec,
             "cores": cores, "kind": "port",
-            "sample": "%d full step(s) of the same %dx%dx%d %s box (clist + solvent DPD + wall DPD + integrate "
-                      "+ bounce)" % (nsteps, *L, "wall-bounded" if wdata is not None else "periodic")}
+            "sample": "%d timed step(s) after 1 warm-up of the same %dx%dx%d %s box (clist + half-shell solvent "
+                      "DPD + wall DPD + integrate + bounce), %d threads" % (nsteps, *L,
+                                                                           "wall-bounded" if wdata is not None
+                                                                           else "periodic", cores)}
+
+
+def c1_leg(ctx, nsteps, cpu):
+    """The 16^3, density-3 periodic fluid (a = 75, gamma = 4.5, kBT = 1,
+    dt = 1e-3, spow = 0.25): 50 warm-up steps, then the median of 5 timed
+    runs of nsteps steps; the CPU restatement (half-shell traversal) runs the
+    same nsteps steps from the same state."""
+    import numpy as np
+    import torch
+
+    import oracle as O
+    from ctc_amd import udx as U
+
+    L, rho, dt = (16, 16, 16), 3, 1e-3
+    n = rho * 16 ** 3
+    rng = np.random.default_rng(1)
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, :3] = rng.uniform(-8, 8, (n, 3)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    pp[:, 3:] -= pp[:, 3:].mean(0)
+    params = U.dpd_params(75.0, 4.5, 1.0, dt, 0.25)
+    flu = U.Flu(ctx, L, n, params, dt, 1.0, (0.0, 0.0, 0.0))
+    flu.graph(True)                      # step pairs replayed from a captured HIP graph
+    d = torch.as_tensor(pp).cuda()
+    flu.set_pp(d)
+    flu.run(50)
+    torch.cuda.synchronize()
+    runs, pairs = [], []
+    for _ in range(5):
+        flu.pairs(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        flu.run(nsteps)
+        torch.cuda.synchronize()
+        runs.append(time.perf_counter() - t0)
+        pairs.append(flu.pairs())
+    flu.close()
+    k = int(np.argsort(runs)[2])
+    out = {"workload": "periodic 16^3, density 3 (12288 particles), a = 75, gamma = 4.5, kBT = 1, dt = 1e-3; "
+                       "graph-replayed step pairs",
+           "steps": nsteps, "steps_per_s": nsteps / runs[k], "pairs_per_s": pairs[k] / runs[k],
+           "pairs_per_step": pairs[k] / nsteps, "ms_per_step": runs[k] / nsteps * 1e3,
+           "runs_s": runs}
+    if cpu:
+        cores = cpu_threads()
+        seeds = O.seeds_table(nsteps)
+        ref = pp.copy()
+        ncell = 16 ** 3
+        t0 = time.perf_counter()
+        cp = 0
+        for s in range(nsteps):
+            starts, counts, srt, _ = O.clist_build(L, ref)
+            ff, npair = O.dpd_forces_half(L, srt, starts, counts, (75.0, 4.5, params.s, 0.25), seeds[s],
+                                          nthreads=cores)
+            ref = O.integrate(L, srt, ff, dt)
+            cp += npair // 2
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"steps_per_s": nsteps / sec, "pairs_per_s": cp / sec, "cores": cores,
+                               "kind": "port", "sample": "%d steps of the same box from the same state "
+                               "(%d cells), half-shell traversal" % (nsteps, ncell)}
+    return out
 
 
 if __name__ == "__main__":

@@ -130,8 +130,28 @@ struct udx_flu {
     struct udx_wall *wall;
     float t;            // step clock, float like the reference's (src/scheme/time_line/imp/type.h:1-5)
     int timing;
-    hipEvent_t ev[5];
+    // timing: five events per step (clist, DPD, walls and objects, move,
+    // end), recorded on the library's stream and read after the run, so the
+    // timed steps run without host synchronisation
+    std::vector<hipEvent_t> ev;
+    hipEvent_t *evk = nullptr;   // the current step's five, null when not timing
     float tms[5];
+    // graph replay of step pairs (udx_flu_set_graph): the even/odd step pair
+    // captured on an own stream (capture needs one) and replayed; a device
+    // step index selects each step's seeds from the run's table
+    int use_graph = 0;
+    hipStream_t gs = nullptr;       // the stream steps launch on while capturing and replaying
+    hipEvent_t gdone = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    struct GraphKey {
+        const float *table;
+        int n, cur, mode;
+        const void *wall;
+    } gkey{};
+    int *step_dev = nullptr;        // index of the current step in the seed table
+    float *seeds_cur = nullptr;     // that step's NSEED seeds
+    hipStream_t step_stream = nullptr;   // non-null: flu_step launches there
     // membranes in the step (udx_flu_set_rbc)
     struct Membranes {
         udx_rbc *r = nullptr;
@@ -330,9 +350,28 @@ int udx_fluforces_prepare(udx_fluforces *f, int n, const float *pp) {
     return aos_to_f4(f->ctx->stream, n, pp, f->pos, f->vel);
 }
 
+static int fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
+                           const float seeds[27], float *ff, float *ss);
+
 int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
                               const float seeds[27], float *ff) {
     UDX_CHECK(f && p && starts && seeds, UDX_EINVAL, "udx_fluforces_apply_seeds: null argument");
+    return fluforces_apply(f, p, starts, periodic, seeds, ff, nullptr);
+}
+
+int udx_fluforces_apply_stress(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
+                               const float seeds[27], float *ff, float *ss) {
+    UDX_CHECK(f && p && starts && ff && ss, UDX_EINVAL, "udx_fluforces_apply_stress: null argument");
+    float own[NSEED];
+    if (!seeds) {
+        f->trunks.next(own);
+        seeds = own;
+    }
+    return fluforces_apply(f, p, starts, periodic, seeds, ff, ss);
+}
+
+static int fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
+                           const float seeds[27], float *ff, float *ss) {
     hipStream_t s = f->ctx->stream;
     if (f->n == 0) return UDX_OK;
     UDX_HIP(hipMemcpyAsync(f->seeds_dev, seeds, 27 * sizeof(float), hipMemcpyHostToDevice, s));
@@ -351,6 +390,7 @@ int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int
     a.ff_stride = 3;
     a.accumulate = 1;
     a.pairs = f->pairs_dev;
+    a.ss = ss;
     UDX_TRY(dpd_forces_mode(s, a, f->mode));
     // the seeds live on the host stack: wait for the copy before returning
     UDX_HIP(hipStreamSynchronize(s));
@@ -401,6 +441,7 @@ int udx_diag_part(udx_ctx *ctx, int n, const float *pp, double out[5]) {
 
 // ------------------------------------------------------------------ flu
 int udx_flu_fin(udx_flu *f);
+static void flu_drop_graph(udx_flu *f);
 
 int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
     UDX_CHECK(ctx && conf && out, UDX_EINVAL, "udx_flu_ini: null argument");
@@ -429,11 +470,6 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
         set_error("udx_flu_ini: memset failed");
         s = UDX_EHIP;
     }
-    for (int e = 0; e < 5 && !s; e++)
-        if (hipEventCreate(&f->ev[e]) != hipSuccess) {
-            set_error("udx_flu_ini: event create failed");
-            s = UDX_EHIP;
-        }
     if (s) {
         udx_flu_fin(f);
         return s;
@@ -454,8 +490,13 @@ int udx_flu_fin(udx_flu *f) {
     dev_free(f->seeds_dev);
     dev_free(f->pairs_dev);
     if (f->seeds_host) (void)hipHostFree(f->seeds_host);
-    for (int e = 0; e < 5; e++)
-        if (f->ev[e]) (void)hipEventDestroy(f->ev[e]);
+    for (hipEvent_t e : f->ev) (void)hipEventDestroy(e);
+    if (f->gs) (void)hipStreamSynchronize(f->gs);
+    flu_drop_graph(f);
+    if (f->gs) (void)hipStreamDestroy(f->gs);
+    if (f->gdone) (void)hipEventDestroy(f->gdone);
+    dev_free(f->step_dev);
+    dev_free(f->seeds_cur);
     flu_free_membranes(f);
     delete f;
     return UDX_OK;
@@ -465,6 +506,7 @@ int udx_flu_set_pp(udx_flu *f, int n, const float *pp) {
     UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_pp: null handle");
     UDX_CHECK(n >= 0 && n <= f->conf.maxp, UDX_ECAP, "udx_flu_set_pp: %d particles exceed capacity %d", n,
               f->conf.maxp);
+    if (n != f->n) flu_drop_graph(f);
     f->n = n;
     f->cur = 0;
     return aos_to_f4(f->ctx->stream, n, pp, f->pos[0], f->vel[0]);
@@ -529,12 +571,12 @@ static int flu_seed_table(udx_flu *f, int nsteps, const float *seeds_host) {
 // redistribution, DPD forces, walls, FSI, body force + solvent kick-drift +
 // bounce + periodic wrap, membrane sub-steps (FSI + internal forces, kick-drift)
 static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, const float *mwall_seed) {
-    hipStream_t s = f->ctx->stream;
+    hipStream_t s = f->step_stream ? f->step_stream : f->ctx->stream;
     const int nc = f->box.ncells();
     const int a = f->cur, b = 1 - f->cur;
     udx_flu::Membranes &m = f->mb;
     const int nvtx = m.nc * m.nv;
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[0], s));
+    if (f->evk) UDX_HIP(hipEventRecord(f->evk[0], s));
     UDX_TRY(clist_sort(s, &f->cl, f->n, f->pos[a]));
     UDX_TRY(gather_f4(s, &f->cl, f->n, f->pos[a], f->vel[a], f->pos[b], f->vel[b]));
     UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts));
@@ -543,7 +585,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     // objects_save_mesh (src/sim/imp/run.h:30-31): the vertices at the start of the step
     if (bounce && nvtx > 0)
         UDX_HIP(hipMemcpyAsync(m.start_pp, m.pp, 6 * sizeof(float) * (size_t)nvtx, hipMemcpyDeviceToDevice, s));
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[1], s));
+    if (f->evk) UDX_HIP(hipEventRecord(f->evk[1], s));
     DpdLaunch d;
     d.pos = f->pos[b];
     d.vel = f->vel[b];
@@ -561,7 +603,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     d.accumulate = 0;
     d.pairs = f->pairs_dev;
     UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[2], s));
+    if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
     const SdfField *sdf = nullptr;
     WallVel wv{};
     if (f->wall) {
@@ -597,7 +639,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
         (void)cnt_draw(m.cnt);
         (void)cnt_draw(m.cnt);
     }
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[3], s));
+    if (f->evk) UDX_HIP(hipEventRecord(f->evk[3], s));
     // store_solvent (src/sim/imp/update.h:11-16): positions before the move
     if (bounce && f->n > 0)
         UDX_HIP(hipMemcpyAsync(m.pos0, f->pos[b], sizeof(float4) * (size_t)f->n, hipMemcpyDeviceToDevice, s));
@@ -621,9 +663,83 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     if (bounce)
         UDX_TRY(mbb_step(s, m.bb, f->conf.dt, f->conf.mass, m.mass, f->box, m.nc, m.start_pp, m.pp, f->cl.starts,
                          f->n, f->cl.starts + nc, f->pos[b], f->vel[b], m.pos0));
-    if (f->timing) UDX_HIP(hipEventRecord(f->ev[4], s));
+    if (f->evk) UDX_HIP(hipEventRecord(f->evk[4], s));
     f->cur = b;
     f->t += f->conf.dt;   // time_line_advance (src/scheme/time_line/imp/main.h:10-21)
+    return UDX_OK;
+}
+
+namespace {
+// the seeds of step *step from the run's table, and the step counter
+__global__ void k_flu_seeds(const float *table, const int *step, float *cur) {
+    const int k = threadIdx.x;
+    if (k < NSEED) cur[k] = table[(long)*step * NSEED + k];
+}
+__global__ void k_flu_next(int *step) {
+    if (threadIdx.x == 0) *step += 1;
+}
+}  // namespace
+
+static void flu_drop_graph(udx_flu *f) {
+    if (f->gexec) (void)hipGraphExecDestroy(f->gexec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
+    f->gexec = nullptr;
+    f->graph = nullptr;
+}
+
+// steps that a captured graph can replay: everything the step does is on
+// the device and the same from step to step (no membranes, whose seeds and
+// sub-steps are host-driven; no time-dependent wall velocity; no timing)
+static bool flu_graph_ok(const udx_flu *f) {
+    return f->use_graph && !f->mb.r && !f->timing && !(f->wall && f->wall->wvel.type == UDX_WVEL_SHEAR_SIN);
+}
+
+// capture the step pair starting from the current parity on f->gs
+static int flu_capture(udx_flu *f) {
+    flu_drop_graph(f);
+    const float t0 = f->t;
+    const int cur0 = f->cur;
+    f->step_stream = f->gs;
+    UDX_HIP(hipStreamBeginCapture(f->gs, hipStreamCaptureModeThreadLocal));
+    int st = UDX_OK;
+    for (int k = 0; k < 2 && !st; k++) {
+        hipLaunchKernelGGL(k_flu_seeds, dim3(1), dim3(64), 0, f->gs, f->seeds_dev, f->step_dev, f->seeds_cur);
+        st = hipGetLastError() == hipSuccess ? UDX_OK : UDX_EHIP;
+        if (!st) st = flu_step(f, f->seeds_cur, nullptr, nullptr);
+        if (!st) {
+            hipLaunchKernelGGL(k_flu_next, dim3(1), dim3(64), 0, f->gs, f->step_dev);
+            st = hipGetLastError() == hipSuccess ? UDX_OK : UDX_EHIP;
+        }
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(f->gs, &g);
+    f->step_stream = nullptr;
+    f->t = t0;
+    f->cur = cur0;
+    if (st && g) (void)hipGraphDestroy(g);
+    UDX_TRY(st);
+    UDX_CHECK(ce == hipSuccess && g, UDX_EHIP, "udx_flu_run: graph capture failed: %s", hipGetErrorString(ce));
+    f->graph = g;
+    UDX_HIP(hipGraphInstantiate(&f->gexec, g, nullptr, nullptr, 0));
+    f->gkey = udx_flu::GraphKey{f->seeds_dev, f->n, f->cur, f->conf.mode, f->wall};
+    return UDX_OK;
+}
+
+// nsteps >= 2 steps by graph replay (pairs), the last one eagerly when odd
+static int flu_run_graph(udx_flu *f, int nsteps) {
+    hipStream_t s = f->ctx->stream;
+    UDX_HIP(hipMemsetAsync(f->step_dev, 0, sizeof(int), s));
+    UDX_HIP(hipEventRecord(f->gdone, s));
+    UDX_HIP(hipStreamWaitEvent(f->gs, f->gdone, 0));
+    const udx_flu::GraphKey k{f->seeds_dev, f->n, f->cur, f->conf.mode, f->wall};
+    if (!f->gexec || memcmp(&k, &f->gkey, sizeof k) != 0) UDX_TRY(flu_capture(f));
+    const int pairs = nsteps / 2;
+    for (int p = 0; p < pairs; p++) UDX_HIP(hipGraphLaunch(f->gexec, f->gs));
+    UDX_HIP(hipEventRecord(f->gdone, f->gs));
+    UDX_HIP(hipStreamWaitEvent(s, f->gdone, 0));
+    f->t += (float)(2 * pairs) * f->conf.dt;
+    if (nsteps & 1) UDX_TRY(flu_step(f, f->seeds_dev + NSEED * (nsteps - 1), flu_fsi_seeds(f, nsteps - 1),
+                                     flu_mwall_seed(f, nsteps - 1)));
     return UDX_OK;
 }
 
@@ -631,19 +747,29 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
     UDX_CHECK(f && nsteps >= 0, UDX_EINVAL, "udx_flu_run: bad arguments");
     if (nsteps == 0) return UDX_OK;
     UDX_TRY(flu_seed_table(f, nsteps, seeds_host));
-    float acc[4] = {0, 0, 0, 0};
-    for (int k = 0; k < nsteps; k++) {
-        UDX_TRY(flu_step(f, f->seeds_dev + NSEED * k, flu_fsi_seeds(f, k), flu_mwall_seed(f, k)));
-        if (f->timing) {
-            UDX_HIP(hipEventSynchronize(f->ev[4]));
-            for (int c = 0; c < 4; c++) {
-                float t;
-                UDX_HIP(hipEventElapsedTime(&t, f->ev[c], f->ev[c + 1]));
-                acc[c] += t;
-            }
+    if (nsteps >= 2 && flu_graph_ok(f)) return flu_run_graph(f, nsteps);
+    if (f->timing) {
+        while (f->ev.size() < 5 * (size_t)nsteps) {
+            hipEvent_t e;
+            UDX_HIP(hipEventCreate(&e));
+            f->ev.push_back(e);
         }
     }
+    for (int k = 0; k < nsteps; k++) {
+        f->evk = f->timing ? f->ev.data() + 5 * (size_t)k : nullptr;
+        const int st = flu_step(f, f->seeds_dev + NSEED * k, flu_fsi_seeds(f, k), flu_mwall_seed(f, k));
+        f->evk = nullptr;
+        UDX_TRY(st);
+    }
     if (f->timing) {
+        float acc[4] = {0, 0, 0, 0};
+        UDX_HIP(hipEventSynchronize(f->ev[5 * (size_t)(nsteps - 1) + 4]));
+        for (int k = 0; k < nsteps; k++)
+            for (int c = 0; c < 4; c++) {
+                float t;
+                UDX_HIP(hipEventElapsedTime(&t, f->ev[5 * (size_t)k + c], f->ev[5 * (size_t)k + c + 1]));
+                acc[c] += t;
+            }
         for (int c = 0; c < 4; c++) f->tms[c] = acc[c] / nsteps;
         f->tms[4] = f->tms[0] + f->tms[1] + f->tms[2] + f->tms[3];
     }
@@ -911,7 +1037,24 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w) {
     UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_wall: null handle");
     UDX_CHECK(!w || (w->sdf->f.L[0] == f->box.L[0] && w->sdf->f.L[1] == f->box.L[1] && w->sdf->f.L[2] == f->box.L[2]),
               UDX_EINVAL, "udx_flu_set_wall: wall and solvent boxes differ");
+    if (w != f->wall) flu_drop_graph(f);
     f->wall = w;
+    return UDX_OK;
+}
+
+int udx_flu_set_graph(udx_flu *f, int enable) {
+    UDX_CHECK(f, UDX_EINVAL, "udx_flu_set_graph: null handle");
+    if (enable && !f->gs) {
+        UDX_HIP(hipStreamCreate(&f->gs));
+        UDX_HIP(hipEventCreateWithFlags(&f->gdone, hipEventDisableTiming));
+        UDX_TRY(dev_alloc(&f->step_dev, 1));
+        UDX_TRY(dev_alloc(&f->seeds_cur, NSEED));
+    }
+    if (!enable && f->gs) {
+        UDX_HIP(hipStreamSynchronize(f->gs));
+        flu_drop_graph(f);
+    }
+    f->use_graph = enable ? 1 : 0;
     return UDX_OK;
 }
 

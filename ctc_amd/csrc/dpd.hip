@@ -45,6 +45,7 @@ struct DpdK {
     float *ff;
     int ff_stride, accumulate;
     unsigned long long *pairs;
+    float *ss;                // per-particle stress (6 floats, added) or null
 };
 
 // What the force pass reads of a box, read once at kernel entry.  The
@@ -229,7 +230,7 @@ __device__ __forceinline__ void run_range(const DpdP &P, int base, int x0, int x
 
 // The force pass of one block of NT consecutive particles starting at i
 // = first + threadIdx.x; QC queued pairs per lane before a drain.
-template <int WR, bool EXACT, int NT, int QC, int SC>
+template <int WR, bool EXACT, int NT, int QC, int SC, bool STRESS = false>
 __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     __shared__ unsigned q[QC * NT];
     __shared__ float4 sbuf[NT / 64][SC + 4];
@@ -257,6 +258,8 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     }
     const float seed_bulk = P.seeds[26];
     float fx = 0.f, fy = 0.f, fz = 0.f;
+    // stress 1/2 f_a dr_b, (xx, xy, xz, yy, yz, zz) (src/pair/dev.h:87-100)
+    float sxx = 0.f, sxy = 0.f, sxz = 0.f, syy = 0.f, syz = 0.f, szz = 0.f;
     int npair = 0, fill = 0, head = 0;
     unsigned *qlane = q + tid;
 
@@ -290,6 +293,18 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         fx += vb ? f2.x : 0.f;
         fy += vb ? f2.y : 0.f;
         fz += vb ? f2.z : 0.f;
+        if (STRESS) {
+            auto add = [&](bool v, const float3 &f, const PairIn &q) {
+                sxx += v ? 0.5f * f.x * q.dx : 0.f;
+                sxy += v ? 0.5f * f.x * q.dy : 0.f;
+                sxz += v ? 0.5f * f.x * q.dz : 0.f;
+                syy += v ? 0.5f * f.y * q.dy : 0.f;
+                syz += v ? 0.5f * f.y * q.dz : 0.f;
+                szz += v ? 0.5f * f.z * q.dz : 0.f;
+            };
+            add(va, f1, a);
+            add(vb, f2, b);
+        }
         const int took = fill < 2 ? fill : 2;
         head = ring(took);
         fill -= took;
@@ -453,6 +468,11 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
                 f3[0] = fx; f3[1] = fy; f3[2] = fz;
             }
         }
+        if (STRESS) {
+            UDX_GLOBAL float *s6 = gptr(A.ss) + 6 * (long)i;
+            s6[0] += sxx; s6[1] += sxy; s6[2] += sxz;
+            s6[3] += syy; s6[4] += syz; s6[5] += szz;
+        }
     }
     if (A.pairs) {
         unsigned long long c = (unsigned long long)npair;
@@ -472,6 +492,13 @@ template <int WR, bool EXACT>
 // at most 128 VGPRs: four waves per SIMD
 __global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_dpd(DpdK A) {
     dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+}
+
+// with the per-particle stress (the reference's FoSArray path,
+// src/fluforces/bulk/imp/main.h:42-53)
+template <int WR>
+__global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
+    dpd_block<WR, true, DPD_THREADS, QCAP, SCAP, true>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
 }
 
 // Many boxes in one launch: block b belongs to the box k with
@@ -537,6 +564,7 @@ DpdK make_k(const DpdLaunch &a) {
     k.ff_stride = a.ff_stride;
     k.accumulate = a.accumulate;
     k.pairs = a.pairs;
+    k.ss = a.ss;
     return k;
 }
 
@@ -593,6 +621,19 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
     if (a.nmax <= 0) return UDX_OK;
     const DpdK k = make_k(a);
     const float sp = a.par.spow;
+    if (a.ss) {
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(ceil_div(a.nmax, DPD_THREADS)), dim3(DPD_THREADS), 0, s, k);
+            return hipGetLastError();
+        };
+        hipError_t e;
+        if (sp == 0.25f) e = go(k_dpd_stress<WR_QUARTER>);
+        else if (sp == 0.5f) e = go(k_dpd_stress<WR_HALF>);
+        else if (sp == 1.0f) e = go(k_dpd_stress<WR_ONE>);
+        else e = go(k_dpd_stress<WR_POW>);
+        UDX_CHECK(e == hipSuccess, UDX_EHIP, "dpd: stress kernel launch: %s", hipGetErrorString(e));
+        return UDX_OK;
+    }
     if (exact) {
         if (sp == 0.25f) return launch_dpd<WR_QUARTER, true>(s, k, a.nmax);
         if (sp == 0.5f) return launch_dpd<WR_HALF, true>(s, k, a.nmax);

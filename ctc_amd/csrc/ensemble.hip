@@ -123,6 +123,19 @@ __global__ __launch_bounds__(256) void k_ens_move(int n, int nbox, const int *po
     vel[i] = v;
 }
 
+// periodic wrap of the particles into their boxes (set_pp): the boxes keep
+// every particle, so a position rounded onto +L/2 must come back inside
+__global__ __launch_bounds__(256) void k_ens_wrap(int n, int nbox, const int *poff, const MoveBox *mb, float4 *pos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const MoveBox m = mb[find_box_wave(poff, nbox, i)];
+    float4 p = pos[i];
+    p.x = wrap_coord(p.x, m.Lx);
+    p.y = wrap_coord(p.y, m.Ly);
+    p.z = wrap_coord(p.z, m.Lz);
+    pos[i] = p;
+}
+
 // the step's 27 seeds of every box from the run's table, then the counter
 __global__ void k_ens_seeds(int nper, const float *table, int *step, float *cur) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,6 +446,10 @@ int udx_ens_set_pp(udx_ens *e, const float *pp) {
     UDX_HIP(hipEventRecord(e->done, e->ctx->stream));
     UDX_HIP(hipStreamWaitEvent(e->s, e->done, 0));
     UDX_TRY(aos_to_f4(e->s, e->ntot, pp, e->pos[0], e->vel[0]));
+    if (e->ntot > 0)
+        hipLaunchKernelGGL(k_ens_wrap, dim3(ceil_div(e->ntot, 256)), dim3(256), 0, e->s, e->ntot, e->nbox, e->poff,
+                           e->mb, e->pos[0]);
+    UDX_LAUNCHED();
     e->cur = 0;
     UDX_HIP(hipEventRecord(e->done, e->s));
     UDX_HIP(hipStreamWaitEvent(e->ctx->stream, e->done, 0));

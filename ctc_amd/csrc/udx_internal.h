@@ -135,6 +135,7 @@ struct DpdLaunch {
     int accumulate;
     unsigned long long *pairs;   // ordered-pair counter or null
     int i0 = 0;             // first particle of the box in pos/vel (noise indices are relative)
+    float *ss = nullptr;    // per-particle stress, 6 floats added (null: forces only)
 };
 int dpd_forces(hipStream_t s, const DpdLaunch &a);
 // mode: UDX_MODE_FAST or UDX_MODE_EXACT

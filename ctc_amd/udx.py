@@ -82,6 +82,7 @@ _SIGS = {
     "udx_clist_starts": (_VP, [_VP]),
     "udx_clist_counts": (_VP, [_VP]),
     "udx_clist_order": (_VP, [_VP]),
+    "udx_clist_build_noalloc": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
     "udx_fluforces_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
     "udx_fluforces_fin": (_I, [_VP]),
     "udx_fluforces_prepare": (_I, [_VP, _I, _VP]),
@@ -162,6 +163,8 @@ _SIGS = {
     "udx_rbc_force_apply": (_I, [_VP, C.POINTER(RbcParams), _F, _I, _VP, _VP]),
     "udx_rbc_stretch_apply": (_I, [_VP, _I, _I, _VP, _VP]),
     "udx_rbc_force_adjoint": (_I, [_VP, C.POINTER(RbcParams), _I, _VP, _VP, _VP, C.POINTER(C.c_double)]),
+    "udx_flu_set_graph": (_I, [_VP, _I]),
+    "udx_fluforces_apply_stress": (_I, [_VP, C.POINTER(PairDpd), _VP, _I, C.POINTER(_F), _VP, _VP]),
     "udx_flu_set_rbc": (_I, [_VP, _VP, _VP, C.POINTER(RbcParams), C.POINTER(PairDpd), _F, _I, _I, _VP]),
     "udx_rbc_distribute": (_I, [_VP, _I, _I, _I, _I, _VP]),
     "udx_fsi_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
@@ -333,6 +336,17 @@ class Clist:
             self.h = C.c_void_p()
 
 
+# static working space of udx_clist_build_noalloc (include/udx.h)
+NOALLOC_MAXCELLS, NOALLOC_MAXP = 1 << 18, 1 << 20
+
+
+def clist_build_noalloc(ctx, L, pp, ppout, starts, counts, order):
+    """udx_clist_build_noalloc: the cell list with caller-owned outputs and no
+    device allocation (torch device tensors; ppout may be None)."""
+    _ok(lib().udx_clist_build_noalloc(ctx.h, int(L[0]), int(L[1]), int(L[2]), int(pp.shape[0]), _ptr(pp),
+                                      _ptr(ppout), _ptr(starts), _ptr(counts), _ptr(order)))
+
+
 def _copy_dev(dst, src_ptr, nbytes):
     """Device-to-device copy from a raw pointer owned by libudx."""
     import torch
@@ -366,6 +380,12 @@ class FluForces:
             arr = (C.c_float * 27)(*[float(x) for x in seeds])
             _ok(lib().udx_fluforces_apply_seeds(self.h, C.byref(params), starts_ptr, 1 if periodic else 0,
                                                 arr, _ptr(ff)))
+
+    def apply_stress(self, params, starts_ptr, ff, ss, periodic=True, seeds=None):
+        """forces and per-particle stresses (ss: torch (n, 6) float32), added"""
+        arr = None if seeds is None else (C.c_float * 27)(*[float(x) for x in seeds])
+        _ok(lib().udx_fluforces_apply_stress(self.h, C.byref(params), starts_ptr, 1 if periodic else 0, arr,
+                                             _ptr(ff), _ptr(ss)))
 
     def pairs(self, reset=False):
         v = C.c_ulonglong()
@@ -444,6 +464,10 @@ class Flu:
     def set_wall(self, wall):
         self.wall = wall
         _ok(lib().udx_flu_set_wall(self.h, wall.h if wall is not None else None))
+
+    def graph(self, enable=True):
+        """replay captured step pairs (udx_flu_set_graph)"""
+        _ok(lib().udx_flu_set_graph(self.h, 1 if enable else 0))
 
     def set_time(self, t):
         """the step clock (time of the next step), advanced by dt per step"""

@@ -66,6 +66,20 @@ int udx_clist_build(udx_clist *c, int n, const float *pp, float *ppout, int *nou
 const int *udx_clist_starts(const udx_clist *c);   /* device, ncells + 1 */
 const int *udx_clist_counts(const udx_clist *c);   /* device, ncells     */
 const int *udx_clist_order(const udx_clist *c);    /* device, input index per output slot */
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  udx_clist_build without a handle: the caller passes every
+ * output (ppout: n x 6 floats or NULL, starts: ncells + 1, counts: ncells,
+ * order: n ints; device), same order and tables as udx_clist_build.  The
+ * working arrays live in a static device buffer that each GPU holds for the
+ * process (about 17 MB), sized for UDX_NOALLOC_MAXCELLS cells and
+ * UDX_NOALLOC_MAXP particles; larger calls return UDX_ECAP.  Calls that
+ * overlap on one GPU share that buffer and race.  Does not synchronise: the
+ * number kept is starts[ncells] on the device. */
+#define UDX_NOALLOC_MAXCELLS (1 << 18)
+#define UDX_NOALLOC_MAXP (1 << 20)
+int udx_clist_build_noalloc(udx_ctx *ctx, int Lx, int Ly, int Lz, int n, const float *pp, float *ppout,
+                            int *starts, int *counts, int *order);
 
 /* ---- fluid-fluid DPD forces: replaces fluforces_bulk_ini / prepare / apply
  *      and fluforces_halo_ini / prepare / apply (src/fluforces/imp.h:8-25) ----
@@ -87,6 +101,15 @@ int udx_fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *star
  * seeds[26] for the bulk */
 int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int *starts,
                               int periodic, const float seeds[27], float *ff);
+/* forces and the per-particle stress: the reference's FoSArray path
+ * (src/fluforces/bulk/imp/main.h:42-53 dispatch_stress, src/pair/dev.h:87-100):
+ * ss (device, 6 floats per particle: xx, xy, xz, yy, yz, zz) gets, per pair
+ * of the particle, 1/2 f_a dr_b with f the force on it and dr = r - r_other
+ * (both particles of a pair get the same), ADDED like the forces.  seeds as
+ * udx_fluforces_apply_seeds, or NULL for the next draws of the trunks.
+ * Correctly rounded arithmetic (the mode setting does not apply). */
+int udx_fluforces_apply_stress(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
+                               const float seeds[27], float *ff, float *ss);
 /* force kernel modes; both gather the full shell of each particle in a fixed
  * order (no atomics), so results are identical from run to run */
 #define UDX_MODE_FAST 0    /* default: hardware reciprocal and roots (1 ulp) */
@@ -200,6 +223,13 @@ int udx_flu_set_wall(udx_flu *f, udx_wall *w);
 /* the step clock: time of the next step (advanced by dt per step; the walls'
  * velocity is evaluated at the step's time) */
 int udx_flu_set_time(udx_flu *f, float t);
+/* enable != 0: udx_flu_run captures the step pair (even, odd) into a HIP
+ * graph on the solver's own stream and replays it (the same results; one
+ * launch per two steps instead of about fifteen per step).  Runs with
+ * membranes attached, a time-dependent wall velocity or timing enabled stay
+ * eager.  No counterpart in the reference (its steps are launched from the
+ * host one kernel at a time). */
+int udx_flu_set_graph(udx_flu *f, int enable);
 /* per-step kernel timing of the last udx_flu_run (ms, HIP events on the
  * bound stream): [0] cell list + reorder + fragment tables, [1] solvent DPD,
  * [2] wall DPD, [3] integrate + bounce-back, [4] their sum */
@@ -456,7 +486,8 @@ typedef struct udx_ens udx_ens;
 int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **e);
 int udx_ens_fin(udx_ens *e);
 int udx_ens_set_mode(udx_ens *e, int mode);      /* UDX_MODE_FAST or UDX_MODE_EXACT */
-/* pp: AoS particles of all boxes, box after box (sum of n) */
+/* pp: AoS particles of all boxes, box after box (sum of n); positions are
+ * wrapped into their box's [-L/2, L/2) (each box keeps all its particles) */
 int udx_ens_set_pp(udx_ens *e, const float *pp);
 int udx_ens_get_pp(udx_ens *e, float *pp);       /* cell order within each box */
 /* seeds: NULL (the boxes' trunks) or nsteps x nbox x 27 floats (26 halo

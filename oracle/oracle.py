@@ -76,6 +76,9 @@ def _declare(L):
         "orc_pair_force": (C.c_int, [_p(Dpd), F32P, F32P, C.c_float, F32P]),
         "orc_dpd_forces": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
                                       C.c_int, F32P]),
+        "orc_dpd_forces_stress": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
+                                             C.c_int, F32P, F32P]),
+        "orc_dpd_forces_half": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int, F32P]),
         "orc_integrate": (None, [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P]),
         "orc_move": (None, [C.c_int, C.c_float, C.c_float, F32P, F32P]),
         "orc_diag": (None, [C.c_int, F32P, F64P]),
@@ -225,6 +228,32 @@ def dpd_forces(L, pp, starts, counts, params, seeds, periodic=True, nthreads=1, 
     np_ = lib().orc_dpd_forces(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
                                _ptr(_i32(counts), I32P), Dpd(*params), _ptr(seeds, F32P),
                                1 if periodic else 0, nthreads, _ptr(ff, F32P))
+    return ff, int(np_)
+
+
+def dpd_forces_stress(L, pp, starts, counts, params, seeds, periodic=True, nthreads=1):
+    """Forces and per-particle stresses (n, 6: xx, xy, xz, yy, yz, zz);
+    returns (ff, ss, ordered pair count)."""
+    L = _L(L)
+    pp = _f32(pp).reshape(-1, 6)
+    n = pp.shape[0]
+    ff = np.zeros((n, 3), np.float32)
+    ss = np.zeros((n, 6), np.float32)
+    np_ = lib().orc_dpd_forces_stress(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
+                                      _ptr(_i32(counts), I32P), Dpd(*params), _ptr(_f32(seeds), F32P),
+                                      1 if periodic else 0, nthreads, _ptr(ff, F32P), _ptr(ss, F32P))
+    return ff, ss, int(np_)
+
+
+def dpd_forces_half(L, pp, starts, counts, params, seeds, nthreads=1, ff=None):
+    """The reference's half-shell traversal (CPU baseline); as dpd_forces."""
+    L = _L(L)
+    pp = _f32(pp).reshape(-1, 6)
+    n = pp.shape[0]
+    ff = np.zeros((n, 3), np.float32) if ff is None else _f32(ff).copy()
+    np_ = lib().orc_dpd_forces_half(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
+                                    _ptr(_i32(counts), I32P), Dpd(*params), _ptr(_f32(seeds), F32P), nthreads,
+                                    _ptr(ff, F32P))
     return ff, int(np_)
 
 

@@ -398,13 +398,17 @@ static int frag_pos(const orc_grid *g, int fid, const int c[3], int idx) {
     return g->fss[g->fss_off[fid] + h] + (idx - g->starts[cid]);
 }
 
+/* so (may be NULL): the particle's stress, sum over its pairs of
+ * 1/2 f (x) dr as (xx, xy, xz, yy, yz, zz) with s_ab = 1/2 f_a dr_b
+ * (src/pair/dev.h:87-100) */
 static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, const float seeds[27],
-                       int periodic, int i, float *fo) {
+                       int periodic, int i, float *fo, float *so) {
     const int *L = g->L;
     const float *pi = pp + 6 * i;
     int ci[3];
     long npairs = 0;
     float acc[3] = {0.0f, 0.0f, 0.0f};
+    float sacc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     (void)n;
     if (orc_cell_of(L, pi, ci) < 0) return 0;
     for (int dz = -1; dz <= 1; dz++)
@@ -450,11 +454,40 @@ static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, con
             float f[3];
             orc_pair_force(&p, a, pj, rnd, f);
             acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2];
+            if (so) {
+                sacc[0] += 0.5f * f[0] * ddx;
+                sacc[1] += 0.5f * f[0] * ddy;
+                sacc[2] += 0.5f * f[0] * ddz;
+                sacc[3] += 0.5f * f[1] * ddy;
+                sacc[4] += 0.5f * f[1] * ddz;
+                sacc[5] += 0.5f * f[2] * ddz;
+            }
             npairs++;
         }
     }
     fo[0] += acc[0]; fo[1] += acc[1]; fo[2] += acc[2];
+    if (so)
+        for (int k = 0; k < 6; k++) so[k] += sacc[k];
     return npairs;
+}
+
+long orc_dpd_forces_stress(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                           orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff, float *ss) {
+    int tsz = orc_frag_table_size(L);
+    int *fss = (int *)malloc(sizeof(int) * (size_t)tsz);
+    int fss_off[27];
+    orc_frag_tables(L, counts, fss, fss_off);
+    orc_grid g = {L, starts, fss, fss_off};
+    long tot = 0;
+    (void)nthreads;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+    #pragma omp parallel for schedule(dynamic, 256) reduction(+:tot) num_threads(nthreads)
+#endif
+    for (int i = 0; i < n; i++)
+        tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, ss ? ss + 6 * (size_t)i : NULL);
+    free(fss);
+    return tot;
 }
 
 long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, const int *counts,
@@ -470,7 +503,106 @@ long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, c
     if (nthreads < 1) nthreads = 1;
     #pragma omp parallel for schedule(dynamic, 256) reduction(+:tot) num_threads(nthreads)
 #endif
-    for (int i = 0; i < n; i++) tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i);
+    for (int i = 0; i < n; i++) tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, NULL);
+    free(fss);
+    return tot;
+}
+
+/* Half-shell pass of particle i, the reference's own traversal
+ * (src/fluforces/bulk/dev/main.h:23-97 for pairs inside the box: each
+ * unordered pair once, from its larger index, the reaction added to the
+ * smaller; src/fluforces/halo/dev/main.h:7-83 for pairs across the periodic
+ * boundary: one-sided, each side with its own fragment noise).  Forces go
+ * to the thread's buffer f. */
+static long half_one(const orc_grid *g, const float *pp, orc_dpd p, const float seeds[27], int i, float *f) {
+    const int *L = g->L;
+    const float *pi = pp + 6 * i;
+    int ci[3];
+    long npairs = 0;
+    if (orc_cell_of(L, pi, ci) < 0) return 0;
+    const int cid_i = ci[0] + L[0] * (ci[1] + L[1] * ci[2]);
+    for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+        int o[3] = {dx, dy, dz}, cj[3], w[3];
+        for (int k = 0; k < 3; k++) {
+            int c = ci[k] + o[k];
+            w[k] = c < 0 ? -1 : (c >= L[k] ? 1 : 0);
+            cj[k] = c - w[k] * L[k];
+        }
+        const int wrapped = w[0] | w[1] | w[2];
+        const int cid = cj[0] + L[0] * (cj[1] + L[1] * cj[2]);
+        if (!wrapped && cid > cid_i) continue;      /* the other half of the shell */
+        float a[6];
+        memcpy(a, pi, sizeof a);
+        for (int k = 0; k < 3; k++) if (w[k]) a[k] = a[k] - (float)(w[k] * L[k]);
+        int fid = 0, aid = 0, mask = 0;
+        if (wrapped) {
+            fid = orc_frag_d2i(w[0], w[1], w[2]);
+            aid = frag_pos(g, fid, ci, i);
+            mask = fid < orc_frag_anti(fid);
+        }
+        const int jend = (!wrapped && cid == cid_i) ? i : g->starts[cid + 1];
+        for (int j = g->starts[cid]; j < jend; j++) {
+            if (j == i) continue;
+            const float *pj = pp + 6 * j;
+            float ddx = a[0] - pj[0], ddy = a[1] - pj[1], ddz = a[2] - pj[2];
+            float r2 = fmaf(ddz, ddz, fmaf(ddx, ddx, ddy * ddy));
+            if (!(r2 < 1.0f)) continue;
+            float rnd, fij[3];
+            if (!wrapped) {
+                rnd = orc_mean0var1ii(seeds[26], i, j);
+            } else {
+                int bid = frag_pos(g, orc_frag_anti(fid), cj, j);
+                uint32_t a1 = (uint32_t)(mask ? aid : bid), a2 = (uint32_t)(mask ? bid : aid);
+                rnd = orc_mean0var1uu(seeds[fid], a1, a2);
+            }
+            orc_pair_force(&p, a, pj, rnd, fij);
+            for (int k = 0; k < 3; k++) f[3 * i + k] += fij[k];
+            if (!wrapped) {
+                for (int k = 0; k < 3; k++) f[3 * j + k] -= fij[k];
+                npairs += 2;
+            } else {
+                npairs++;
+            }
+        }
+    }
+    return npairs;
+}
+
+long orc_dpd_forces_half(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                         orc_dpd p, const float seeds[27], int nthreads, float *ff) {
+    int tsz = orc_frag_table_size(L);
+    int *fss = (int *)malloc(sizeof(int) * (size_t)tsz);
+    int fss_off[27];
+    orc_frag_tables(L, counts, fss, fss_off);
+    orc_grid g = {L, starts, fss, fss_off};
+    if (nthreads < 1) nthreads = 1;
+    const size_t m = 3 * (size_t)n;
+    float *buf = (float *)calloc((size_t)nthreads * m + 1, sizeof(float));
+    long tot = 0;
+#ifdef _OPENMP
+    #pragma omp parallel num_threads(nthreads) reduction(+:tot)
+#endif
+    {
+#ifdef _OPENMP
+        float *mine = buf + m * (size_t)omp_get_thread_num();
+        #pragma omp for schedule(dynamic, 256)
+#else
+        float *mine = buf;
+#endif
+        for (int i = 0; i < n; i++) tot += half_one(&g, pp, p, seeds, i, mine);
+    }
+    /* the threads' buffers added in thread order */
+#ifdef _OPENMP
+    #pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (long k = 0; k < (long)m; k++) {
+        float s = 0.0f;
+        for (int t = 0; t < nthreads; t++) s += buf[(size_t)t * m + k];
+        ff[k] += s;
+    }
+    free(buf);
     free(fss);
     return tot;
 }

@@ -93,6 +93,21 @@ int orc_pair_force(const orc_dpd *p, const float *a, const float *b, float rnd, 
 long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, const int *counts,
                     orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff);
 
+/* orc_dpd_forces that also ADDS each particle's stress to ss (6 per
+ * particle: xx, xy, xz, yy, yz, zz): the sum over its pairs of 1/2 f_a dr_b
+ * with f the force on the particle and dr = r_particle - r_other
+ * (src/pair/dev.h:87-100; both particles of a pair get the same stress) */
+long orc_dpd_forces_stress(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                           orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff, float *ss);
+
+/* The same forces with the reference's traversal: a half shell inside the
+ * box (each pair once, reaction to the smaller index), one-sided pairs across
+ * the periodic boundary; per-thread force buffers added in thread order.
+ * The CPU baseline's force pass (not a checker: its FP32 sums are in another
+ * order).  Returns ordered pairs as orc_dpd_forces. */
+long orc_dpd_forces_half(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                         orc_dpd p, const float seeds[27], int nthreads, float *ff);
+
 /* ---- integrator: src/scheme/move/dev/vv.h:1-10 + body force + wrap ------ */
 void orc_integrate(const int L[3], int n, float dt, float mass, const float accel[3],
                    const float *ff, float *pp);

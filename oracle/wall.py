@@ -210,12 +210,15 @@ def integrate_wall(L, pp, ff, dt, sdf=None, mass=1.0, accel=(0.0, 0.0, 0.0), wv=
 
 
 def flu_wall_step(L, pp, params, seeds28, dt, sdf, wpp, wstarts, mass=1.0, accel=(0.0, 0.0, 0.0), nthreads=1,
-                  wv=None):
+                  wv=None, half=False):
     """One wall-bounded solvent step: clist, DPD forces, wall forces, kick-drift,
-    bounce-back, wrap.  Returns (cell-ordered new pp, solvent pairs, wall pairs)."""
+    bounce-back, wrap.  Returns (cell-ordered new pp, solvent pairs, wall pairs).
+    half=True: the solvent forces by the reference's half-shell traversal
+    (the CPU baseline; not a checker)."""
     from . import oracle as O
     starts, counts, srt, _ = O.clist_build(L, pp)
-    ff, npair = O.dpd_forces(L, srt, starts, counts, params, seeds28[:27], nthreads=nthreads)
+    forces = O.dpd_forces_half if half else O.dpd_forces
+    ff, npair = forces(L, srt, starts, counts, params, seeds28[:27], nthreads=nthreads)
     nw = 0
     if sdf is not None:
         ff, nw = wall_forces(L, srt, sdf, wpp, wstarts, params, seeds28[27], ff, wv)

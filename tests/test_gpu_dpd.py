@@ -248,3 +248,60 @@ def test_solver_fast_mode_conservation(ctx):
     # GPU FP64 diagnostics equal the oracle's on the same state
     ctx_out = U.diag_part(ctx, dev(got))
     assert np.allclose(ctx_out, dg, rtol=1e-12, atol=1e-9)
+
+
+def test_graph_replay_equals_eager_steps(ctx):
+    """udx_flu_set_graph: step pairs replayed from a captured graph give the
+    eager steps' state bit for bit, over several runs (odd and even step
+    counts, a run after new particles), with eager work and host
+    synchronisation between the replays; pair counts equal."""
+    L, rho = (12, 10, 14), 5
+    params = (25.0, 4.5, O.dpd_sigma(1.0, 4.5, 1e-3), 0.25)
+    pp = random_box(L, rho, 21)
+    seeds = O.seeds_table(40)
+    runs = [7, 2, 6, 1, 9]
+    res = {}
+    for graph in (False, True):
+        flu = U.Flu(ctx, L, len(pp), U.PairDpd(*params), 1e-3, 1.0, (0.3, 0.0, 0.0))
+        flu.graph(graph)
+        d = dev(pp)
+        flu.set_pp(d)
+        out, k = [], 0
+        for m in runs:
+            flu.run(m, seeds[k:k + m])
+            k += m
+            o = torch.zeros_like(d)
+            flu.get_pp(o)
+            _ = torch.ones(1000, device="cuda").sum().item()     # eager work between replays
+            out.append((host(o), flu.pairs(reset=True)))
+        res[graph] = out
+        flu.close()
+    for r, ((a, pa), (b, pb)) in enumerate(zip(res[False], res[True])):
+        assert pa == pb, "run %d" % r
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), "run %d" % r
+
+
+@pytest.mark.parametrize("case", CASES[:3])
+def test_stress_bit_exact(ctx, case):
+    """udx_fluforces_apply_stress: forces and per-particle stresses equal the
+    oracle's bit for bit (added to nonzero arrays)"""
+    L, rho, a, g, kBT, dt, spow = case
+    say("stress", case)
+    pp = random_box(L, rho, 17)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    seeds = O.seeds_table(3)[2]
+    par = (a, g, O.dpd_sigma(kBT, g, dt), spow)
+    f_ref, s_ref, np_ref = O.dpd_forces_stress(L, srt, starts, counts, par, seeds)
+    rng = np.random.default_rng(2)
+    f0 = rng.normal(0, 1, (len(srt), 3)).astype(np.float32)
+    s0 = rng.normal(0, 1, (len(srt), 6)).astype(np.float32)
+    f = U.FluForces(ctx, L, len(srt))
+    ff, ss = dev(f0), dev(s0)
+    sd = dev(starts, torch.int32)
+    f.prepare(dev(srt))
+    f.apply_stress(U.PairDpd(*par), U.C.c_void_p(sd.data_ptr()), ff, ss, seeds=seeds)
+    got_f, got_s = host(ff), host(ss)
+    assert f.pairs() == np_ref // 2
+    f.close()
+    assert np.array_equal(got_f.view(np.uint32), (f0 + f_ref).view(np.uint32))
+    assert np.array_equal(got_s.view(np.uint32), (s0 + s_ref).view(np.uint32))

@@ -36,6 +36,14 @@ def host(t):
     return t.cpu().numpy()
 
 
+def inside(r, L):
+    """float32 positions in [-L/2, L/2): a product rounded onto +L/2 is
+    wrapped (the single-box solver would drop it, the ensemble wraps it)"""
+    r = np.asarray(r, np.float32)
+    h = np.asarray(L, np.float32) / 2
+    return np.where(r >= h, r - np.asarray(L, np.float32), r).astype(np.float32)
+
+
 def make_boxes(nbox, seed, edges=(3, 12), dt=1e-3):
     """uneven boxes: edges in [3, 12], density in {3, 5, 10}, gamma in
     [4, 10], kBT in [0.04, 1]"""
@@ -48,7 +56,7 @@ def make_boxes(nbox, seed, edges=(3, 12), dt=1e-3):
         kBT = float(rng.uniform(0.04, 1.0))
         n = rho * L[0] * L[1] * L[2]
         pp = np.zeros((n, 6), np.float32)
-        pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+        pp[:, :3] = inside(rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L), L)
         pp[:, 3:] = rng.normal(0, np.sqrt(kBT), (n, 3)).astype(np.float32)
         par = (25.0, g, O.dpd_sigma(kBT, g, dt), 0.25)
         boxes.append(dict(L=L, n=n, params=U.PairDpd(*par), dt=dt, mass=1.0, accel=(0.1 * b, 0.0, 0.0),
@@ -196,7 +204,7 @@ def c5_boxes(seed=2024, nbox=64, dt=1e-3):
         g, kBT = float(rng.uniform(4, 10)), float(rng.uniform(0.04, 1.0))
         n = rho * L[0] * L[1] * L[2]
         pp = np.zeros((n, 6), np.float32)
-        pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+        pp[:, :3] = inside(rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L), L)
         pp[:, 3:] = rng.normal(0, np.sqrt(kBT), (n, 3)).astype(np.float32)
         par = (25.0, g, O.dpd_sigma(kBT, g, dt), 0.25)
         boxes.append(dict(L=L, n=n, params=U.PairDpd(*par), dt=dt, mass=1.0, accel=(0.0, 0.0, 0.0), par=par))

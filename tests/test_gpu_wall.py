@@ -234,10 +234,12 @@ def test_repulse_tolerance(ctx):
 
 @pytest.mark.parametrize("L,h,rho,accel", [((16, 12, 24), 8.0, 4, (0.5, 0.0, 0.0)),
                                            ((12, 10, 16), 5.0, 10, (0.2, 0.1, 0.0))])
-def test_wall_solver_trajectory_bit_exact(ctx, L, h, rho, accel):
+@pytest.mark.parametrize("graph", [False, True])
+def test_wall_solver_trajectory_bit_exact(ctx, L, h, rho, accel, graph):
     """clist + solvent DPD + wall DPD + body force + kick-drift + bounce-back
-    + wrap over 20 steps in exact mode: equal to the oracle bit for bit."""
-    say("wall trajectory", L, h, rho)
+    + wrap over 20 steps in exact mode: equal to the oracle bit for bit, also
+    when the steps are replayed from a captured graph."""
+    say("wall trajectory", L, h, rho, "graph", graph)
     nsteps = 20
     dt = 1e-3
     s = Setup(ctx, L, h, rho, 9)
@@ -253,6 +255,7 @@ def test_wall_solver_trajectory_bit_exact(ctx, L, h, rho, accel):
         pairs_ref += npair
         wpairs_ref += nw
     flu = U.Flu(ctx, L, len(s.solvent), U.PairDpd(*params), dt, 1.0, accel, exact=True)
+    flu.graph(graph)
     ppd = dev(s.solvent)
     flu.set_pp(ppd)
     flu.set_wall(s.w)

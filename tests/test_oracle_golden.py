@@ -286,3 +286,65 @@ def test_wrapped_pair_noise_matches_fragment_lists(L):
             want[i] += par[2] * float(xi) * (1 - rr) ** 0.25 * d / rr
     scale = np.sqrt((want ** 2).sum(1).mean())
     assert np.max(np.linalg.norm(ff - want, axis=1)) <= 1e-5 * scale
+
+
+@pytest.mark.parametrize("L,rho", [((9, 8, 7), 5), ((3, 4, 3), 3)])
+def test_half_shell_traversal_matches_the_gather(L, rho):
+    """The CPU baseline's force pass (the reference's half shell with
+    reactions + one-sided wrapped pairs) against the oracle's full-shell
+    gather: the same pairs, forces within FP32 summation order."""
+    rng = np.random.default_rng(4)
+    n = rho * int(np.prod(L))
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, :3] = (rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)).astype(np.float32)
+    pp[:, 3:] = rng.normal(0, 1, (n, 3)).astype(np.float32)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    par = (25.0, 4.5, O.dpd_sigma(1.0, 4.5, 1e-3), 0.25)
+    seeds = O.seeds_table(2)[1]
+    f1, n1 = O.dpd_forces(L, srt, starts, counts, par, seeds)
+    f2, n2 = O.dpd_forces_half(L, srt, starts, counts, par, seeds, nthreads=3)
+    assert n1 == n2
+    mag = np.linalg.norm(f1.astype(np.float64), axis=1)
+    err = np.linalg.norm(f2.astype(np.float64) - f1, axis=1) / np.maximum(mag, 1e-2 * np.sqrt((mag ** 2).mean()))
+    assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("L", [(4, 5, 6), (3, 3, 4)])
+def test_stress_brute_force(L):
+    """Per-particle stress (src/pair/dev.h:87-100: s_ab = 1/2 f_a dr_b summed
+    over the particle's pairs, both particles of a pair getting the same)
+    against all-pairs minimum image in float64 (xi = 0); the forces equal
+    the force-only gather bit for bit; tr s = 1/2 sum f0 r."""
+    rng = np.random.default_rng(31 + sum(L))
+    n = 4 * int(np.prod(L))
+    pp = np.zeros((n, 6), np.float32)
+    pp[:, :3] = rng.uniform(-0.5, 0.5, (n, 3)) * np.array(L)
+    pp[:, 3:] = rng.normal(0, 1, (n, 3))
+    starts, counts, out, _ = O.clist_build(L, pp)
+    par = (25.0, 4.5, 0.0, 0.25)
+    z = np.zeros(27, np.float32)
+    ff, ss, npair = O.dpd_forces_stress(L, out, starts, counts, par, z)
+    f2, n2 = O.dpd_forces(L, out, starts, counts, par, z)
+    assert npair == n2 and np.array_equal(ff, f2)
+    r = out[:, :3].astype(np.float64)
+    v = out[:, 3:].astype(np.float64)
+    want = np.zeros((n, 6))
+    tr = np.zeros(n)
+    for i in range(n):
+        d = r[i] - r
+        d -= np.array(L) * np.round(d / np.array(L))
+        r2 = (d * d).sum(1)
+        m = (r2 < 1) & (np.arange(n) != i) & (r2 >= 1e-6)
+        rr = np.sqrt(r2[m])
+        e = d[m] / rr[:, None]
+        ev = (e * (v[i] - v[m])).sum(1)
+        w = (1 - rr) ** par[3]
+        f0 = -par[1] * w * w * ev + par[0] * (1 - rr)
+        f = f0[:, None] * e
+        dm = d[m]
+        want[i] = 0.5 * np.array([(f[:, 0] * dm[:, 0]).sum(), (f[:, 0] * dm[:, 1]).sum(), (f[:, 0] * dm[:, 2]).sum(),
+                                  (f[:, 1] * dm[:, 1]).sum(), (f[:, 1] * dm[:, 2]).sum(), (f[:, 2] * dm[:, 2]).sum()])
+        tr[i] = 0.5 * (f0 * rr).sum()
+    scale = np.abs(want).max()
+    assert np.abs(ss - want).max() <= 1e-5 * scale
+    assert np.abs(ss[:, 0] + ss[:, 3] + ss[:, 5] - tr).max() <= 1e-5 * scale
