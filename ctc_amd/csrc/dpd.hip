@@ -5,16 +5,20 @@
 // one-sided forces of pairs across the periodic boundary with their own RNG
 // (src/fluforces/halo/dev/main.h:7-83).  Here one pass gathers the full shell
 // of each particle in a fixed order, so forces are deterministic and need no
-// atomics, and each wrapped pair is computed exactly like the reference's
-// halo: the local particle shifted by -L d into the neighbour frame and the
-// noise drawn from the fragment's trunk with packed-list positions.
+// atomics: first the pairs inside the box, then (only for particles in a
+// boundary cell of a periodic box) the pairs across the boundary, each
+// computed exactly like the reference's halo: the local particle shifted by
+// -L d into the neighbour frame and the noise drawn from the fragment's trunk
+// with packed-list positions.
 //
 // Kernel shape: one thread per cell-ordered particle.  A candidate pass walks
 // the 9 rows (dz, dy) of the 27-cell shell, each row a contiguous run of at
-// most 3 cells (two runs where x wraps), tests the distance and appends hits
-// to a per-lane queue in LDS.  When a queue fills (and at the end) the wave
-// drains the queues together, so the expensive pair body (RNG + force) runs
-// with nearly all lanes active instead of the ~15% that pass the cutoff.
+// most 3 cells cut to the particle's x window, tests the distance and appends
+// hits to a per-lane queue in LDS.  When a queue fills (and at the end of a
+// pass) the wave drains the queues together, so the expensive pair body (RNG
+// + force) runs with nearly all lanes active instead of the ~25% of the
+// candidates that pass the cutoff.
+#include <type_traits>
 #include <vector>
 
 #include "udx_dev.h"
@@ -26,7 +30,7 @@ namespace {
 
 // 128 threads: measured 5% faster per step than 256 on the 96^3, density-10 box
 constexpr int DPD_THREADS = 128;
-constexpr int QCAP = 24;   // queued pairs per lane before a drain
+constexpr int QCAP = 16;   // queued pairs per lane (a power of two)
 // staged candidates per wave and row (320 for batches of small boxes measured
 // 4% slower than 160: more staging loads, fewer blocks per CU)
 constexpr int SCAP = 160, SCAP_BATCH = 160;
@@ -98,26 +102,26 @@ __device__ __forceinline__ DpdP dpd_view(const DpdK &A) {
                 A.par};
 }
 
-// x-runs of row (dy, dz) for a particle in column cx: seg 0 and seg 1 in
-// ascending dx order (dx = -1, 0, +1 across the wrap)
-__device__ __forceinline__ bool xrun(int sg, int cx, int Lx, int periodic, int *x0, int *x1, int *wx) {
+// Runs of row (dy, dz) of a particle in column cx of a periodic box, in
+// ascending dx order: seg 0 and seg 1 (two runs where x wraps, Lx >= 3);
+// wx is the periodic shift of the run's cells seen from the particle.
+__device__ __forceinline__ bool xrun(int sg, int cx, int Lx, int *x0, int *x1, int *wx) {
     if (sg == 0) {
-        if (periodic && cx == 0) { *x0 = *x1 = Lx - 1; *wx = -1; return true; }
-        *x0 = cx > 0 ? cx - 1 : 0;
+        if (cx == 0) { *x0 = *x1 = Lx - 1; *wx = -1; return true; }
+        *x0 = cx - 1;
         *x1 = cx + 1 < Lx ? cx + 1 : Lx - 1;
         *wx = 0;
         return true;
     }
-    if (periodic && cx == 0) { *x0 = 0; *x1 = Lx > 1 ? 1 : 0; *wx = 0; return true; }
-    if (periodic && cx == Lx - 1) { *x0 = *x1 = 0; *wx = 1; return true; }
+    if (cx == 0) { *x0 = 0; *x1 = 1; *wx = 0; return true; }
+    if (cx == Lx - 1) { *x0 = *x1 = 0; *wx = 1; return true; }
     return false;
 }
 
-__device__ __forceinline__ int wrap_dir(int c, int L) { return c < 0 ? -1 : (c >= L ? 1 : 0); }
-
 // Queue entry of a candidate that passed the cutoff: the neighbour's index j
 // (27 bits) and the periodic wrap w of its cell seen from the home cell,
-// packed as (wx + 1) + 3 (wy + 1) + 9 (wz + 1); 13 means no wrap.
+// packed as (wx + 1) + 3 (wy + 1) + 9 (wz + 1); 13 means no wrap.  Entries of
+// the bulk pass carry j only.
 constexpr unsigned W_NONE = 13u;
 
 __device__ __forceinline__ unsigned wcode(int wx, int wy, int wz) {
@@ -130,41 +134,8 @@ struct PairIn {
     float dx, dy, dz, r2, dvx, dvy, dvz, x;
 };
 
-__device__ __forceinline__ PairIn pair_in(const DpdK &A, const DpdP &P, unsigned ent, int i, const float4 &pi,
-                                          const float4 &vi, int cx, int cy, int cz, int istart, float seed_bulk) {
-    const int j = (int)(ent & 0x7FFFFFFu);
-    const unsigned w = ent >> 27;
-    const float4 pj = P.p(j), vj = P.v(j);
-    float ax = pi.x, ay = pi.y, az = pi.z;
-    // bulk pair: mean0var1ii(seed, max, min) (src/fluforces/bulk/dev/main.h:32-38)
-    int u = (i > j ? i : j) - P.i0, v = (i > j ? j : i) - P.i0;
-    float seed = seed_bulk;
-    if (w != W_NONE) {
-        // pair across the periodic boundary: the local particle is moved into
-        // the neighbour frame (src/fluforces/halo/dev/main.h:40-48) and the
-        // noise is mean0var1uu(seed_d, ...) of fragment d = w with positions
-        // in the packed lists (src/fluforces/halo/dev/main.h:7-12,
-        // src/exch/flu/dev/pack.h:2-40)
-        const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
-        const int wx = (int)(w % 3u) - 1, wy = (int)((w / 3u) % 3u) - 1, wz = (int)(w / 9u) - 1;
-        ax = pi.x - (float)(wx * Lx);
-        ay = pi.y - (float)(wy * Ly);
-        az = pi.z - (float)(wz * Lz);
-        const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
-        const int aid = P.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] - P.fss[A.foff[fid]] + (i - istart);
-        int jx = 0, jy = 0, jz = 0;
-        cell_coord(pj.x, Lx, &jx);
-        cell_coord(pj.y, Ly, &jy);
-        cell_coord(pj.z, Lz, &jz);
-        const int jstart = P.starts[jx + Lx * (jy + Ly * jz)];
-        const int bid = P.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] - P.fss[A.foff[anti]] +
-                        (j - jstart);
-        const bool mask = fid < anti;
-        u = mask ? aid : bid;
-        v = mask ? bid : aid;
-        seed = P.seeds[fid];
-    }
-    PairIn o;
+__device__ __forceinline__ void pair_kin(PairIn &o, float ax, float ay, float az, const float4 &vi, const float4 &pj,
+                                         const float4 &vj) {
     o.dx = ax - pj.x;
     o.dy = ay - pj.y;
     o.dz = az - pj.z;
@@ -172,7 +143,50 @@ __device__ __forceinline__ PairIn pair_in(const DpdK &A, const DpdP &P, unsigned
     o.dvx = vi.x - vj.x;
     o.dvy = vi.y - vj.y;
     o.dvz = vi.z - vj.z;
-    o.x = seed - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
+}
+
+// a pair inside the box: noise mean0var1ii(seed, max, min)
+// (src/fluforces/bulk/dev/main.h:32-38)
+__device__ __forceinline__ PairIn pair_in_bulk(const DpdP &P, unsigned ent, int i, const float4 &pi,
+                                               const float4 &vi, float seed_bulk) {
+    const int j = (int)(ent & 0x7FFFFFFu);
+    const float4 pj = P.p(j), vj = P.v(j);
+    const int u = (i > j ? i : j) - P.i0, v = (i > j ? j : i) - P.i0;
+    PairIn o;
+    pair_kin(o, pi.x, pi.y, pi.z, vi, pj, vj);
+    o.x = seed_bulk - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
+    return o;
+}
+
+// a pair across the periodic boundary: the local particle is moved into the
+// neighbour frame (src/fluforces/halo/dev/main.h:40-48) and the noise is
+// mean0var1uu(seed_d, ...) of fragment d = w with positions in the packed
+// lists (src/fluforces/halo/dev/main.h:7-12, src/exch/flu/dev/pack.h:2-40).
+// An entry with w = W_NONE (an idle slot) is evaluated as a bulk pair.
+__device__ __forceinline__ PairIn pair_in_halo(const DpdK &A, const DpdP &P, unsigned ent, int i, const float4 &pi,
+                                               const float4 &vi, int cx, int cy, int cz, int istart,
+                                               float seed_bulk) {
+    const unsigned w = ent >> 27;
+    if (w == W_NONE) return pair_in_bulk(P, ent, i, pi, vi, seed_bulk);
+    const int j = (int)(ent & 0x7FFFFFFu);
+    const float4 pj = P.p(j), vj = P.v(j);
+    const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
+    const int wx = (int)(w % 3u) - 1, wy = (int)((w / 3u) % 3u) - 1, wz = (int)(w / 9u) - 1;
+    const float ax = pi.x - (float)(wx * Lx), ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
+    const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
+    const int aid = P.fss[A.foff[fid] + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] - P.fss[A.foff[fid]] + (i - istart);
+    int jx = 0, jy = 0, jz = 0;
+    cell_coord(pj.x, Lx, &jx);
+    cell_coord(pj.y, Ly, &jy);
+    cell_coord(pj.z, Lz, &jz);
+    const int jstart = P.starts[jx + Lx * (jy + Ly * jz)];
+    const int bid =
+        P.fss[A.foff[anti] + frag_cell(-wx, -wy, -wz, jx, jy, jz, Lx, Ly)] - P.fss[A.foff[anti]] + (j - jstart);
+    const bool mask = fid < anti;
+    const int u = mask ? aid : bid, v = mask ? bid : aid;
+    PairIn o;
+    pair_kin(o, ax, ay, az, vi, pj, vj);
+    o.x = P.seeds[fid] - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
     return o;
 }
 
@@ -228,14 +242,26 @@ __device__ __forceinline__ void run_range(const DpdP &P, int base, int x0, int x
     *n = P.bstarts[sb + (int)hi + 1] - *j;
 }
 
+template <bool B>
+using PassTag = std::integral_constant<bool, B>;
+
 // The force pass of one block of NT consecutive particles starting at i
-// = first + threadIdx.x; QC queued pairs per lane before a drain.
+// = first + threadIdx.x; QC (a power of two) queued pairs per lane.
+//
+// Bulk pass: the 9 rows (dz, dy) of the 27-cell shell clipped to the box,
+// each a contiguous run of at most 3 cells cut to the particle's x window.
+// The lanes are consecutive cell-ordered particles, so the union of the
+// wave's runs of a row is one index range: staged in LDS with coalesced
+// loads (or read from global memory when it does not fit) and tested four
+// candidates per iteration.  Halo pass (periodic boxes, only waves with a
+// particle in a boundary cell): the wrapped parts of the 9 rows, read from
+// global memory.  Each lane sums its pairs in candidate order, bulk first.
 template <int WR, bool EXACT, int NT, int QC, int SC, bool STRESS = false>
 __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
+    static_assert(QC >= 8 && (QC & (QC - 1)) == 0, "the queue length is a power of two");
     __shared__ unsigned q[QC * NT];
     __shared__ float4 sbuf[NT / 64][SC + 4];
     __shared__ unsigned long long lpairs[NT / 64];
-    constexpr int DPD_THREADS = NT, QCAP = QC;
     const int tid = threadIdx.x;
     const DpdP P = dpd_view(A);
     const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
@@ -260,137 +286,114 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     float fx = 0.f, fy = 0.f, fz = 0.f;
     // stress 1/2 f_a dr_b, (xx, xy, xz, yy, yz, zz) (src/pair/dev.h:87-100)
     float sxx = 0.f, sxy = 0.f, sxz = 0.f, syy = 0.f, syz = 0.f, szz = 0.f;
-    int npair = 0, fill = 0, head = 0;
-    unsigned *qlane = q + tid;
+    int npair = 0;
 
-    // Each lane's queue is a ring of QCAP entries (head, fill) consumed in
-    // queue order (= candidate order), two pairs per lane per step so two
-    // noise chains share each round-toward-zero window.  When a lane runs
-    // out of room the wave takes DRAIN_STEPS steps only: lanes fill their
-    // queues at very different rates (a lane near the -z face of its cell
-    // hits most in the dz = -1 rows), and emptying every queue at that point
-    // would run the expensive pair body with most lanes idle.
-    // An idle slot evaluates a discarded pair with an in-bounds particle
-    // (i itself, or the box's first particle for a lane past the end).
+    // Each lane's queue is a ring of QC LDS slots (entry k at slot k mod QC):
+    // head is the next entry to evaluate, tail the next free slot.  A pair
+    // step evaluates the next two entries of every lane, so two noise chains
+    // share each round-toward-zero window.  When a lane runs out of room the
+    // wave takes DRAIN_STEPS steps only: lanes fill their queues at very
+    // different rates (a lane near the -z face of its cell hits most in the
+    // dz = -1 rows), and emptying every queue at that point would run the
+    // expensive pair body with most lanes idle.  An idle slot evaluates a
+    // discarded pair with an in-bounds particle (i itself, or the box's first
+    // particle for a lane past the end).
+    // head and tail count in units of STEP bytes (one slot of every lane), so
+    // the LDS address of an entry is (count & QMASK) | (lane address)
+    constexpr unsigned STEP = 4u * NT, QMASK = (unsigned)(QC - 1) * STEP;
+    unsigned head = 0, tail = 0;
+    const unsigned qlane = (unsigned)(tid * 4);
+    auto slot = [&](unsigned k) { return (unsigned *)((char *)q + ((k & QMASK) | qlane)); };
     const unsigned idle = (unsigned)(active ? i : P.i0) | (W_NONE << 27);
-    auto ring = [&](int k) {
-        const int s = head + k;
-        return s >= QCAP ? s - QCAP : s;
+    auto add = [&](bool v, const float3 &f, const PairIn &pq) {
+        if (v) {
+            fx += f.x;
+            fy += f.y;
+            fz += f.z;
+            if (STRESS) {
+                sxx += 0.5f * f.x * pq.dx;
+                sxy += 0.5f * f.x * pq.dy;
+                sxz += 0.5f * f.x * pq.dz;
+                syy += 0.5f * f.y * pq.dy;
+                syz += 0.5f * f.y * pq.dz;
+                szz += 0.5f * f.z * pq.dz;
+            }
+        }
     };
-    auto drain_step = [&]() {
-        const bool va = fill > 0, vb = fill > 1;
-        const unsigned ea = va ? qlane[ring(0) * DPD_THREADS] : idle;
-        const unsigned eb = vb ? qlane[ring(1) * DPD_THREADS] : idle;
-        PairIn a = pair_in(A, P, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
-        PairIn b = pair_in(A, P, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+    auto pair_step = [&](auto pass) {
+        constexpr bool BULK = decltype(pass)::value;
+        const unsigned fl = tail - head;
+        const bool va = fl > 0u, vb = fl > STEP;
+        const unsigned ea = va ? *slot(head) : idle;
+        const unsigned eb = vb ? *slot(head + STEP) : idle;
+        PairIn a, b;
+        if (BULK) {
+            a = pair_in_bulk(P, ea, i, pi, vi, seed_bulk);
+            b = pair_in_bulk(P, eb, i, pi, vi, seed_bulk);
+        } else {
+            a = pair_in_halo(A, P, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+            b = pair_in_halo(A, P, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+        }
         float xa = a.x, xb = b.x;
         logistic18x2(&xa, &xb);
         const float3 f1 = dpd_pair<WR, EXACT>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
         const float3 f2 = dpd_pair<WR, EXACT>(P.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
-        fx += va ? f1.x : 0.f;
-        fy += va ? f1.y : 0.f;
-        fz += va ? f1.z : 0.f;
-        fx += vb ? f2.x : 0.f;
-        fy += vb ? f2.y : 0.f;
-        fz += vb ? f2.z : 0.f;
-        if (STRESS) {
-            auto add = [&](bool v, const float3 &f, const PairIn &q) {
-                sxx += v ? 0.5f * f.x * q.dx : 0.f;
-                sxy += v ? 0.5f * f.x * q.dy : 0.f;
-                sxz += v ? 0.5f * f.x * q.dz : 0.f;
-                syy += v ? 0.5f * f.y * q.dy : 0.f;
-                syz += v ? 0.5f * f.y * q.dz : 0.f;
-                szz += v ? 0.5f * f.z * q.dz : 0.f;
-            };
-            add(va, f1, a);
-            add(vb, f2, b);
-        }
-        const int took = fill < 2 ? fill : 2;
-        head = ring(took);
-        fill -= took;
-        npair += took;
+        add(va, f1, a);
+        add(vb, f2, b);
+        const unsigned took = fl < 2u * STEP ? fl : 2u * STEP;
+        head += took;
+        npair += (int)(took / STEP);
     };
     // room for `need` more entries in every lane (need <= 2 DRAIN_STEPS)
-    auto make_room = [&](int need) {
-        if (wave_any(fill + need > QCAP))
+    auto make_room = [&](auto pass, unsigned need) {
+        if (wave_any(tail - head + need * STEP > (unsigned)QC * STEP))
 #pragma unroll 1
-            for (int k = 0; k < DRAIN_STEPS; k++) drain_step();
+            for (int k = 0; k < DRAIN_STEPS; k++) pair_step(pass);
     };
-    auto drain = [&]() {
-        while (wave_any(fill > 0)) drain_step();
+    auto drain = [&](auto pass) {
+        while (wave_any(tail != head)) pair_step(pass);
     };
-
-    // the write slot head + fill (mod QCAP) is kept as an LDS pointer: a
-    // drain moves head and fill together and leaves it in place
-    unsigned *wq = qlane;
-    unsigned *const qend = qlane + QCAP * DPD_THREADS;
-    auto push = [&](bool hit, int j, unsigned w) {
-        if (hit) {
-            *wq = (unsigned)j | (w << 27);
-            wq += DPD_THREADS;
-            if (wq == qend) wq = qlane;
-            fill++;
-        }
+    // the slot at tail is free whenever there is room (make_room first)
+    auto push = [&](bool hit, unsigned e) {
+        unsigned *const w = slot(tail);
+        if (hit) *w = e;
+        tail += hit ? STEP : 0u;
     };
 
-    // Candidate pass, one row (dz, dy) of the 27-cell shell at a time: a row
-    // is one run of up to 3 cells, or two runs where x wraps; the loop shape
-    // is the same for every lane.  The union of the wave's runs of a row is
-    // one contiguous index range when no lane's row wraps (the lanes are
-    // consecutive cell-ordered particles): it is staged in LDS with coalesced
-    // loads and tested four candidates per iteration; a wave with wrapped runs
-    // is staged the same way when their union fits; otherwise the wave reads
-    // its candidates from global memory.
+    // ---- bulk pass: rows 0-8; only the particle's own row (4) can hold i
     float4 *sb = sbuf[tid >> 6];
-    for (int row = 0; row < 9; row++) {
+    auto bulk_row = [&](int row, auto self) {
+        constexpr bool SELF = decltype(self)::value;
         const int dz = row / 3 - 1, dy = row % 3 - 1;
-        int yy = cy + dy, zz = cz + dz;
-        const int wy = yy < 0 ? -1 : (yy >= Ly ? 1 : 0), wz = zz < 0 ? -1 : (zz >= Lz ? 1 : 0);
-        const bool rowok = active && (P.periodic || (wy | wz) == 0);
-        yy -= wy * Ly;
-        zz -= wz * Lz;
-        const int base = Lx * (yy + Ly * zz);
-        int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
-        if (rowok) {
-            const float hw = row_halfwidth(dy, dz, ry, rz);
-            if (xrun(0, cx, Lx, P.periodic, &x0, &x1, &wx0))
-                run_range(P, base, x0, x1, pi.x - (float)(wx0 * Lx), hw, &j0, &n0);
-            if (xrun(1, cx, Lx, P.periodic, &x0, &x1, &wx1))
-                run_range(P, base, x0, x1, pi.x - (float)(wx1 * Lx), hw, &j1, &n1);
+        const int yy = cy + dy, zz = cz + dz;
+        int j0 = 0, n0 = 0;
+        if (active && yy >= 0 && yy < Ly && zz >= 0 && zz < Lz) {
+            const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < Lx ? cx + 1 : Lx - 1;
+            run_range(P, Lx * (yy + Ly * zz), x0, x1, pi.x, row_halfwidth(dy, dz, ry, rz), &j0, &n0);
         }
-        const bool simple = !rowok || ((wy | wz | wx0) == 0 && n1 == 0);
-        const bool all_simple = !wave_any(!simple);
         // the union of the wave's candidate ranges of this row
-        int lo = 0x7fffffff, hi = 0;
-        if (n0 > 0) {
-            lo = j0;
-            hi = j0 + n0;
-        }
-        if (n1 > 0) {
-            lo = min(lo, j1);
-            hi = max(hi, j1 + n1);
-        }
+        int lo = n0 > 0 ? j0 : 0x7fffffff, hi = n0 > 0 ? j0 + n0 : 0;
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
             lo = min(lo, __shfl_xor(lo, d, 64));
             hi = max(hi, __shfl_xor(hi, d, 64));
         }
-        const bool fits = hi > lo && hi - lo <= SC;
-        const bool lds_row = fits && all_simple;
-        if (lds_row) {
+        if (hi <= lo) return;
+        // candidate k of the lane's range is i itself (own row only)
+        const int kself = SELF ? i - j0 : -1;
+        if (hi - lo <= SC) {
             __builtin_amdgcn_wave_barrier();
             for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = P.p(lo + k);
             __builtin_amdgcn_wave_barrier();
             const float4 *sp = sb + (n0 > 0 ? j0 - lo : 0);
             for (int t = 0; wave_any(t < n0); t += 4) {
-                make_room(4);
+                make_room(PassTag<true>{}, 4u);
                 const float4 c0 = sp[t], c1 = sp[t + 1], c2 = sp[t + 2], c3 = sp[t + 3];
 #define UDX_TEST(C, U)                                                                             \
     {                                                                                              \
         const float ddx = pi.x - C.x, ddy = pi.y - C.y, ddz = pi.z - C.z;                          \
         const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));            \
-        const int j = j0 + t + U;                                                                  \
-        push(t + U < n0 && j != i && r2 < 1.0f, j, W_NONE);                                       \
+        push(t + U < n0 && (!SELF || t + U != kself) && r2 < 1.0f, (unsigned)(j0 + t + U));       \
     }
                 UDX_TEST(c0, 0)
                 UDX_TEST(c1, 1)
@@ -399,60 +402,69 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
 #undef UDX_TEST
             }
             __builtin_amdgcn_wave_barrier();
-        } else if (fits) {
-            // some lane's row wraps, but the wave's runs (a wrapped x run lies
-            // at the other end of the same x row; rows wrapped in y or z are
-            // shared by the lanes of one y row) still fit the staging buffer:
-            // staged as above, each candidate taking its run's shift
-            __builtin_amdgcn_wave_barrier();
-            for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = P.p(lo + k);
-            __builtin_amdgcn_wave_barrier();
-            const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
-            const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
-            const unsigned w0 = wcode(wx0, wy, wz), w1 = wcode(wx1, wy, wz);
-            const int len = n0 + n1;
-            const int o0 = j0 - lo, o1 = j1 - lo;
-            for (int t = 0; wave_any(t < len); t += 4) {
-                make_room(4);
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int tt = t + u;
-                    const bool s1 = tt >= n0;
-                    const int idx = s1 ? o1 + (tt - n0) : o0 + tt;
-                    const bool ok = tt < len;
-                    const float4 c = sb[ok ? idx : 0];
-                    const float ax = s1 ? ax1 : ax0;
-                    const float ddx = ax - c.x, ddy = ay - c.y, ddz = az - c.z;
-                    const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));
-                    const int j = s1 ? j1 + (tt - n0) : j0 + tt;
-                    push(ok && j != i && r2 < 1.0f, j, s1 ? w1 : w0);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
         } else {
+            for (int t = 0; wave_any(t < n0); t += 2) {
+                make_room(PassTag<true>{}, 2u);
+                const bool oka = t < n0, okb = t + 1 < n0;
+                const int ja = j0 + t, jb = j0 + t + 1;
+                const float4 pa = P.p(oka ? ja : P.i0), pb = P.p(okb ? jb : P.i0);
+                const float dxa = pi.x - pa.x, dya = pi.y - pa.y, dza = pi.z - pa.z;
+                const float dxb = pi.x - pb.x, dyb = pi.y - pb.y, dzb = pi.z - pb.z;
+                const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
+                const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
+                push(oka && (!SELF || t != kself) && r2a < 1.0f, (unsigned)ja);
+                push(okb && (!SELF || t + 1 != kself) && r2b < 1.0f, (unsigned)jb);
+            }
+        }
+    };
+    for (int row = 0; row < 4; row++) bulk_row(row, PassTag<false>{});
+    bulk_row(4, PassTag<true>{});
+    for (int row = 5; row < 9; row++) bulk_row(row, PassTag<false>{});
+    drain(PassTag<true>{});
+
+    // ---- halo pass: the cells across the periodic boundary
+    const bool edge =
+        active && P.periodic && (cx == 0 || cx == Lx - 1 || cy == 0 || cy == Ly - 1 || cz == 0 || cz == Lz - 1);
+    if (wave_any(edge)) {
+        for (int row = 0; row < 9; row++) {
+            const int dz = row / 3 - 1, dy = row % 3 - 1;
+            int yy = cy + dy, zz = cz + dz;
+            const int wy = yy < 0 ? -1 : (yy >= Ly ? 1 : 0), wz = zz < 0 ? -1 : (zz >= Lz ? 1 : 0);
+            yy -= wy * Ly;
+            zz -= wz * Lz;
+            const bool yzwrap = (wy | wz) != 0;
+            int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
+            if (edge) {
+                const float hw = row_halfwidth(dy, dz, ry, rz);
+                const int base = Lx * (yy + Ly * zz);
+                if (xrun(0, cx, Lx, &x0, &x1, &wx0) && (yzwrap || wx0 != 0))
+                    run_range(P, base, x0, x1, pi.x - (float)(wx0 * Lx), hw, &j0, &n0);
+                if (xrun(1, cx, Lx, &x0, &x1, &wx1) && (yzwrap || wx1 != 0))
+                    run_range(P, base, x0, x1, pi.x - (float)(wx1 * Lx), hw, &j1, &n1);
+            }
             const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
             const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
-            const unsigned w0 = wcode(wx0, wy, wz), w1 = wcode(wx1, wy, wz);
+            const unsigned w0 = wcode(wx0, wy, wz) << 27, w1 = wcode(wx1, wy, wz) << 27;
             const int len = n0 + n1;
             for (int t = 0; wave_any(t < len); t += 2) {
-                make_room(2);
+                make_room(PassTag<false>{}, 2u);
                 const int ta = t, tb = t + 1;
                 const bool oka = ta < len, okb = tb < len;
                 const int ja = ta >= n0 ? j1 + (ta - n0) : j0 + ta;
                 const int jb = tb >= n0 ? j1 + (tb - n0) : j0 + tb;
-                const float4 pa = P.p(oka ? ja : 0);
-                const float4 pb = P.p(okb ? jb : 0);
+                const float4 pa = P.p(oka ? ja : P.i0);
+                const float4 pb = P.p(okb ? jb : P.i0);
                 const float axa = ta >= n0 ? ax1 : ax0, axb = tb >= n0 ? ax1 : ax0;
                 const float dxa = axa - pa.x, dya = ay - pa.y, dza = az - pa.z;
                 const float dxb = axb - pb.x, dyb = ay - pb.y, dzb = az - pb.z;
                 const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
                 const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
-                push(oka && ja != i && r2a < 1.0f, ja, ta >= n0 ? w1 : w0);
-                push(okb && jb != i && r2b < 1.0f, jb, tb >= n0 ? w1 : w0);
+                push(oka && r2a < 1.0f, (unsigned)ja | (ta >= n0 ? w1 : w0));
+                push(okb && r2b < 1.0f, (unsigned)jb | (tb >= n0 ? w1 : w0));
             }
         }
+        drain(PassTag<false>{});
     }
-    drain();
 
     if (active) {
         if (A.ff_stride == 4) {
@@ -482,7 +494,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         __syncthreads();
         if (tid == 0) {
             unsigned long long s = 0;
-            for (int w = 0; w < DPD_THREADS / 64; w++) s += lpairs[w];
+            for (int w = 0; w < NT / 64; w++) s += lpairs[w];
             if (s) atomicAdd(A.pairs, s);
         }
     }
@@ -504,10 +516,10 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
 // Many boxes in one launch: block b belongs to the box k with
 // bblock[k] <= b < bblock[k + 1]; each box's blocks start at its first
 // particle, so no block spans two boxes.
+// (four waves per SIMD unless the queues' LDS allows only three blocks per CU)
 template <int WR, bool EXACT, int NT, int QC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_dpd_batch(const DpdK *boxes,
-                                                                                           const int *bblock,
-                                                                                           int nbox) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 4096 ? 3 : 4))) void k_dpd_batch(
+    const DpdK *boxes, const int *bblock, int nbox) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     int lo = 0, hi = nbox;
     while (hi - lo > 1) {
@@ -650,7 +662,7 @@ int dpd_forces(hipStream_t s, const DpdLaunch &a) { return dpd_forces_mode(s, a,
 
 // ---------------------------------------------------------------- batches
 namespace {
-constexpr int VAR_SHAPE[DPD_NVARIANT][2] = {{128, 24}, {128, 16}, {128, 20}, {128, 32}, {256, 16}, {256, 24}};
+constexpr int VAR_SHAPE[DPD_NVARIANT][2] = {{128, 16}, {128, 8}, {128, 32}, {256, 16}, {256, 8}, {256, 32}};
 
 
 template <int WR, bool EXACT, int NT, int QC>
@@ -665,7 +677,7 @@ int launch_batch(hipStream_t s, const DpdBatch &b) {
 
 template <int WR, bool EXACT>
 int launch_batch_wr(hipStream_t s, const DpdBatch &b) {
-    return launch_batch<WR, EXACT, 128, 24>(s, b);
+    return launch_batch<WR, EXACT, 128, 16>(s, b);
 }
 }  // namespace
 
@@ -720,12 +732,12 @@ int dpd_forces_batch(hipStream_t s, const DpdBatch &b, int mode, int variant) {
     }
     if (sp == 0.25f) {
         switch (variant) {
-            case 1: return launch_batch<WR_QUARTER, false, 128, 16>(s, b);
-            case 2: return launch_batch<WR_QUARTER, false, 128, 20>(s, b);
-            case 3: return launch_batch<WR_QUARTER, false, 128, 32>(s, b);
-            case 4: return launch_batch<WR_QUARTER, false, 256, 16>(s, b);
-            case 5: return launch_batch<WR_QUARTER, false, 256, 24>(s, b);
-            default: return launch_batch<WR_QUARTER, false, 128, 24>(s, b);
+            case 1: return launch_batch<WR_QUARTER, false, 128, 8>(s, b);
+            case 2: return launch_batch<WR_QUARTER, false, 128, 32>(s, b);
+            case 3: return launch_batch<WR_QUARTER, false, 256, 16>(s, b);
+            case 4: return launch_batch<WR_QUARTER, false, 256, 8>(s, b);
+            case 5: return launch_batch<WR_QUARTER, false, 256, 32>(s, b);
+            default: return launch_batch<WR_QUARTER, false, 128, 16>(s, b);
         }
     }
     if (sp == 0.5f) return launch_batch_wr<WR_HALF, false>(s, b);

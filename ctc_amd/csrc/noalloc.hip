@@ -72,6 +72,9 @@ extern "C" int udx_clist_build_noalloc(udx_ctx *ctx, int Lx, int Ly, int Lz, int
     UDX_CHECK(n <= UDX_NOALLOC_MAXP, UDX_ECAP,
               "udx_clist_build_noalloc: %d particles exceed the static working space (%d particles)", n,
               UDX_NOALLOC_MAXP);
+    UDX_CHECK(scan_scratch_size(NA_SLABS) <= NA_SCAN_INTS, UDX_ECAP,
+              "udx_clist_build_noalloc: the scan needs %d ints of working space, %ld reserved",
+              scan_scratch_size(NA_SLABS), NA_SCAN_INTS);
     int *base = nullptr;
     UDX_TRY(scratch_of(ctx->device, &base));
     ClistBufs c{};

@@ -90,7 +90,7 @@ __device__ __forceinline__ float low_discrepancy(float fu_lo, float fu, float fv
     float s = __builtin_fmaf(fu_lo, R_GOLD, fu * R_BRONZE);
     s = __builtin_fmaf(fv_lo, R_SILVER, s);
     s = __builtin_fmaf(fv, R_TIN, s);
-    return s - __builtin_floorf(s);
+    return __builtin_amdgcn_fractf(s);   // s - floor(s), exact for s >= 0
 }
 
 // src/math/rnd/dev.h:57-63
@@ -139,6 +139,18 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return ru > 0.f ? su : t;
 }
 
+// Correctly rounded reciprocal of r in [2^-24, 1]: the hardware reciprocal
+// (within 1 ulp) refined by one Newton step with fused multiply-adds.  The
+// result equals the IEEE quotient 1/r for every float of that range (checked
+// exhaustively on gfx950 against the double quotient rounded to float; the
+// library's self-check entry point repeats the check), at 3 instructions
+// instead of the 10 of a general division.
+__device__ __forceinline__ float rcp_rn(float r) {
+    const float y = __builtin_amdgcn_rcpf(r);
+    const float e = __builtin_fmaf(-r, y, 1.0f);
+    return __builtin_fmaf(y, e, y);
+}
+
 // w_R = (1 - r)^spow for the exponents the kernels specialise
 enum WrKind { WR_QUARTER = 0, WR_HALF = 1, WR_ONE = 2, WR_POW = 3 };
 
@@ -166,7 +178,7 @@ __device__ __forceinline__ float3 dpd_pair(const DpdPar &p, float dx, float dy, 
         ex = ey = ez = r = 0.0f;
     } else {
         r = sqrt_rn(r2);
-        const float invr = 1.0f / r;
+        const float invr = rcp_rn(r);   // r >= 1e-3
         ex = dx * invr; ey = dy * invr; ez = dz * invr;
     }
     float ev = __builtin_fmaf(ez, dvz, __builtin_fmaf(ex, dvx, ey * dvy));

@@ -76,6 +76,7 @@ _SIGS = {
     "udx_ctx_fin": (_I, [_VP]),
     "udx_ctx_sync": (_I, [_VP]),
     "udx_rnd_branch": (_I, [_VP, _F, _I, _VP, _VP, _I, _VP]),
+    "udx_selfcheck": (_I, [_VP, _I, _F, _F, C.POINTER(C.c_ulonglong)]),
     "udx_clist_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
     "udx_clist_fin": (_I, [_VP]),
     "udx_clist_build": (_I, [_VP, _I, _VP, _VP, C.POINTER(_I)]),
@@ -271,6 +272,18 @@ def dpd_params(a, g, kBT, dt, spow=0.25):
 def rnd_branch(ctx, seed, u, v, out, uu=False):
     """rnd::mean0var1ii/uu on the GPU for int32 device tensors u, v."""
     _ok(lib().udx_rnd_branch(ctx.h, seed, u.shape[0], _ptr(u), _ptr(v), 1 if uu else 0, _ptr(out)))
+
+
+CHECK_RCP, CHECK_SQRT, CHECK_FRACT, CHECK_RCP_ESTIMATE = 0, 1, 2, 3
+
+
+def selfcheck(ctx, what, lo, hi):
+    """floats x in [lo, hi] where the device routine `what` (CHECK_RCP,
+    CHECK_SQRT, CHECK_FRACT, CHECK_RCP_ESTIMATE) differs from the correctly
+    rounded result"""
+    v = C.c_ulonglong()
+    _ok(lib().udx_selfcheck(ctx.h, what, lo, hi, C.byref(v)))
+    return v.value
 
 
 # ---------------------------------------------------------------- context

@@ -53,6 +53,17 @@ int udx_ctx_sync(udx_ctx *ctx);
  *      uu != 0 selects the unsigned variant ---- */
 int udx_rnd_branch(udx_ctx *ctx, float seed, int n, const int *u, const int *v, int uu, float *out);
 
+/* ---- self-check of the device arithmetic (no counterpart in the reference):
+ *      over every float x in [lo, hi] (0 <= lo <= hi, fewer than 2^31 floats)
+ *      compares the library's device routine with the correctly rounded
+ *      result and returns the number of floats that differ (synchronises).
+ *      UDX_CHECK_RCP: the reciprocal of the pair force (for x in [2^-24, 1]);
+ *      UDX_CHECK_SQRT: its square root; UDX_CHECK_FRACT: x - floor(x) of the
+ *      branch generator; UDX_CHECK_RCP_ESTIMATE: the hardware reciprocal
+ *      estimate alone (not correctly rounded: shows that the check counts). ---- */
+enum { UDX_CHECK_RCP = 0, UDX_CHECK_SQRT = 1, UDX_CHECK_FRACT = 2, UDX_CHECK_RCP_ESTIMATE = 3 };
+int udx_selfcheck(udx_ctx *ctx, int what, float lo, float hi, unsigned long long *mismatches);
+
 /* ---- cell list: replaces clist_ini / clist_ini_map / clist_build /
  *      clists_get_ss / clists_get_cc (src/clist/imp.h:5-33) ----
  * Stable counting sort: particles are ordered by cell, ties keep their
@@ -110,11 +121,12 @@ int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int
  * Correctly rounded arithmetic (the mode setting does not apply). */
 int udx_fluforces_apply_stress(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
                                const float seeds[27], float *ff, float *ss);
-/* force kernel modes; both gather the full shell of each particle in a fixed
- * order (no atomics), so results are identical from run to run */
-#define UDX_MODE_FAST 0    /* default: hardware reciprocal and roots (1 ulp) */
-#define UDX_MODE_EXACT 1   /* correctly rounded division and roots: equal to
-                              the CPU restatement bit for bit (slower) */
+/* force kernel modes.  Every kernel gathers the full shell of each particle
+ * in a fixed order (no atomics) with correctly rounded reciprocals and roots,
+ * so both modes give the same bits, equal to the CPU restatement; the mode is
+ * kept so callers written against the two-mode interface keep working. */
+#define UDX_MODE_FAST 0    /* default */
+#define UDX_MODE_EXACT 1
 int udx_fluforces_set_mode(udx_fluforces *f, int mode);
 /* unordered pair interactions (r^2 < 1) counted by apply calls since the last reset */
 int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset);

@@ -382,7 +382,9 @@ int orc_pair_force(const orc_dpd *p, const float *a, const float *b, float rnd, 
 /* ======================================================================== */
 /* DPD forces: half-shell of src/fluforces/bulk/dev/main.h:23-97 plus the     */
 /* one-sided halo of src/fluforces/halo/dev/main.h:7-83, restated as one      */
-/* full-shell gather per particle in a fixed order.                          */
+/* full-shell gather per particle in a fixed order: the pairs inside the box  */
+/* first, then the pairs across the periodic boundary (as the reference adds  */
+/* the halo forces after the bulk's).                                        */
 /* ======================================================================== */
 typedef struct {
     const int *L, *starts, *fss, *fss_off;
@@ -411,6 +413,10 @@ static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, con
     float sacc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     (void)n;
     if (orc_cell_of(L, pi, ci) < 0) return 0;
+    /* pass 0: the cells inside the box (the bulk, src/fluforces/bulk);
+     * pass 1: the cells across the periodic boundary (the halo,
+     * src/fluforces/halo), each pass in (dz, dy, dx) order */
+    for (int pass = 0; pass < 2; pass++)
     for (int dz = -1; dz <= 1; dz++)
     for (int dy = -1; dy <= 1; dy++)
     for (int dx = -1; dx <= 1; dx++) {
@@ -423,6 +429,7 @@ static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, con
         }
         if (skip) continue;
         int wrapped = w[0] | w[1] | w[2];
+        if ((wrapped != 0) != (pass == 1)) continue;
         int cid = cj[0] + L[0] * (cj[1] + L[1] * cj[2]);
         /* the local particle is moved into the neighbour's frame
          * (src/fluforces/halo/dev/main.h:40-48) */

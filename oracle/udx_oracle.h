@@ -85,6 +85,9 @@ int orc_pair_force(const orc_dpd *p, const float *a, const float *b, float rnd, 
 
 /* ---- DPD forces (full-shell gather, defined order) ---------------------- */
 /* pp: cell-ordered AoS particles (r[3], v[3]); seeds[27] as above.
+ * Each particle sums its pairs in (dz, dy, dx) cell order and cell order
+ * inside a cell, first over the cells inside the box, then over the cells
+ * across the periodic boundary.
  * periodic=1: neighbour cells wrap, wrapped pairs use the halo RNG of
  * src/fluforces/halo/dev/main.h:7-12 with fragment positions;
  * periodic=0: cells outside the box are skipped (bulk-only unit harness,

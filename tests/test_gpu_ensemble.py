@@ -231,7 +231,12 @@ def test_c5_workload_batched_equals_per_box(ctx):
     for v in range(6):
         say("C5: tile variant %d" % v)
         got, pairs = run_ensemble(ctx, boxes, pps, nsteps, seeds, False, graph=(v % 2 == 0), variant=v)
-        assert pairs == total, "variant %d" % v
+        # Unordered pairs are ordered evaluations / 2.  A pair across the
+        # periodic boundary is evaluated by each side in its own frame, and
+        # the rounding of the shift can put it inside the cutoff on one side
+        # only, so a box's ordered count can be odd: the per-box counts are
+        # each rounded down, the batched total once.
+        assert 0 <= pairs - total <= nbox // 2, "variant %d: %d vs %d" % (v, pairs, total)
         for b in range(nbox):
             assert np.array_equal(got[off[b]:off[b + 1]].view(np.uint32), single[b].view(np.uint32)), \
                 "variant %d box %d" % (v, b)
