@@ -89,6 +89,7 @@ struct udx_fluforces {
     FragTables frag;
     Trunks trunks;
     unsigned long long *pairs_dev;
+    double *acc;        // FP64 force sums of the fast pass (3 per particle, kept zero between calls)
 };
 
 struct udx_sdf {
@@ -119,6 +120,7 @@ struct udx_flu {
     Box box;
     int n;
     float4 *pos[2], *vel[2], *ff;
+    double *acc;        // FP64 force sums of the fast pass (3 per particle, kept zero between steps)
     int cur;
     ClistBufs cl;
     FragTables frag;
@@ -311,7 +313,9 @@ int udx_fluforces_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_flufor
     if (!s) s = dev_alloc(&f->seeds_dev, 27);
     if (!s) s = dev_alloc(&f->pairs_dev, 1);
     if (!s) s = frag_alloc(&f->frag, f->box);
-    if (!s && hipMemset(f->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess) {
+    if (!s) s = dev_alloc(&f->acc, 3 * (size_t)(maxp > 0 ? maxp : 1));
+    if (!s && (hipMemset(f->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess ||
+               hipMemset(f->acc, 0, sizeof(double) * 3 * (size_t)(maxp > 0 ? maxp : 1)) != hipSuccess)) {
         set_error("udx_fluforces_ini: memset failed");
         s = UDX_EHIP;
     }
@@ -329,6 +333,7 @@ int udx_fluforces_fin(udx_fluforces *f) {
     dev_free(f->vel);
     dev_free(f->seeds_dev);
     dev_free(f->pairs_dev);
+    dev_free(f->acc);
     frag_free(&f->frag);
     delete f;
     return UDX_OK;
@@ -336,7 +341,7 @@ int udx_fluforces_fin(udx_fluforces *f) {
 
 int udx_fluforces_set_mode(udx_fluforces *f, int mode) {
     UDX_CHECK(f, UDX_EINVAL, "udx_fluforces_set_mode: null handle");
-    UDX_CHECK(mode >= UDX_MODE_FAST && mode <= UDX_MODE_EXACT, UDX_EINVAL, "udx_fluforces_set_mode: mode %d",
+    UDX_CHECK(mode >= UDX_MODE_FAST && mode <= UDX_MODE_HALF, UDX_EINVAL, "udx_fluforces_set_mode: mode %d",
               mode);
     f->mode = mode;
     return UDX_OK;
@@ -391,6 +396,7 @@ static int fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *s
     a.accumulate = 1;
     a.pairs = f->pairs_dev;
     a.ss = ss;
+    a.acc = f->acc;
     UDX_TRY(dpd_forces_mode(s, a, f->mode));
     // the seeds live on the host stack: wait for the copy before returning
     UDX_HIP(hipStreamSynchronize(s));
@@ -449,7 +455,7 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
     UDX_CHECK(L[0] >= 3 && L[1] >= 3 && L[2] >= 3, UDX_EINVAL,
               "udx_flu_ini: a periodic box needs at least 3 cells per side (got %d %d %d)", L[0], L[1], L[2]);
     UDX_CHECK(conf->maxp > 0 && conf->mass > 0, UDX_EINVAL, "udx_flu_ini: bad capacity or mass");
-    UDX_CHECK(conf->mode >= UDX_MODE_FAST && conf->mode <= UDX_MODE_EXACT, UDX_EINVAL, "udx_flu_ini: mode %d",
+    UDX_CHECK(conf->mode >= UDX_MODE_FAST && conf->mode <= UDX_MODE_HALF, UDX_EINVAL, "udx_flu_ini: mode %d",
               conf->mode);
     udx_flu *f = new udx_flu();
     f->ctx = ctx;
@@ -463,6 +469,11 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
         if (!s) s = dev_alloc(&f->vel[b], conf->maxp);
     }
     if (!s) s = dev_alloc(&f->ff, conf->maxp);
+    if (!s) s = dev_alloc(&f->acc, 3 * (size_t)conf->maxp);
+    if (!s && hipMemset(f->acc, 0, sizeof(double) * 3 * (size_t)conf->maxp) != hipSuccess) {
+        set_error("udx_flu_ini: memset failed");
+        s = UDX_EHIP;
+    }
     if (!s) s = clist_alloc(&f->cl, f->box, conf->maxp, SOLVENT_SUB);
     if (!s) s = frag_alloc(&f->frag, f->box);
     if (!s) s = dev_alloc(&f->pairs_dev, 2);
@@ -485,6 +496,7 @@ int udx_flu_fin(udx_flu *f) {
         dev_free(f->vel[b]);
     }
     dev_free(f->ff);
+    dev_free(f->acc);
     clist_free(&f->cl);
     frag_free(&f->frag);
     dev_free(f->seeds_dev);
@@ -602,6 +614,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     d.ff_stride = 4;
     d.accumulate = 0;
     d.pairs = f->pairs_dev;
+    d.acc = f->acc;
     UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
     if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
     const SdfField *sdf = nullptr;

@@ -50,6 +50,7 @@ struct DpdK {
     int ff_stride, accumulate;
     unsigned long long *pairs;
     float *ss;                // per-particle stress (6 floats, added) or null
+    double *acc;              // FP64 force accumulators (half-shell pass) or null
 };
 
 // What the force pass reads of a box, read once at kernel entry.  The
@@ -531,6 +532,356 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 40
     dpd_block<WR, EXACT, NT, QC, SCAP_BATCH>(A, A.i0 + (blk - bblock[lo]) * NT);
 }
 
+// ---------------------------------------------------------------- half shell
+// The production pass (UDX_MODE_FAST with FP64 accumulators): each pair
+// inside the box is evaluated once, by its higher-indexed particle, as the
+// reference's bulk kernel does (src/fluforces/bulk/dev/main.h:23-97), and
+// its force is added to one particle and subtracted from the other.
+//
+// Sums are exact: every pair force component is rounded to a multiple of
+// 2^-32 (nothing changes for |f| >= 2^-9, whose float ulp is at least that)
+// and added in FP64, where sums of such multiples stay exact while their
+// magnitude is below 2^21.  The sum is therefore independent of the order of
+// the additions (lanes, LDS and global atomics), the same from run to run,
+// and the float force is that sum rounded once.  The pair force itself is
+// antisymmetric bit for bit (negated differences give negated results), so
+// which particle of a pair evaluates it does not matter either.
+//
+// A wave owns 64 consecutive cell-ordered particles.  The half shell of a
+// particle is the part of its full shell with lower indices: the rows
+// (dz, dy) = (-1,-1), (-1,0), (-1,1), (0,-1) and its own row (0,0) up to
+// itself.  For each row the union of the wave's candidate runs is staged in
+// LDS (positions, velocities and FP64 reaction accumulators, in chunks of
+// HS_SC).  Candidates that pass the cutoff go to one queue per wave (slots
+// from ballots); whenever 128 are queued every lane evaluates two of them,
+// so the pair body runs with all lanes busy however unevenly the lanes hit.
+// The force goes to the home particle's LDS accumulator and, negated, to the
+// staged particle's; at the end of a chunk the staged reactions are added to
+// the global accumulators, at the end of the wave the home sums.  Pairs
+// across the periodic boundary keep the reference's one-sided halo noise
+// (the full-shell gather's halo pass) and add to the home sum only.
+constexpr int HS_SC = 128;    // staged candidates per wave and chunk of a row
+constexpr int HS_QN = 512;    // wave queue ring: entries owner lane | slot << 6
+constexpr int HS_HQC = 16;    // halo pass: queued pairs per lane (a power of two)
+
+__device__ __forceinline__ double quant32(float f) {
+    return (double)(__builtin_rintf(f * 4294967296.0f) * 2.3283064365386963e-10f);
+}
+
+struct HsWave {
+    float4 hp[64], hv[64];            // home particles, read by the lane that evaluates a pair
+    double hacc[3][64];               // home sums
+    union {
+        struct {
+            float4 sp[HS_SC + 4];     // staged positions (4 reads past a lane's run)
+            float4 sv[HS_SC];         // staged velocities
+        } st;
+        unsigned ring[HS_HQC * 64];   // halo pass: per-lane ring queues
+    } u;
+    double racc[3][HS_SC];            // reactions on the staged particles
+    unsigned short q[HS_QN];
+};
+
+template <class T>
+__device__ __forceinline__ void lds_add(T *p, T v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+__device__ __forceinline__ void gacc_add(UDX_GLOBAL double *p, double v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int WR, int NT>
+__device__ __forceinline__ void dpd_half_block(const DpdK &A, int first, HsWave *lds) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    HsWave &W = lds[tid >> 6];
+    const DpdP P = dpd_view(A);
+    const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
+    const int n = P.starts[Lx * Ly * Lz];
+    const int wfirst = first + (tid & ~63);
+    const int i = first + tid;
+    const bool active = i < n && i - P.i0 < P.nmax;
+    UDX_GLOBAL double *const gacc = (UDX_GLOBAL double *)A.acc;
+
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
+    int cx = 0, cy = 0, cz = 0, istart = 0;
+    float ry = 0.f, rz = 0.f;
+    if (active) {
+        pi = P.p(i);
+        vi = P.v(i);
+        cell_coord(pi.x, Lx, &cx);
+        cell_coord(pi.y, Ly, &cy);
+        cell_coord(pi.z, Lz, &cz);
+        istart = P.starts[cx + Lx * (cy + Ly * cz)];
+        ry = (float)((double)pi.y + 0.5 * (double)Ly - (double)cy);
+        rz = (float)((double)pi.z + 0.5 * (double)Lz - (double)cz);
+    }
+    W.hp[lane] = pi;
+    W.hv[lane] = vi;
+    W.hacc[0][lane] = W.hacc[1][lane] = W.hacc[2][lane] = 0.0;
+    const float seed_bulk = P.seeds[26];
+    const int ubase = wfirst - P.i0;   // noise index of the wave's first particle
+    constexpr unsigned QM = HS_QN - 1;
+    unsigned head = 0, tail = 0;       // wave-uniform queue counters
+    unsigned bulk = 0;                 // pairs evaluated in the bulk pass
+
+    // k = 1 or 2 queued pairs per lane; c0 is the chunk's first particle
+    auto batch = [&](auto two, int c0) {
+        constexpr bool TWO = decltype(two)::value;
+        const unsigned avail = tail - head;
+        const unsigned e0 = W.q[head & QM];
+        const bool va = (unsigned)lane < avail, vb = TWO && (unsigned)lane + 64u < avail;
+        const unsigned ea = va ? W.q[(head + lane) & QM] : e0;
+        const unsigned eb = vb ? W.q[(head + 64u + lane) & QM] : e0;
+        auto in = [&](unsigned e, PairIn &o) {
+            const int ow = (int)(e & 63u), k = (int)(e >> 6);
+            const float4 a = W.hp[ow], av = W.hv[ow], b = W.u.st.sp[k], bv = W.u.st.sv[k];
+            pair_kin(o, a.x, a.y, a.z, av, b, bv);
+            const int u = ubase + ow, v = c0 + k - P.i0;   // u > v: the home particle has the higher index
+            o.x = seed_bulk - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
+        };
+        auto out = [&](bool ok, unsigned e, const float3 &f) {
+            if (ok) {
+                const int ow = (int)(e & 63u), k = (int)(e >> 6);
+                const double qx = quant32(f.x), qy = quant32(f.y), qz = quant32(f.z);
+                lds_add(&W.hacc[0][ow], qx);
+                lds_add(&W.hacc[1][ow], qy);
+                lds_add(&W.hacc[2][ow], qz);
+                lds_add(&W.racc[0][k], -qx);
+                lds_add(&W.racc[1][k], -qy);
+                lds_add(&W.racc[2][k], -qz);
+            }
+        };
+        PairIn a, b;
+        in(ea, a);
+        if (TWO) {
+            in(eb, b);
+            float xa = a.x, xb = b.x;
+            logistic18x2(&xa, &xb);
+            const float3 fa = dpd_pair<WR, false>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+            const float3 fb = dpd_pair<WR, false>(P.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+            out(va, ea, fa);
+            out(vb, eb, fb);
+        } else {
+            const float xa = logistic18(a.x);
+            const float3 fa = dpd_pair<WR, false>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+            out(va, ea, fa);
+        }
+        const unsigned took = avail < (TWO ? 128u : 64u) ? avail : (TWO ? 128u : 64u);
+        head += took;
+        bulk += took;
+    };
+
+    // ---- bulk pass over the half shell
+    for (int row = 0; row < 5; row++) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        const int yy = cy + dy, zz = cz + dz;
+        int j0 = 0, n0 = 0;
+        if (active && yy >= 0 && yy < Ly && zz >= 0) {
+            const int x0 = cx > 0 ? cx - 1 : 0;
+            const int x1 = row == 4 ? cx : (cx + 1 < Lx ? cx + 1 : Lx - 1);
+            run_range(P, Lx * (yy + Ly * zz), x0, x1, pi.x, row_halfwidth(dy, dz, ry, rz), &j0, &n0);
+            if (row == 4) n0 = max(0, min(n0, i - j0));   // own row: lower indices only
+        }
+        int lo = n0 > 0 ? j0 : 0x7fffffff, hi = n0 > 0 ? j0 + n0 : 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+        }
+        for (int c0 = lo; c0 < hi; c0 += HS_SC) {
+            const int len = min(HS_SC, hi - c0);
+            __builtin_amdgcn_wave_barrier();
+            for (int k = lane; k < len; k += 64) {
+                const f4v p = P.pos[c0 + k], v = P.vel[c0 + k];
+                W.u.st.sp[k] = make_float4(p.x, p.y, p.z, 0.f);
+                W.u.st.sv[k] = make_float4(v.x, v.y, v.z, 0.f);
+                W.racc[0][k] = W.racc[1][k] = W.racc[2][k] = 0.0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // this lane's candidates in the chunk: slots [s0, s0 + m)
+            const int ja = max(j0, c0), jb = min(j0 + n0, c0 + len);
+            const int m = n0 > 0 && jb > ja ? jb - ja : 0;
+            const int s0 = m > 0 ? ja - c0 : 0;
+            const float4 *sp = W.u.st.sp + s0;
+            for (int t = 0; wave_any(t < m); t += 4) {
+                const float4 c[4] = {sp[t], sp[t + 1], sp[t + 2], sp[t + 3]};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const float ddx = pi.x - c[u].x, ddy = pi.y - c[u].y, ddz = pi.z - c[u].z;
+                    const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));
+                    const bool hit = t + u < m && r2 < 1.0f;
+                    const unsigned long long mk = __ballot(hit);
+                    const unsigned pos =
+                        tail + __builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+                    if (hit) W.q[pos & QM] = (unsigned short)(lane | ((s0 + t + u) << 6));
+                    tail += (unsigned)__popcll(mk);
+                }
+                while (tail - head >= 128u) batch(PassTag<true>{}, c0);
+            }
+            if (tail - head > 64u) batch(PassTag<true>{}, c0);
+            else if (tail != head) batch(PassTag<false>{}, c0);
+            __builtin_amdgcn_wave_barrier();
+            // the chunk's reactions into the global sums
+            for (int k = lane; k < len; k += 64) {
+                const double rx = W.racc[0][k], rY = W.racc[1][k], rZ = W.racc[2][k];
+                if (rx != 0.0 || rY != 0.0 || rZ != 0.0) {
+                    UDX_GLOBAL double *g = gacc + 3 * (long)(c0 + k);
+                    gacc_add(g, rx);
+                    gacc_add(g + 1, rY);
+                    gacc_add(g + 2, rZ);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+
+    // ---- halo pass: pairs across the periodic boundary, one-sided (the
+    // full-shell gather's halo pass with the sums exact)
+    double hx = 0.0, hy = 0.0, hz = 0.0;
+    int nhalo = 0;
+    const bool edge =
+        active && P.periodic && (cx == 0 || cx == Lx - 1 || cy == 0 || cy == Ly - 1 || cz == 0 || cz == Lz - 1);
+    if (wave_any(edge)) {
+        constexpr unsigned STEP = 4u * 64u, RMASK = (unsigned)(HS_HQC - 1) * STEP;
+        unsigned rh = 0, rt = 0;
+        const unsigned qlane = (unsigned)(lane * 4);
+        auto slot = [&](unsigned k) { return (unsigned *)((char *)W.u.ring + ((k & RMASK) | qlane)); };
+        const unsigned idle = (unsigned)(active ? i : P.i0) | (W_NONE << 27);
+        auto step = [&]() {
+            const unsigned fl = rt - rh;
+            const bool va = fl > 0u, vb = fl > STEP;
+            const unsigned ea = va ? *slot(rh) : idle;
+            const unsigned eb = vb ? *slot(rh + STEP) : idle;
+            const PairIn a = pair_in_halo(A, P, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+            const PairIn b = pair_in_halo(A, P, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
+            float xa = a.x, xb = b.x;
+            logistic18x2(&xa, &xb);
+            const float3 fa = dpd_pair<WR, false>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+            const float3 fb = dpd_pair<WR, false>(P.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+            if (va) {
+                hx += quant32(fa.x);
+                hy += quant32(fa.y);
+                hz += quant32(fa.z);
+            }
+            if (vb) {
+                hx += quant32(fb.x);
+                hy += quant32(fb.y);
+                hz += quant32(fb.z);
+            }
+            const unsigned took = fl < 2u * STEP ? fl : 2u * STEP;
+            rh += took;
+            nhalo += (int)(took / STEP);
+        };
+        auto push = [&](bool hit, unsigned e) {
+            unsigned *const w = slot(rt);
+            if (hit) *w = e;
+            rt += hit ? STEP : 0u;
+        };
+        __builtin_amdgcn_wave_barrier();
+        for (int row = 0; row < 9; row++) {
+            const int dz = row / 3 - 1, dy = row % 3 - 1;
+            int yy = cy + dy, zz = cz + dz;
+            const int wy = yy < 0 ? -1 : (yy >= Ly ? 1 : 0), wz = zz < 0 ? -1 : (zz >= Lz ? 1 : 0);
+            yy -= wy * Ly;
+            zz -= wz * Lz;
+            const bool yzwrap = (wy | wz) != 0;
+            int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
+            if (edge) {
+                const float hw = row_halfwidth(dy, dz, ry, rz);
+                const int base = Lx * (yy + Ly * zz);
+                if (xrun(0, cx, Lx, &x0, &x1, &wx0) && (yzwrap || wx0 != 0))
+                    run_range(P, base, x0, x1, pi.x - (float)(wx0 * Lx), hw, &j0, &n0);
+                if (xrun(1, cx, Lx, &x0, &x1, &wx1) && (yzwrap || wx1 != 0))
+                    run_range(P, base, x0, x1, pi.x - (float)(wx1 * Lx), hw, &j1, &n1);
+            }
+            const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
+            const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
+            const unsigned w0 = wcode(wx0, wy, wz) << 27, w1 = wcode(wx1, wy, wz) << 27;
+            const int len = n0 + n1;
+            for (int t = 0; wave_any(t < len); t += 2) {
+                if (wave_any(rt - rh + 2u * STEP > (unsigned)HS_HQC * STEP))
+#pragma unroll 1
+                    for (int k = 0; k < DRAIN_STEPS; k++) step();
+                const int ta = t, tb = t + 1;
+                const bool oka = ta < len, okb = tb < len;
+                const int ja = ta >= n0 ? j1 + (ta - n0) : j0 + ta;
+                const int jb = tb >= n0 ? j1 + (tb - n0) : j0 + tb;
+                const float4 pa = P.p(oka ? ja : P.i0);
+                const float4 pb = P.p(okb ? jb : P.i0);
+                const float axa = ta >= n0 ? ax1 : ax0, axb = tb >= n0 ? ax1 : ax0;
+                const float dxa = axa - pa.x, dya = ay - pa.y, dza = az - pa.z;
+                const float dxb = axb - pb.x, dyb = ay - pb.y, dzb = az - pb.z;
+                const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
+                const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
+                push(oka && r2a < 1.0f, (unsigned)ja | (ta >= n0 ? w1 : w0));
+                push(okb && r2b < 1.0f, (unsigned)jb | (tb >= n0 ? w1 : w0));
+            }
+        }
+        while (wave_any(rt != rh)) step();
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    if (active) {
+        UDX_GLOBAL double *g = gacc + 3 * (long)i;
+        gacc_add(g, W.hacc[0][lane] + hx);
+        gacc_add(g + 1, W.hacc[1][lane] + hy);
+        gacc_add(g + 2, W.hacc[2][lane] + hz);
+    }
+    if (A.pairs) {
+        // ordered pairs, as the gather counts them: each bulk pair twice
+        unsigned long long c = (unsigned long long)nhalo;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+        c += 2ull * bulk;
+        if (lane == 0 && c) atomicAdd(A.pairs, c);
+    }
+}
+
+template <int WR>
+__global__ __launch_bounds__(DPD_THREADS) void k_dpd_half(DpdK A) {
+    __shared__ HsWave lds[DPD_THREADS / 64];
+    dpd_half_block<WR, DPD_THREADS>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS, lds);
+}
+
+template <int WR>
+__global__ __launch_bounds__(DPD_THREADS) void k_dpd_half_batch(const DpdK *boxes, const int *bblock, int nbox) {
+    __shared__ HsWave lds[DPD_THREADS / 64];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    int lo = 0, hi = nbox;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (bblock[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    const DpdK &A = boxes[lo];
+    dpd_half_block<WR, DPD_THREADS>(A, A.i0 + (blk - bblock[lo]) * DPD_THREADS, lds);
+}
+
+// float forces from the FP64 sums (rounded once), which are cleared for the
+// next pass: particles i0 + k, k < nmax, below *n_dev (the end of the cell list)
+__global__ __launch_bounds__(256) void k_acc_to_ff(int nmax, const int *n_dev, int i0, double *acc, float *ff,
+                                                     int stride, int accumulate) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmax || i0 + k >= *n_dev) return;
+    const long i = i0 + k;
+    double *a = acc + 3 * i;
+    const float fx = (float)a[0], fy = (float)a[1], fz = (float)a[2];
+    a[0] = a[1] = a[2] = 0.0;
+    if (stride == 4) {
+        float4 *f4 = (float4 *)ff + i;
+        float4 o = accumulate ? *f4 : make_float4(0.f, 0.f, 0.f, 0.f);
+        *f4 = make_float4(o.x + fx, o.y + fy, o.z + fz, 0.f);
+    } else {
+        float *f3 = ff + 3 * i;
+        if (accumulate) {
+            f3[0] += fx; f3[1] += fy; f3[2] += fz;
+        } else {
+            f3[0] = fx; f3[1] = fy; f3[2] = fz;
+        }
+    }
+}
+
 // Counts of the cells of the 26 fragments in fragment-cell order (x
 // fastest, src/exch/flu/dev/map.h:22-42), one extra 0 after each fragment.
 // An exclusive scan over this array gives every fragment's packed-list
@@ -577,6 +928,7 @@ DpdK make_k(const DpdLaunch &a) {
     k.accumulate = a.accumulate;
     k.pairs = a.pairs;
     k.ss = a.ss;
+    k.acc = a.acc;
     return k;
 }
 
@@ -652,6 +1004,21 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
         if (sp == 1.0f) return launch_dpd<WR_ONE, true>(s, k, a.nmax);
         return launch_dpd<WR_POW, true>(s, k, a.nmax);
     }
+    if (mode == UDX_MODE_HALF) {
+        UDX_CHECK(a.acc, UDX_EINVAL, "dpd: the half-shell mode needs FP64 accumulators");
+        // half shell with exact FP64 sums, then the float forces
+        const dim3 g(ceil_div(a.nmax, DPD_THREADS)), b(DPD_THREADS);
+        if (sp == 0.25f) hipLaunchKernelGGL(k_dpd_half<WR_QUARTER>, g, b, 0, s, k);
+        else if (sp == 0.5f) hipLaunchKernelGGL(k_dpd_half<WR_HALF>, g, b, 0, s, k);
+        else if (sp == 1.0f) hipLaunchKernelGGL(k_dpd_half<WR_ONE>, g, b, 0, s, k);
+        else hipLaunchKernelGGL(k_dpd_half<WR_POW>, g, b, 0, s, k);
+        UDX_LAUNCHED();
+        const int nc = a.box.ncells();
+        hipLaunchKernelGGL(k_acc_to_ff, dim3(ceil_div(a.nmax, 256)), dim3(256), 0, s, a.nmax, a.starts + nc, a.i0,
+                           a.acc, a.ff, a.ff_stride, a.accumulate);
+        UDX_LAUNCHED();
+        return UDX_OK;
+    }
     if (sp == 0.25f) return launch_dpd<WR_QUARTER, false>(s, k, a.nmax);
     if (sp == 0.5f) return launch_dpd<WR_HALF, false>(s, k, a.nmax);
     if (sp == 1.0f) return launch_dpd<WR_ONE, false>(s, k, a.nmax);
@@ -712,6 +1079,13 @@ int dpd_batch_build(DpdBatch *b, const DpdLaunch *boxes, int nbox) {
     }
     b->nbox = nbox;
     b->spow = boxes[0].par.spow;
+    b->acc = boxes[0].acc;
+    b->ff = boxes[0].ff;
+    for (int k = 0; k < nbox; k++)
+        UDX_CHECK(boxes[k].acc == b->acc && boxes[k].ff == b->ff && boxes[k].ff_stride == 4 && !boxes[k].accumulate,
+                  UDX_EINVAL, "dpd: the boxes of a batch share one float4 force array and one FP64 sum array");
+    b->ntot = boxes[nbox - 1].i0 + boxes[nbox - 1].nmax;
+    b->n_end = boxes[nbox - 1].starts + boxes[nbox - 1].box.ncells();
     return UDX_OK;
 }
 
@@ -724,6 +1098,25 @@ void dpd_batch_free(DpdBatch *b) {
 
 int dpd_forces_batch(hipStream_t s, const DpdBatch &b, int mode, int variant) {
     const float sp = b.spow;
+    if (mode == UDX_MODE_HALF) {
+        UDX_CHECK(b.acc, UDX_EINVAL, "dpd: the half-shell mode needs FP64 accumulators");
+        if (b.nblocks[0] > 0) {
+            const dim3 g(b.nblocks[0]), t(DPD_THREADS);
+            const DpdK *tab = (const DpdK *)b.table;
+            if (sp == 0.25f) hipLaunchKernelGGL(k_dpd_half_batch<WR_QUARTER>, g, t, 0, s, tab, b.bblock[0], b.nbox);
+            else if (sp == 0.5f) hipLaunchKernelGGL(k_dpd_half_batch<WR_HALF>, g, t, 0, s, tab, b.bblock[0], b.nbox);
+            else if (sp == 1.0f) hipLaunchKernelGGL(k_dpd_half_batch<WR_ONE>, g, t, 0, s, tab, b.bblock[0], b.nbox);
+            else hipLaunchKernelGGL(k_dpd_half_batch<WR_POW>, g, t, 0, s, tab, b.bblock[0], b.nbox);
+            UDX_LAUNCHED();
+        }
+        if (b.ntot > 0) {
+            hipLaunchKernelGGL(k_acc_to_ff, dim3(ceil_div(b.ntot, 256)), dim3(256), 0, s, b.ntot, b.n_end, 0, b.acc,
+                               b.ff, 4, 0);
+            UDX_LAUNCHED();
+        }
+        (void)variant;
+        return UDX_OK;
+    }
     if (mode == UDX_MODE_EXACT) {
         if (sp == 0.25f) return launch_batch_wr<WR_QUARTER, true>(s, b);
         if (sp == 0.5f) return launch_batch_wr<WR_HALF, true>(s, b);

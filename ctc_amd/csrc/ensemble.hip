@@ -173,6 +173,7 @@ struct udx_ens {
     int4 *boxL;
     MoveBox *mb;
     float4 *pos[2], *vel[2], *ff;
+    double *acc;                   // FP64 force sums of the fast pass (kept zero between steps)
     ClistBufs cl;
     int *fcnt, *fss, *fscan;
     float *seeds_table, *seeds_cur, *seeds_host;
@@ -206,6 +207,7 @@ static int ens_build_batches(udx_ens *e) {
             a.accumulate = 0;
             a.pairs = e->pairs;
             a.i0 = e->poff_h[b];
+            a.acc = e->acc;
         }
         UDX_TRY(dpd_batch_build(&e->batch[par], d.data(), e->nbox));
     }
@@ -302,6 +304,7 @@ int udx_ens_fin(udx_ens *e) {
         dev_free(e->vel[k]);
     }
     dev_free(e->ff);
+    dev_free(e->acc);
     dev_free(e->poff);
     dev_free(e->coff);
     dev_free(e->fbase);
@@ -386,6 +389,9 @@ int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **out)
         if (!st) st = dev_alloc(&e->vel[k], (size_t)e->ntot);
     }
     if (!st) st = dev_alloc(&e->ff, (size_t)e->ntot);
+    if (!st) st = dev_alloc(&e->acc, 3 * (size_t)(e->ntot > 0 ? e->ntot : 1));
+    if (!st && hipMemset(e->acc, 0, sizeof(double) * 3 * (size_t)(e->ntot > 0 ? e->ntot : 1)) != hipSuccess)
+        st = UDX_EHIP;
     if (!st) st = clist_alloc(&e->cl, Box{{e->ctot, 1, 1}}, e->ntot > 0 ? e->ntot : 1, SOLVENT_SUB);
     if (!st) st = dev_alloc(&e->fcnt, (size_t)e->ftot);
     if (!st) st = dev_alloc(&e->fss, (size_t)e->ftot + 1);
@@ -421,7 +427,7 @@ int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **out)
 }
 
 int udx_ens_set_mode(udx_ens *e, int mode) {
-    UDX_CHECK(e && (mode == UDX_MODE_FAST || mode == UDX_MODE_EXACT), UDX_EINVAL, "udx_ens_set_mode: bad mode");
+    UDX_CHECK(e && mode >= UDX_MODE_FAST && mode <= UDX_MODE_HALF, UDX_EINVAL, "udx_ens_set_mode: bad mode");
     if (mode != e->mode) ens_drop_graph(e);
     e->mode = mode;
     return UDX_OK;

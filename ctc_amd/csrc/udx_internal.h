@@ -136,6 +136,9 @@ struct DpdLaunch {
     unsigned long long *pairs;   // ordered-pair counter or null
     int i0 = 0;             // first particle of the box in pos/vel (noise indices are relative)
     float *ss = nullptr;    // per-particle stress, 6 floats added (null: forces only)
+    // FP64 accumulators, 3 per particle, zero on entry and left zero: with one,
+    // UDX_MODE_FAST evaluates each pair once (half shell) and sums exactly
+    double *acc = nullptr;
 };
 int dpd_forces(hipStream_t s, const DpdLaunch &a);
 // mode: UDX_MODE_FAST or UDX_MODE_EXACT
@@ -148,6 +151,12 @@ struct DpdBatch {
     int nblocks[2] = {0, 0};
     int nbox = 0;
     float spow = 0.25f;
+    // the half-shell pass (UDX_MODE_FAST): FP64 sums shared by the boxes, and
+    // the float forces they are rounded into (all boxes' particles in one array)
+    double *acc = nullptr;
+    float *ff = nullptr;
+    int ntot = 0;
+    const int *n_end = nullptr;   // device: end of the last box's cell list
 };
 // tile variants of the batched force kernel: threads per block and queued
 // pairs per lane; variant 0 is the default

@@ -251,7 +251,7 @@ class RndTrunk:
         return lib().udx_rnd_get(C.byref(self.s))
 
 
-MODE_FAST, MODE_EXACT = 0, 1
+MODE_FAST, MODE_EXACT, MODE_HALF = 0, 1, 2
 
 
 def _mode(exact, mode):

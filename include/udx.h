@@ -121,12 +121,18 @@ int udx_fluforces_apply_seeds(udx_fluforces *f, const udx_pair_dpd *p, const int
  * Correctly rounded arithmetic (the mode setting does not apply). */
 int udx_fluforces_apply_stress(udx_fluforces *f, const udx_pair_dpd *p, const int *starts, int periodic,
                                const float seeds[27], float *ff, float *ss);
-/* force kernel modes.  Every kernel gathers the full shell of each particle
+/* force kernel modes.  FAST and EXACT gather the full shell of each particle
  * in a fixed order (no atomics) with correctly rounded reciprocals and roots,
- * so both modes give the same bits, equal to the CPU restatement; the mode is
- * kept so callers written against the two-mode interface keep working. */
+ * so both give the same bits, equal to the CPU restatement; EXACT is kept so
+ * callers written against the two-mode interface keep working.  HALF (solvent
+ * forces only: udx_fluforces, udx_flu, udx_ens) evaluates each pair inside the
+ * box once, as the reference's half-shell bulk kernel, and sums every
+ * particle's pair forces exactly (each component rounded to a multiple of
+ * 2^-32, added in FP64, rounded to float once): deterministic, independent of
+ * the order of the additions, equal to the CPU restatement's exact sums. */
 #define UDX_MODE_FAST 0    /* default */
 #define UDX_MODE_EXACT 1
+#define UDX_MODE_HALF 2
 int udx_fluforces_set_mode(udx_fluforces *f, int mode);
 /* unordered pair interactions (r^2 < 1) counted by apply calls since the last reset */
 int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset);
@@ -497,7 +503,7 @@ typedef struct udx_ens_box {
 typedef struct udx_ens udx_ens;
 int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **e);
 int udx_ens_fin(udx_ens *e);
-int udx_ens_set_mode(udx_ens *e, int mode);      /* UDX_MODE_FAST or UDX_MODE_EXACT */
+int udx_ens_set_mode(udx_ens *e, int mode);      /* UDX_MODE_FAST, UDX_MODE_EXACT or UDX_MODE_HALF */
 /* pp: AoS particles of all boxes, box after box (sum of n); positions are
  * wrapped into their box's [-L/2, L/2) (each box keeps all its particles) */
 int udx_ens_set_pp(udx_ens *e, const float *pp);

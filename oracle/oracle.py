@@ -79,6 +79,11 @@ def _declare(L):
         "orc_dpd_forces_stress": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
                                              C.c_int, F32P, F32P]),
         "orc_dpd_forces_half": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int, F32P]),
+        "orc_dpd_forces_sum64": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
+                                            C.c_int, F32P]),
+        "orc_dpd_forces_f64": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int,
+                                          C.c_int, F64P]),
+        "orc_dpd_forces_half_sum64": (C.c_long, [I32P, C.c_int, F32P, I32P, I32P, Dpd, F32P, C.c_int, F32P]),
         "orc_integrate": (None, [I32P, C.c_int, C.c_float, C.c_float, F32P, F32P, F32P]),
         "orc_move": (None, [C.c_int, C.c_float, C.c_float, F32P, F32P]),
         "orc_diag": (None, [C.c_int, F32P, F64P]),
@@ -228,6 +233,43 @@ def dpd_forces(L, pp, starts, counts, params, seeds, periodic=True, nthreads=1, 
     np_ = lib().orc_dpd_forces(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
                                _ptr(_i32(counts), I32P), Dpd(*params), _ptr(seeds, F32P),
                                1 if periodic else 0, nthreads, _ptr(ff, F32P))
+    return ff, int(np_)
+
+
+def dpd_forces_sum64(L, pp, starts, counts, params, seeds, periodic=True, nthreads=1, ff=None):
+    """dpd_forces with exact sums (pair forces rounded to multiples of 2^-32,
+    summed exactly, rounded to float once): the production kernel's forces."""
+    L = _L(L)
+    pp = _f32(pp).reshape(-1, 6)
+    n = pp.shape[0]
+    ff = np.zeros((n, 3), np.float32) if ff is None else _f32(ff).copy()
+    np_ = lib().orc_dpd_forces_sum64(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
+                                     _ptr(_i32(counts), I32P), Dpd(*params), _ptr(_f32(seeds), F32P),
+                                     1 if periodic else 0, nthreads, _ptr(ff, F32P))
+    return ff, int(np_)
+
+
+def dpd_forces_half_sum64(L, pp, starts, counts, params, seeds, nthreads=1):
+    """The half-shell traversal with exact sums (periodic box); as dpd_forces."""
+    L = _L(L)
+    pp = _f32(pp).reshape(-1, 6)
+    n = pp.shape[0]
+    ff = np.zeros((n, 3), np.float32)
+    np_ = lib().orc_dpd_forces_half_sum64(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
+                                          _ptr(_i32(counts), I32P), Dpd(*params), _ptr(_f32(seeds), F32P),
+                                          nthreads, _ptr(ff, F32P))
+    return ff, int(np_)
+
+
+def dpd_forces_f64(L, pp, starts, counts, params, seeds, periodic=True, nthreads=1):
+    """The float pair forces of dpd_forces summed in double: (ff64 (n,3), ordered pairs)."""
+    L = _L(L)
+    pp = _f32(pp).reshape(-1, 6)
+    n = pp.shape[0]
+    ff = np.zeros((n, 3), np.float64)
+    np_ = lib().orc_dpd_forces_f64(_ptr(L, I32P), n, _ptr(pp, F32P), _ptr(_i32(starts), I32P),
+                                   _ptr(_i32(counts), I32P), Dpd(*params), _ptr(_f32(seeds), F32P),
+                                   1 if periodic else 0, nthreads, _ptr(ff, F64P))
     return ff, int(np_)
 
 

@@ -403,13 +403,26 @@ static int frag_pos(const orc_grid *g, int fid, const int c[3], int idx) {
 /* so (may be NULL): the particle's stress, sum over its pairs of
  * 1/2 f (x) dr as (xx, xy, xz, yy, yz, zz) with s_ab = 1/2 f_a dr_b
  * (src/pair/dev.h:87-100) */
+/* the pair force rounded to a multiple of 2^-32 (unchanged for |f| >= 2^-9,
+ * whose float ulp is at least 2^-32); sums of such values below 2^21 are
+ * exact in double, whatever the order of the additions */
+static double quant32(float f) {
+    return (double)(rintf(f * 4294967296.0f) * 2.3283064365386963e-10f);
+}
+
+/* q64 = 1: the forces are summed exactly (quant32 terms in double) and the
+ * sum rounded to float once, the production kernel's definition; q64 = 2:
+ * the float pair forces summed in double, added to fo64 (the reference value
+ * every summation order approximates); 0: summed in float in the traversal
+ * order */
 static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, const float seeds[27],
-                       int periodic, int i, float *fo, float *so) {
+                       int periodic, int i, float *fo, float *so, int q64, double *fo64) {
     const int *L = g->L;
     const float *pi = pp + 6 * i;
     int ci[3];
     long npairs = 0;
     float acc[3] = {0.0f, 0.0f, 0.0f};
+    double dacc[3] = {0.0, 0.0, 0.0};
     float sacc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     (void)n;
     if (orc_cell_of(L, pi, ci) < 0) return 0;
@@ -460,7 +473,13 @@ static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, con
             }
             float f[3];
             orc_pair_force(&p, a, pj, rnd, f);
-            acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2];
+            if (q64 == 1) {
+                dacc[0] += quant32(f[0]); dacc[1] += quant32(f[1]); dacc[2] += quant32(f[2]);
+            } else if (q64 == 2) {
+                dacc[0] += (double)f[0]; dacc[1] += (double)f[1]; dacc[2] += (double)f[2];
+            } else {
+                acc[0] += f[0]; acc[1] += f[1]; acc[2] += f[2];
+            }
             if (so) {
                 sacc[0] += 0.5f * f[0] * ddx;
                 sacc[1] += 0.5f * f[0] * ddy;
@@ -471,6 +490,13 @@ static long gather_one(const orc_grid *g, int n, const float *pp, orc_dpd p, con
             }
             npairs++;
         }
+    }
+    if (q64 == 2) {
+        fo64[0] += dacc[0]; fo64[1] += dacc[1]; fo64[2] += dacc[2];
+        return npairs;
+    }
+    if (q64 == 1) {
+        acc[0] = (float)dacc[0]; acc[1] = (float)dacc[1]; acc[2] = (float)dacc[2];
     }
     fo[0] += acc[0]; fo[1] += acc[1]; fo[2] += acc[2];
     if (so)
@@ -492,7 +518,7 @@ long orc_dpd_forces_stress(const int L[3], int n, const float *pp, const int *st
     #pragma omp parallel for schedule(dynamic, 256) reduction(+:tot) num_threads(nthreads)
 #endif
     for (int i = 0; i < n; i++)
-        tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, ss ? ss + 6 * (size_t)i : NULL);
+        tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, ss ? ss + 6 * (size_t)i : NULL, 0, NULL);
     free(fss);
     return tot;
 }
@@ -510,7 +536,49 @@ long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, c
     if (nthreads < 1) nthreads = 1;
     #pragma omp parallel for schedule(dynamic, 256) reduction(+:tot) num_threads(nthreads)
 #endif
-    for (int i = 0; i < n; i++) tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, NULL);
+    for (int i = 0; i < n; i++) tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, NULL, 0, NULL);
+    free(fss);
+    return tot;
+}
+
+/* The same pairs and pair forces with exact sums (the production kernel's
+ * definition: each particle's force is the exact sum of its pair forces
+ * rounded to multiples of 2^-32, rounded to float once; added to ff).  The
+ * traversal order does not matter here: any order gives the same sum. */
+long orc_dpd_forces_sum64(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                          orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff) {
+    int tsz = orc_frag_table_size(L);
+    int *fss = (int *)malloc(sizeof(int) * (size_t)tsz);
+    int fss_off[27];
+    orc_frag_tables(L, counts, fss, fss_off);
+    orc_grid g = {L, starts, fss, fss_off};
+    long tot = 0;
+    (void)nthreads;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+    #pragma omp parallel for schedule(dynamic, 256) reduction(+:tot) num_threads(nthreads)
+#endif
+    for (int i = 0; i < n; i++) tot += gather_one(&g, n, pp, p, seeds, periodic, i, ff + 3 * i, NULL, 1, NULL);
+    free(fss);
+    return tot;
+}
+
+/* The same pairs, the float pair forces summed in double and ADDED to ff64
+ * (3 per particle): the value every float summation order approximates. */
+long orc_dpd_forces_f64(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                        orc_dpd p, const float seeds[27], int periodic, int nthreads, double *ff64) {
+    int tsz = orc_frag_table_size(L);
+    int *fss = (int *)malloc(sizeof(int) * (size_t)tsz);
+    int fss_off[27];
+    orc_frag_tables(L, counts, fss, fss_off);
+    orc_grid g = {L, starts, fss, fss_off};
+    long tot = 0;
+    (void)nthreads;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+    #pragma omp parallel for schedule(dynamic, 256) reduction(+:tot) num_threads(nthreads)
+#endif
+    for (int i = 0; i < n; i++) tot += gather_one(&g, n, pp, p, seeds, periodic, i, NULL, NULL, 2, ff64 + 3 * i);
     free(fss);
     return tot;
 }
@@ -521,7 +589,8 @@ long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, c
  * smaller; src/fluforces/halo/dev/main.h:7-83 for pairs across the periodic
  * boundary: one-sided, each side with its own fragment noise).  Forces go
  * to the thread's buffer f. */
-static long half_one(const orc_grid *g, const float *pp, orc_dpd p, const float seeds[27], int i, float *f) {
+static long half_one(const orc_grid *g, const float *pp, orc_dpd p, const float seeds[27], int i, float *f,
+                     double *fd) {
     const int *L = g->L;
     const float *pi = pp + 6 * i;
     int ci[3];
@@ -565,13 +634,17 @@ static long half_one(const orc_grid *g, const float *pp, orc_dpd p, const float 
                 rnd = orc_mean0var1uu(seeds[fid], a1, a2);
             }
             orc_pair_force(&p, a, pj, rnd, fij);
-            for (int k = 0; k < 3; k++) f[3 * i + k] += fij[k];
-            if (!wrapped) {
-                for (int k = 0; k < 3; k++) f[3 * j + k] -= fij[k];
-                npairs += 2;
+            if (fd) {
+                /* exact sums: quantised terms in double */
+                for (int k = 0; k < 3; k++) fd[3 * i + k] += quant32(fij[k]);
+                if (!wrapped)
+                    for (int k = 0; k < 3; k++) fd[3 * j + k] -= quant32(fij[k]);
             } else {
-                npairs++;
+                for (int k = 0; k < 3; k++) f[3 * i + k] += fij[k];
+                if (!wrapped)
+                    for (int k = 0; k < 3; k++) f[3 * j + k] -= fij[k];
             }
+            npairs += wrapped ? 1 : 2;
         }
     }
     return npairs;
@@ -598,7 +671,7 @@ long orc_dpd_forces_half(const int L[3], int n, const float *pp, const int *star
 #else
         float *mine = buf;
 #endif
-        for (int i = 0; i < n; i++) tot += half_one(&g, pp, p, seeds, i, mine);
+        for (int i = 0; i < n; i++) tot += half_one(&g, pp, p, seeds, i, mine, NULL);
     }
     /* the threads' buffers added in thread order */
 #ifdef _OPENMP
@@ -608,6 +681,44 @@ long orc_dpd_forces_half(const int L[3], int n, const float *pp, const int *star
         float s = 0.0f;
         for (int t = 0; t < nthreads; t++) s += buf[(size_t)t * m + k];
         ff[k] += s;
+    }
+    free(buf);
+    free(fss);
+    return tot;
+}
+
+/* The half-shell traversal with exact sums: per-thread FP64 buffers of
+ * quantised pair forces (each pair once inside the box, reaction to the
+ * smaller index), added in double and rounded to float once; ADDED to ff.
+ * Equal bit for bit to orc_dpd_forces_sum64 whatever the thread count, since
+ * every sum is exact. */
+long orc_dpd_forces_half_sum64(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                               orc_dpd p, const float seeds[27], int nthreads, float *ff) {
+    int tsz = orc_frag_table_size(L);
+    int *fss = (int *)malloc(sizeof(int) * (size_t)tsz);
+    int fss_off[27];
+    orc_frag_tables(L, counts, fss, fss_off);
+    orc_grid g = {L, starts, fss, fss_off};
+    if (nthreads < 1) nthreads = 1;
+    const size_t m = 3 * (size_t)n;
+    double *buf = (double *)calloc((size_t)nthreads * m + 1, sizeof(double));
+    long tot = 0;
+#ifdef _OPENMP
+    #pragma omp parallel num_threads(nthreads) reduction(+:tot)
+#endif
+    {
+#ifdef _OPENMP
+        double *mine = buf + m * (size_t)omp_get_thread_num();
+        #pragma omp for schedule(dynamic, 256)
+#else
+        double *mine = buf;
+#endif
+        for (int i = 0; i < n; i++) tot += half_one(&g, pp, p, seeds, i, NULL, mine);
+    }
+    for (size_t k = 0; k < m; k++) {
+        double s = 0.0;
+        for (int t = 0; t < nthreads; t++) s += buf[(size_t)t * m + k];
+        ff[k] += (float)s;
     }
     free(buf);
     free(fss);

@@ -103,6 +103,20 @@ long orc_dpd_forces(const int L[3], int n, const float *pp, const int *starts, c
 long orc_dpd_forces_stress(const int L[3], int n, const float *pp, const int *starts, const int *counts,
                            orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff, float *ss);
 
+/* The same pairs with exact sums, the production (UDX_MODE_FAST) kernel's
+ * definition: each particle's force is the exact sum of its pair forces, each
+ * component rounded to a multiple of 2^-32, rounded to float once and ADDED
+ * to ff (exact in double while the sums stay below 2^21; independent of the
+ * order of the additions).  Returns ordered pairs as orc_dpd_forces. */
+long orc_dpd_forces_sum64(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                          orc_dpd p, const float seeds[27], int periodic, int nthreads, float *ff);
+
+/* The same pairs, the float pair forces summed in double and ADDED to ff64
+ * (3 doubles per particle): the value that every float summation order of
+ * the reference approximates. */
+long orc_dpd_forces_f64(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                        orc_dpd p, const float seeds[27], int periodic, int nthreads, double *ff64);
+
 /* The same forces with the reference's traversal: a half shell inside the
  * box (each pair once, reaction to the smaller index), one-sided pairs across
  * the periodic boundary; per-thread force buffers added in thread order.
@@ -110,6 +124,13 @@ long orc_dpd_forces_stress(const int L[3], int n, const float *pp, const int *st
  * order).  Returns ordered pairs as orc_dpd_forces. */
 long orc_dpd_forces_half(const int L[3], int n, const float *pp, const int *starts, const int *counts,
                          orc_dpd p, const float seeds[27], int nthreads, float *ff);
+
+/* orc_dpd_forces_half with exact sums (quantised pair forces in double, each
+ * pair inside the box once with its reaction): equal bit for bit to
+ * orc_dpd_forces_sum64, which checks that the production kernel's half-shell
+ * evaluation and the gather give the same forces. */
+long orc_dpd_forces_half_sum64(const int L[3], int n, const float *pp, const int *starts, const int *counts,
+                               orc_dpd p, const float seeds[27], int nthreads, float *ff);
 
 /* ---- integrator: src/scheme/move/dev/vv.h:1-10 + body force + wrap ------ */
 void orc_integrate(const int L[3], int n, float dt, float mass, const float accel[3],

@@ -22,11 +22,13 @@ from ctc_amd import udx as U  # noqa: E402
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref")
 
-# Production-kernel force tolerance.  The kernel uses the hardware reciprocal
-# and square roots (about 1 ulp each) where the oracle rounds correctly;
-# summation order is identical.  Per particle:
+# Production-kernel force tolerance.  The fast kernel gathers in the oracle's
+# summation order with correctly rounded reciprocals and roots; per particle
 #   |f_gpu - f_oracle| <= FTOL * max(|f_oracle|, 1e-3 * F_rms)
-# (the per-particle metric of the contract, F_rms the RMS force magnitude)
+# (the per-particle metric of the contract, F_rms the RMS force magnitude).
+# The half-shell mode (UDX_MODE_HALF) sums exactly and equals the oracle's
+# dpd_forces_sum64 bit for bit; against the double sum of the same pair
+# forces it meets the same metric.
 FTOL = 1e-5
 
 
@@ -140,6 +142,7 @@ CASES = [
     ((12, 10, 9), 10, 2.5, 8.0, 0.0444, 5e-4, 0.25),   # density 10 channel parameters
     ((7, 5, 3), 5, 25.0, 4.5, 0.5, 1e-3, 1.0),         # thin box, spow = 1
     ((3, 3, 3), 4, 25.0, 4.5, 1.0, 1e-3, 0.5),         # smallest periodic box
+    ((8, 8, 8), 30, 25.0, 4.5, 1.0, 1e-3, 0.25),       # dense: rows staged in several chunks
 ]
 
 
@@ -159,8 +162,8 @@ def test_forces_exact_mode_bit_exact(ctx, case):
 
 @pytest.mark.parametrize("case", CASES)
 def test_forces_fast_mode_tolerance(ctx, case):
-    """Production kernel: the oracle's summation order with hardware
-    reciprocal and roots; identical from run to run."""
+    """Production kernel: the oracle's summation order; identical from run to
+    run; within the contract metric."""
     L, rho, a, g, kBT, dt, spow = case
     say("forces fast", case)
     pp = random_box(L, rho, 8)
@@ -177,6 +180,43 @@ def test_forces_fast_mode_tolerance(ctx, case):
     err = np.linalg.norm(f_gpu.astype(np.float64) - f_ref, axis=1) / np.maximum(mag, 1e-3 * frms)
     say("  max rel err", err.max(), "F_rms", frms)
     assert err.max() <= FTOL
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_forces_half_mode_exact_sums(ctx, case):
+    """Half shell with exact sums: the oracle's exact-sum forces bit for bit,
+    identical from run to run, the contract metric against the double sum."""
+    L, rho, a, g, kBT, dt, spow = case
+    say("forces half", case)
+    pp = random_box(L, rho, 8)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    seeds = O.seeds_table(3)[2]
+    par = (a, g, O.dpd_sigma(kBT, g, dt), spow)
+    f_q, np_q = O.dpd_forces_sum64(L, srt, starts, counts, par, seeds)
+    f_gpu, np_gpu = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, mode=U.MODE_HALF)
+    f_gpu2, _ = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, mode=U.MODE_HALF)
+    assert np.array_equal(f_gpu.view(np.uint32), f_gpu2.view(np.uint32))
+    assert np_gpu == np_q // 2
+    assert np.array_equal(f_gpu.view(np.uint32), f_q.view(np.uint32))
+    f_d, _ = O.dpd_forces_f64(L, srt, starts, counts, par, seeds)
+    mag = np.linalg.norm(f_d, axis=1)
+    frms = np.sqrt((mag ** 2).mean())
+    err = np.linalg.norm(f_gpu.astype(np.float64) - f_d, axis=1) / np.maximum(mag, 1e-3 * frms)
+    say("  max rel err vs the double sum %.3g (F_rms %.4g)" % (err.max(), frms))
+    assert err.max() <= FTOL
+
+
+def test_forces_half_mode_non_periodic(ctx):
+    """The half shell without the periodic halo (the bulk-only harness)."""
+    L = (9, 7, 6)
+    pp = random_box(L, 5, 11)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    seeds = O.seeds_table(1)[0]
+    par = (25.0, 4.5, O.dpd_sigma(1.0, 4.5, 1e-3), 0.25)
+    f_q, np_q = O.dpd_forces_sum64(L, srt, starts, counts, par, seeds, periodic=False)
+    f_gpu, np_gpu = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, periodic=False, mode=U.MODE_HALF)
+    assert np_gpu == np_q // 2
+    assert np.array_equal(f_gpu.view(np.uint32), f_q.view(np.uint32))
 
 
 @pytest.mark.parametrize("name,n,scale", [("t0", 64, 1e4), ("t1", 640, 1e2)])

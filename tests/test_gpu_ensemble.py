@@ -1,5 +1,5 @@
 """Batched ensemble of periodic boxes: identical to running every box on its
-own (bit for bit, both modes), to the oracle in exact mode, with and without
+own (bit for bit, in every mode), to the oracle in exact mode, with and without
 the captured graph, for every tile variant of the batched force kernel."""
 import sys
 
@@ -65,10 +65,15 @@ def make_boxes(nbox, seed, edges=(3, 12), dt=1e-3):
     return boxes, pps
 
 
+def _kw(exact):
+    # exact: a bool (exact or fast mode) or a U.MODE_* value
+    return dict(mode=exact) if not isinstance(exact, bool) else dict(exact=exact)
+
+
 def run_single(ctx, box, pp, seeds27, exact):
     """one box alone through udx_flu with the same seeds"""
     nsteps = len(seeds27)
-    flu = U.Flu(ctx, box["L"], box["n"], box["params"], box["dt"], 1.0, box["accel"], exact=exact)
+    flu = U.Flu(ctx, box["L"], box["n"], box["params"], box["dt"], 1.0, box["accel"], **_kw(exact))
     d = dev(pp)
     flu.set_pp(d)
     s28 = np.concatenate([seeds27, np.zeros((nsteps, 1), np.float32)], axis=1)
@@ -81,7 +86,7 @@ def run_single(ctx, box, pp, seeds27, exact):
 
 
 def run_ensemble(ctx, boxes, pps, nsteps, seeds, exact, graph=False, variant=None):
-    e = U.Ensemble(ctx, boxes, exact=exact)
+    e = U.Ensemble(ctx, boxes, **_kw(exact))
     if variant is not None:
         e.set_variant(variant)
     e.graph(graph)
@@ -95,7 +100,7 @@ def run_ensemble(ctx, boxes, pps, nsteps, seeds, exact, graph=False, variant=Non
     return host(out), pairs
 
 
-@pytest.mark.parametrize("exact", [True, False])
+@pytest.mark.parametrize("exact", [True, False, U.MODE_HALF])
 def test_ensemble_equals_single_boxes(ctx, exact):
     nbox, nsteps = 7, 12
     boxes, pps = make_boxes(nbox, 3)

@@ -9,7 +9,8 @@
   equal the oracle bit for bit (state and forces); the production force
   kernel on the same state is within the contract metric
   |df| / max(|f|, 1e-3 F_rms) <= 1e-5 per particle with equal pair counts;
-  then 20 production-mode steps leave no solvent particle inside the walls.
+  the half-shell mode equals the oracle's exact sums bit for bit; then 20
+  production-mode steps leave no solvent particle inside the walls.
 - C3 (64^3 suspension, 567 cells of 498 vertices): one exact-mode step of
   solvent + membranes + FSI; the solvent equals the oracle bit for bit and the
   vertices agree within 1e-6 (the membrane force's pow/sqrt ulps).
@@ -122,7 +123,7 @@ def test_c2_full_size_step_bit_exact(ctx):
     seeds28 = np.concatenate([seeds, O.kiss_fill(42, 42, 42, 42, 1)[:, None]], axis=1).astype(np.float32)
     sd = dev(solvent)
     gpu = {}
-    for mode in (U.MODE_EXACT, U.MODE_FAST):
+    for mode in (U.MODE_EXACT, U.MODE_FAST, U.MODE_HALF):
         flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*params), dt, 1.0, accel, mode=mode)
         flu.set_pp(sd)
         flu.set_wall(w)
@@ -152,6 +153,23 @@ def test_c2_full_size_step_bit_exact(ctx):
     say("C2 production kernel: max rel err %.3g (F_rms %.4g, min |f| %.3g, %d particles)"
         % (err.max(), frms, mag.min(), len(mag)))
     assert err.max() <= 1e-5
+    # the half-shell mode: the oracle's exact-sum solvent forces plus the same
+    # wall forces bit for bit, equal pair counts, and the contract metric
+    # against the double sum of the same pair forces
+    _, fhh, hp, hw = gpu[U.MODE_HALF]
+    assert (hp, hw) == (npair // 2, nw)
+    fq0, nq = O.dpd_forces_sum64(L, srt, starts, counts, params, seeds28[0, :27], nthreads=THREADS)
+    assert nq == npair
+    fq, _ = W.wall_forces(L, srt, ref, wpp, wst, params, seeds28[0, 27], fq0)
+    assert np.array_equal(bits(fhh), bits(fq))
+    fd, _ = O.dpd_forces_f64(L, srt, starts, counts, params, seeds28[0, :27], nthreads=THREADS)
+    fd += fq.astype(np.float64) - fq0   # the wall forces
+    magd = np.linalg.norm(fd, axis=1)
+    frmsd = np.sqrt((magd ** 2).mean())
+    errh = np.linalg.norm(fhh.astype(np.float64) - fd, axis=1) / np.maximum(magd, 1e-3 * frmsd)
+    errf = np.linalg.norm(fff.astype(np.float64) - fd, axis=1) / np.maximum(magd, 1e-3 * frmsd)
+    say("C2 half-shell mode: max rel err %.3g vs the double sum (the gather: %.3g)" % (errh.max(), errf.max()))
+    assert errh.max() <= 1e-5
     say("C2: 20 production steps")
     flu = U.Flu(ctx, L, len(solvent), U.PairDpd(*params), dt, 1.0, accel)
     flu.set_pp(sd)
