@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=40, help="timed steps of the ensemble leg")
     ap.add_argument("--adj-cells", type=int, default=256, help="cells of the membrane adjoint leg (0: skip)")
     ap.add_argument("--susp-steps", type=int, default=20, help="timed steps of the RBC suspension leg (0: skip)")
+    ap.add_argument("--half-steps", type=int, default=20,
+                    help="timed steps of the same wall-bounded box in the half-shell force mode (0: skip)")
     ap.add_argument("--adj-reps", type=int, default=20, help="timed forward + adjoint batches")
     return ap.parse_args()
 
@@ -232,6 +234,9 @@ def main():
         "kernel_sources_sha16": kernel_hash(),
     }
 
+    if args.half_steps > 0:
+        out["half_shell_mode"] = half_leg(ctx, flu, L, n, params, dt, fx, wall, args.half_steps)
+
     if args.cpu_steps > 0:
         cur = torch.empty((n, 6), dtype=torch.float32, device="cuda")
         flu.get_pp(cur)
@@ -257,6 +262,36 @@ def main():
     if args.susp_steps > 0:
         out["suspension"] = suspension_leg(ctx, args.susp_steps, 5, args.cpu_steps > 0)
     print(json.dumps(out))
+
+
+def half_leg(ctx, flu, L, n, params, dt, fx, wall, nsteps):
+    """The same state stepped with the half-shell force mode (each pair inside
+    the box evaluated once, exact FP64 sums): per-phase times of nsteps steps."""
+    import torch
+
+    from ctc_amd import udx as U
+
+    cur = torch.empty((n, 6), dtype=torch.float32, device="cuda")
+    flu.get_pp(cur)
+    h = U.Flu(ctx, L, n, params, dt, 1.0, (fx, 0.0, 0.0), mode=U.MODE_HALF)
+    h.set_pp(cur)
+    if wall is not None:
+        h.set_wall(wall)
+    h.run(3)
+    h.pairs(reset=True)
+    h.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    h.run(nsteps)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    t = h.timing(False)
+    pairs = h.pairs()
+    h.close()
+    del cur
+    return {"steps": nsteps, "ms_per_step": sec / nsteps * 1e3, "pairs_per_step": pairs / nsteps,
+            "phase_ms": {"clist": t[0], "dpd": t[1], "wall_dpd": t[2], "integrate": t[3]},
+            "kernels": "k_dpd_half (half shell, wave-level hit queue, FP64 reactions) + k_acc_to_ff"}
 
 
 def ensemble_leg(ctx, nbox, nsteps):

@@ -123,6 +123,9 @@ struct udx_flu {
     double *acc;        // FP64 force sums of the fast pass (3 per particle, kept zero between steps)
     int cur;
     ClistBufs cl;
+    // the last step's move counted pos[cur] into the cell list's x slabs, so
+    // the next sort starts from its scan
+    int counted = 0;
     FragTables frag;
     Trunks trunks;
     float *seeds_dev;   // seed_cap x NSEED
@@ -521,6 +524,7 @@ int udx_flu_set_pp(udx_flu *f, int n, const float *pp) {
     if (n != f->n) flu_drop_graph(f);
     f->n = n;
     f->cur = 0;
+    f->counted = 0;
     return aos_to_f4(f->ctx->stream, n, pp, f->pos[0], f->vel[0]);
 }
 
@@ -589,7 +593,8 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     udx_flu::Membranes &m = f->mb;
     const int nvtx = m.nc * m.nv;
     if (f->evk) UDX_HIP(hipEventRecord(f->evk[0], s));
-    UDX_TRY(clist_sort(s, &f->cl, f->n, f->pos[a]));
+    if (f->counted) UDX_TRY(clist_finish(s, &f->cl, f->n));
+    else UDX_TRY(clist_sort(s, &f->cl, f->n, f->pos[a]));
     UDX_TRY(gather_f4(s, &f->cl, f->n, f->pos[a], f->vel[a], f->pos[b], f->vel[b]));
     UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts));
     if (m.r) UDX_TRY(rbc_redistribute(s, f->box, m.nc, m.nv, m.pp, m.tmp, m.cfid, m.cnew));
@@ -656,8 +661,11 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     // store_solvent (src/sim/imp/update.h:11-16): positions before the move
     if (bounce && f->n > 0)
         UDX_HIP(hipMemcpyAsync(m.pos0, f->pos[b], sizeof(float4) * (size_t)f->n, hipMemcpyDeviceToDevice, s));
+    // without the membrane bounce-back (which moves particles after the
+    // move and wraps them itself) the move also counts the next step's slabs
     UDX_TRY(move_f4_wall(s, f->n, f->cl.starts + nc, f->box, f->conf.dt, f->conf.mass, f->conf.accel, f->ff,
-                         f->pos[b], f->vel[b], sdf, wv, bounce ? 0 : 1));
+                         f->pos[b], f->vel[b], sdf, wv, bounce ? 0 : 1, bounce ? nullptr : &f->cl));
+    f->counted = bounce ? 0 : 1;
     if (m.r && nvtx > 0) {
         // objects_advance (src/sim/objects/imp/main.h:40-61): per sub-step the
         // fast force starts as the FSI force, the membrane force is added and
@@ -712,6 +720,8 @@ static int flu_capture(udx_flu *f) {
     flu_drop_graph(f);
     const float t0 = f->t;
     const int cur0 = f->cur;
+    const int counted0 = f->counted;
+    f->counted = 1;   // replays start counted (flu_run_graph counts first when not)
     f->step_stream = f->gs;
     UDX_HIP(hipStreamBeginCapture(f->gs, hipStreamCaptureModeThreadLocal));
     int st = UDX_OK;
@@ -729,6 +739,7 @@ static int flu_capture(udx_flu *f) {
     f->step_stream = nullptr;
     f->t = t0;
     f->cur = cur0;
+    f->counted = counted0;
     if (st && g) (void)hipGraphDestroy(g);
     UDX_TRY(st);
     UDX_CHECK(ce == hipSuccess && g, UDX_EHIP, "udx_flu_run: graph capture failed: %s", hipGetErrorString(ce));
@@ -741,6 +752,11 @@ static int flu_capture(udx_flu *f) {
 // nsteps >= 2 steps by graph replay (pairs), the last one eagerly when odd
 static int flu_run_graph(udx_flu *f, int nsteps) {
     hipStream_t s = f->ctx->stream;
+    // the captured steps expect the current positions counted into the slabs
+    if (!f->counted) {
+        UDX_TRY(clist_count_only(s, &f->cl, f->n, f->pos[f->cur]));
+        f->counted = 1;
+    }
     UDX_HIP(hipMemsetAsync(f->step_dev, 0, sizeof(int), s));
     UDX_HIP(hipEventRecord(f->gdone, s));
     UDX_HIP(hipStreamWaitEvent(f->gs, f->gdone, 0));
@@ -751,6 +767,7 @@ static int flu_run_graph(udx_flu *f, int nsteps) {
     UDX_HIP(hipEventRecord(f->gdone, f->gs));
     UDX_HIP(hipStreamWaitEvent(s, f->gdone, 0));
     f->t += (float)(2 * pairs) * f->conf.dt;
+    f->counted = 1;   // the replayed moves counted
     if (nsteps & 1) UDX_TRY(flu_step(f, f->seeds_dev + NSEED * (nsteps - 1), flu_fsi_seeds(f, nsteps - 1),
                                      flu_mwall_seed(f, nsteps - 1)));
     return UDX_OK;

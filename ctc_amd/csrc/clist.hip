@@ -70,8 +70,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(const int *part,
     if (threadIdx.x == 0) part_ex[nb] = carry;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int *in, int n, const int *part_ex,
-                                                             int nb, int *out) {
+// zero: the input is cleared after it is read (counts consumed by the scan
+// are ready for the next counting pass)
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, const int *part_ex, int nb, int *out,
+                                                             int zero) {
     __shared__ int lw[4];
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
@@ -80,6 +82,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int *in, int 
         v[k] = (base + k < n) ? in[base + k] : 0;
         s += v[k];
     }
+    if (zero)
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++)
+            if (base + k < n) in[base + k] = 0;
     int inc = block_inclusive_scan(s, lw);
     int run = part_ex[blockIdx.x] + inc - s;
 #pragma unroll
@@ -108,32 +114,13 @@ struct PosAos {
     }
 };
 
-// Slab of each particle and a provisional slot in it.  Consecutive lanes
-// with the same slab (the usual case: the input is the previous step's
-// cell-ordered array) form a segment that takes its slots with one atomic.
 template <typename Pos>
 __global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int Ly, int Lz, int sub, int *counts,
                                                      int2 *cellslot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    int cid = -1;
-    if (i < n) {
-        const float3 r = pos(i);
-        int bx, cy, cz;
-        if (slab_coord(r.x, Lx, sub, &bx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz))
-            cid = bx + Lx * sub * (cy + Ly * cz);
-    }
-    const int prev = __shfl_up(cid, 1, 64);
-    const bool head = lane == 0 || prev != cid;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-    const int hlane = 63 - __clzll(heads & upto);
-    const unsigned long long above = heads & ~upto;
-    const int next = above ? __ffsll(above) - 1 : 64;
-    int base = 0;
-    if (head && cid >= 0) base = atomicAdd(counts + cid, next - lane);
-    base = __shfl(base, hlane, 64);
-    if (i < n) cellslot[i] = make_int2(cid, base + (lane - hlane));
+    float3 r = make_float3(0.f, 0.f, 0.f);
+    if (i < n) r = pos(i);
+    clist_count_one(i < n, i, r, Lx, Ly, Lz, sub, counts, cellslot);
 }
 
 // cell tables from the slab starts: starts[c] = bstarts[c sub]
@@ -222,7 +209,7 @@ int scan_scratch_size(int n) {
     return 2 * nb + 2;
 }
 
-int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch) {
+static int scan_impl(hipStream_t s, int *in, int n, int *out, int *scratch, int zero) {
     int nb = ceil_div(n, SCAN_TILE);
     if (nb == 0) nb = 1;
     int *part = scratch, *part_ex = scratch + nb;
@@ -230,9 +217,17 @@ int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch) 
     UDX_LAUNCHED();
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_THREADS), 0, s, part, nb, part_ex);
     UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part_ex, nb, out);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part_ex, nb, out, zero);
     UDX_LAUNCHED();
     return UDX_OK;
+}
+
+int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch) {
+    return scan_impl(s, (int *)in, n, out, scratch, 0);
+}
+
+int scan_exclusive_consume(hipStream_t s, int *in, int n, int *out, int *scratch) {
+    return scan_impl(s, in, n, out, scratch, 1);
 }
 
 int clist_alloc(ClistBufs *c, const Box &b, int maxp, int sub) {
@@ -293,7 +288,11 @@ int clist_clear(hipStream_t s, ClistBufs *c) {
 
 int clist_finish(hipStream_t s, ClistBufs *c, int n) {
     const int nc = c->box.ncells(), nb = nc * c->sub;
-    UDX_TRY(scan_exclusive(s, c->bcounts, nb, c->bstarts, c->scan_tmp));
+    // with x slabs the slab counts are read by the scan only: it clears them,
+    // so a move kernel can count the next step's slabs (counts[] of the
+    // cells come from the slab starts); without slabs counts[] is the output
+    if (c->sub > 1) UDX_TRY(scan_exclusive_consume(s, c->bcounts, nb, c->bstarts, c->scan_tmp));
+    else UDX_TRY(scan_exclusive(s, c->bcounts, nb, c->bstarts, c->scan_tmp));
     if (n > 0) {
         hipLaunchKernelGGL(k_clist_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c->cellslot,
                            c->bstarts, c->order);
@@ -312,6 +311,18 @@ int clist_finish(hipStream_t s, ClistBufs *c, int n) {
 
 int clist_sort(hipStream_t s, ClistBufs *c, int n, const float4 *pos4) {
     return clist_sort_impl(s, c, n, PosF4{pos4});
+}
+
+int clist_count_only(hipStream_t s, ClistBufs *c, int n, const float4 *pos4) {
+    UDX_CHECK(n >= 0 && n <= c->maxp, UDX_ECAP, "clist: %d particles exceed capacity %d", n, c->maxp);
+    UDX_TRY(clist_clear(s, c));
+    if (n > 0) {
+        const Box &b = c->box;
+        hipLaunchKernelGGL(k_clist_count<PosF4>, dim3(ceil_div(n, 256)), dim3(256), 0, s, PosF4{pos4}, n, b.L[0],
+                           b.L[1], b.L[2], c->sub, c->bcounts, c->cellslot);
+        UDX_LAUNCHED();
+    }
+    return UDX_OK;
 }
 
 int clist_sort_aos(hipStream_t s, ClistBufs *c, int n, const float *pp) {

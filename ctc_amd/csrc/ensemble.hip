@@ -52,34 +52,25 @@ __global__ __launch_bounds__(256) void k_ens_count(int n, int nbox, const int *p
                                                    const int4 *boxL, const float4 *pos, int *counts,
                                                    int2 *cellslot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    int cid = -1;
+    int b = 0;
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < n) {
-        const int b = find_box_wave(poff, nbox, i);
-        const int4 L = boxL[b];
-        const float4 r = pos[i];
-        int bx, cy, cz;
-        if (slab_coord(r.x, L.x, SOLVENT_SUB, &bx) && cell_coord(r.y, L.y, &cy) && cell_coord(r.z, L.z, &cz))
-            cid = coff[b] * SOLVENT_SUB + bx + L.x * SOLVENT_SUB * (cy + L.y * cz);
+        b = find_box_wave(poff, nbox, i);
+        r = pos[i];
     }
-    const int prev = __shfl_up(cid, 1, 64);
-    const bool head = lane == 0 || prev != cid;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-    const int hlane = 63 - __clzll(heads & upto);
-    const unsigned long long above = heads & ~upto;
-    const int next = above ? __ffsll(above) - 1 : 64;
-    int base = 0;
-    if (head && cid >= 0) base = atomicAdd(counts + cid, next - lane);
-    base = __shfl(base, hlane, 64);
-    if (i < n) cellslot[i] = make_int2(cid, base + (lane - hlane));
+    const int4 L = boxL[b];
+    clist_count_one(i < n, i, make_float3(r.x, r.y, r.z), L.x, L.y, L.z, SOLVENT_SUB, counts, cellslot,
+                    coff[b] * SOLVENT_SUB);
 }
 
-// cell counts of the 26 fragments of every box (as k_frag_counts)
+// cell counts of the 26 fragments of every box (as k_frag_counts); the
+// first threads also copy the step's seeds of every box from the run's table
 __global__ __launch_bounds__(256) void k_ens_frag_counts(int total, int nbox, const int *fbase, const int *foff,
                                                          const int *coff, const int4 *boxL, const int *starts,
-                                                         int *cnt) {
+                                                         int *cnt, const float *table, const int *step,
+                                                         float *seeds) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < 27 * nbox) seeds[g] = table[(long)*step * 27 * nbox + g];
     if (g >= total) return;
     const int b = find_box(fbase, nbox, g);
     const int4 L = boxL[b];
@@ -106,21 +97,33 @@ struct MoveBox {
     float mass, ax, ay, az;
 };
 
+// kick-drift + wrap, then the count of the next step's slabs (as
+// k_ens_count) and the step counter
 __global__ __launch_bounds__(256) void k_ens_move(int n, int nbox, const int *poff, const MoveBox *mb,
-                                                  const float4 *ff, float4 *pos, float4 *vel) {
+                                                  const float4 *ff, float4 *pos, float4 *vel, const int *coff,
+                                                  const int4 *boxL, int *counts, int2 *cellslot, int *step) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const MoveBox m = mb[find_box_wave(poff, nbox, i)];
-    float4 p = pos[i], v = vel[i];
-    const float4 f = ff[i];
-    kick_drift(m.dt, m.mass, m.ax, f.x, &p.x, &v.x);
-    kick_drift(m.dt, m.mass, m.ay, f.y, &p.y, &v.y);
-    kick_drift(m.dt, m.mass, m.az, f.z, &p.z, &v.z);
-    p.x = wrap_coord(p.x, m.Lx);
-    p.y = wrap_coord(p.y, m.Ly);
-    p.z = wrap_coord(p.z, m.Lz);
-    pos[i] = p;
-    vel[i] = v;
+    if (i == 0) *step += 1;
+    int b = 0;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n) {
+        b = find_box_wave(poff, nbox, i);
+        const MoveBox m = mb[b];
+        p = pos[i];
+        float4 v = vel[i];
+        const float4 f = ff[i];
+        kick_drift(m.dt, m.mass, m.ax, f.x, &p.x, &v.x);
+        kick_drift(m.dt, m.mass, m.ay, f.y, &p.y, &v.y);
+        kick_drift(m.dt, m.mass, m.az, f.z, &p.z, &v.z);
+        p.x = wrap_coord(p.x, m.Lx);
+        p.y = wrap_coord(p.y, m.Ly);
+        p.z = wrap_coord(p.z, m.Lz);
+        pos[i] = p;
+        vel[i] = v;
+    }
+    const int4 L = boxL[b];
+    clist_count_one(i < n, i, make_float3(p.x, p.y, p.z), L.x, L.y, L.z, SOLVENT_SUB, counts, cellslot,
+                    coff[b] * SOLVENT_SUB);
 }
 
 // periodic wrap of the particles into their boxes (set_pp): the boxes keep
@@ -134,17 +137,6 @@ __global__ __launch_bounds__(256) void k_ens_wrap(int n, int nbox, const int *po
     p.y = wrap_coord(p.y, m.Ly);
     p.z = wrap_coord(p.z, m.Lz);
     pos[i] = p;
-}
-
-// the step's 27 seeds of every box from the run's table, then the counter
-__global__ void k_ens_seeds(int nper, const float *table, int *step, float *cur) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = *step;
-    if (k < nper) cur[k] = table[(long)s * nper + k];
-}
-
-__global__ void k_ens_step(int *step) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 __global__ void k_ens_unpack(int n, const float4 *pos, const float4 *vel, float *pp) {
@@ -166,6 +158,7 @@ struct udx_ens {
     hipStream_t s;                 // own stream (graph capture needs a non-null stream)
     hipEvent_t done;
     int nbox, ntot, ctot, ftot, mode, variant, cur, use_graph, seed_cap;
+    int counted;                   // the last move counted pos[cur] into the slabs
     std::vector<udx_ens_box> box;
     std::vector<int> poff_h, coff_h, fbase_h, foff_h;
     std::vector<udx_rnd> trunk;    // 27 per box
@@ -215,28 +208,33 @@ static int ens_build_batches(udx_ens *e) {
 }
 
 // one step: state in pos/vel[cur] -> pos/vel[1 - cur]
+static int ens_count(udx_ens *e, int cur) {
+    const int n = e->ntot;
+    UDX_TRY(clist_clear(e->s, &e->cl));
+    hipLaunchKernelGGL(k_ens_count, dim3(ceil_div(n, 256)), dim3(256), 0, e->s, n, e->nbox, e->poff, e->coff,
+                       e->boxL, e->pos[cur], e->cl.bcounts, e->cl.cellslot);
+    UDX_LAUNCHED();
+    return UDX_OK;
+}
+
+// The move counts the next step's slabs, so a step starts from the scan
+// (ens_count first when the positions were set since).
 static int ens_step(udx_ens *e, int cur) {
     hipStream_t s = e->s;
     const int a = cur, b = 1 - cur, n = e->ntot;
-    hipLaunchKernelGGL(k_ens_seeds, dim3(ceil_div(27 * e->nbox, 256)), dim3(256), 0, s, 27 * e->nbox,
-                       e->seeds_table, e->step, e->seeds_cur);
-    UDX_LAUNCHED();
-    UDX_TRY(clist_clear(s, &e->cl));
-    hipLaunchKernelGGL(k_ens_count, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, e->nbox, e->poff, e->coff,
-                       e->boxL, e->pos[a], e->cl.bcounts, e->cl.cellslot);
-    UDX_LAUNCHED();
+    if (!e->counted) UDX_TRY(ens_count(e, a));
     UDX_TRY(clist_finish(s, &e->cl, n));
     UDX_TRY(gather_f4(s, &e->cl, n, e->pos[a], e->vel[a], e->pos[b], e->vel[b]));
-    hipLaunchKernelGGL(k_ens_frag_counts, dim3(ceil_div(e->ftot, 256)), dim3(256), 0, s, e->ftot, e->nbox,
-                       e->fbase, e->foff, e->coff, e->boxL, e->cl.starts, e->fcnt);
+    const int nf = e->ftot > 27 * e->nbox ? e->ftot : 27 * e->nbox;
+    hipLaunchKernelGGL(k_ens_frag_counts, dim3(ceil_div(nf, 256)), dim3(256), 0, s, e->ftot, e->nbox, e->fbase,
+                       e->foff, e->coff, e->boxL, e->cl.starts, e->fcnt, e->seeds_table, e->step, e->seeds_cur);
     UDX_LAUNCHED();
     UDX_TRY(scan_exclusive(s, e->fcnt, e->ftot, e->fss, e->fscan));
     UDX_TRY(dpd_forces_batch(s, e->batch[b], e->mode, e->variant));
-    hipLaunchKernelGGL(k_ens_move, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, e->nbox, e->poff, e->mb, e->ff,
-                       e->pos[b], e->vel[b]);
+    hipLaunchKernelGGL(k_ens_move, dim3(ceil_div(n > 0 ? n : 1, 256)), dim3(256), 0, s, n, e->nbox, e->poff, e->mb,
+                       e->ff, e->pos[b], e->vel[b], e->coff, e->boxL, e->cl.bcounts, e->cl.cellslot, e->step);
     UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_ens_step, dim3(1), dim3(64), 0, s, e->step);
-    UDX_LAUNCHED();
+    e->counted = 1;
     return UDX_OK;
 }
 
@@ -250,11 +248,17 @@ static void ens_drop_graph(udx_ens *e) {
 // the even/odd step pair as one graph (state ends where it started)
 static int ens_capture(udx_ens *e) {
     ens_drop_graph(e);
+    // the captured steps start from counted positions (udx_ens_run counts
+    // before the first replay when they are not)
+    const int counted0 = e->counted;
+    e->counted = 1;
     UDX_HIP(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
     int st = ens_step(e, e->cur);
     if (!st) st = ens_step(e, 1 - e->cur);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e->s, &g);
+    e->counted = counted0;
+    if (st && g) (void)hipGraphDestroy(g);
     UDX_TRY(st);
     UDX_CHECK(ce == hipSuccess && g, UDX_EHIP, "ensemble: graph capture failed: %s", hipGetErrorString(ce));
     e->graph = g;
@@ -457,6 +461,7 @@ int udx_ens_set_pp(udx_ens *e, const float *pp) {
                            e->mb, e->pos[0]);
     UDX_LAUNCHED();
     e->cur = 0;
+    e->counted = 0;
     UDX_HIP(hipEventRecord(e->done, e->s));
     UDX_HIP(hipStreamWaitEvent(e->ctx->stream, e->done, 0));
     return UDX_OK;
@@ -481,6 +486,10 @@ int udx_ens_run(udx_ens *e, int nsteps, const float *seeds) {
     UDX_TRY(ens_seed_table(e, nsteps, seeds));
     int k = 0;
     if (e->use_graph && nsteps >= 2) {
+        if (!e->counted) {
+            UDX_TRY(ens_count(e, e->cur));
+            e->counted = 1;
+        }
         if (!e->exec || e->graph_variant != e->variant * 2 + e->cur) UDX_TRY(ens_capture(e));
         for (; k + 2 <= nsteps; k += 2) UDX_HIP(hipGraphLaunch(e->exec, e->s));
     }

@@ -8,23 +8,32 @@ namespace udx {
 
 namespace {
 
+// With sc.counts set (and the wrap) the kernel also counts the new
+// positions into the next step's x slabs, as the cell list's counting pass
+// would: the next sort starts from its scan.
 __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
                                                  float dt, float mass, float ax, float ay, float az,
-                                                 const float4 *ff, float4 *pos, float4 *vel, int wrap) {
+                                                 const float4 *ff, float4 *pos, float4 *vel, int wrap, SlabCount sc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nmax || i >= *n_dev) return;
-    float4 p = pos[i], v = vel[i];
-    const float4 f = ff[i];
-    kick_drift(dt, mass, ax, f.x, &p.x, &v.x);
-    kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
-    kick_drift(dt, mass, az, f.z, &p.z, &v.z);
-    if (wrap) {
-        p.x = wrap_coord(p.x, Lx);
-        p.y = wrap_coord(p.y, Ly);
-        p.z = wrap_coord(p.z, Lz);
+    const bool valid = i < nmax && i < *n_dev;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
+        p = pos[i];
+        float4 v = vel[i];
+        const float4 f = ff[i];
+        kick_drift(dt, mass, ax, f.x, &p.x, &v.x);
+        kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
+        kick_drift(dt, mass, az, f.z, &p.z, &v.z);
+        if (wrap) {
+            p.x = wrap_coord(p.x, Lx);
+            p.y = wrap_coord(p.y, Ly);
+            p.z = wrap_coord(p.z, Lz);
+        }
+        pos[i] = p;
+        vel[i] = v;
     }
-    pos[i] = p;
-    vel[i] = v;
+    if (sc.counts) clist_count_one(valid, i, make_float3(p.x, p.y, p.z), sc.L[0], sc.L[1], sc.L[2], sc.sub, sc.counts,
+                                   sc.cellslot);
 }
 
 // the same with the wall bounce-back between the drift and the wrap (the
@@ -33,26 +42,32 @@ __global__ __launch_bounds__(256) void k_move_f4(int nmax, const int *n_dev, flo
 __global__ __launch_bounds__(256) void k_move_f4_wall(int nmax, const int *n_dev, float Lx, float Ly, float Lz,
                                                       float dt, float mass, float ax, float ay, float az,
                                                       const float4 *ff, float4 *pos, float4 *vel, SdfView sdf,
-                                                      WallVel wv, int wrap) {
+                                                      WallVel wv, int wrap, SlabCount sc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nmax || i >= *n_dev) return;
-    float4 p = pos[i], v = vel[i];
-    const float4 f = ff[i];
-    kick_drift(dt, mass, ax, f.x, &p.x, &v.x);
-    kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
-    kick_drift(dt, mass, az, f.z, &p.z, &v.z);
-    float r[3] = {p.x, p.y, p.z}, u[3] = {v.x, v.y, v.z};
-    if (sdf_bounce(sdf, wv, dt, r, u)) {
-        p.x = r[0]; p.y = r[1]; p.z = r[2];
-        v.x = u[0]; v.y = u[1]; v.z = u[2];
+    const bool valid = i < nmax && i < *n_dev;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
+        p = pos[i];
+        float4 v = vel[i];
+        const float4 f = ff[i];
+        kick_drift(dt, mass, ax, f.x, &p.x, &v.x);
+        kick_drift(dt, mass, ay, f.y, &p.y, &v.y);
+        kick_drift(dt, mass, az, f.z, &p.z, &v.z);
+        float r[3] = {p.x, p.y, p.z}, u[3] = {v.x, v.y, v.z};
+        if (sdf_bounce(sdf, wv, dt, r, u)) {
+            p.x = r[0]; p.y = r[1]; p.z = r[2];
+            v.x = u[0]; v.y = u[1]; v.z = u[2];
+        }
+        if (wrap) {
+            p.x = wrap_coord(p.x, Lx);
+            p.y = wrap_coord(p.y, Ly);
+            p.z = wrap_coord(p.z, Lz);
+        }
+        pos[i] = p;
+        vel[i] = v;
     }
-    if (wrap) {
-        p.x = wrap_coord(p.x, Lx);
-        p.y = wrap_coord(p.y, Ly);
-        p.z = wrap_coord(p.z, Lz);
-    }
-    pos[i] = p;
-    vel[i] = v;
+    if (sc.counts) clist_count_one(valid, i, make_float3(p.x, p.y, p.z), sc.L[0], sc.L[1], sc.L[2], sc.sub, sc.counts,
+                                   sc.cellslot);
 }
 
 // reference kick-drift on AoS particles, no wrap (src/scheme/move/dev/main.h:1-12)
@@ -191,22 +206,35 @@ int f4_to_force_aos(hipStream_t s, int n, const int *n_dev, const float4 *ff, fl
     return UDX_OK;
 }
 
+static SlabCount slab_count(ClistBufs *next, int wrap) {
+    SlabCount c{};
+    if (next && wrap) {
+        for (int k = 0; k < 3; k++) c.L[k] = next->box.L[k];
+        c.sub = next->sub;
+        c.counts = next->bcounts;
+        c.cellslot = next->cellslot;
+    }
+    return c;
+}
+
 int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-            const float4 *ff, float4 *pos, float4 *vel, int wrap) {
+            const float4 *ff, float4 *pos, float4 *vel, int wrap, ClistBufs *next) {
     if (nmax <= 0) return UDX_OK;
     hipLaunchKernelGGL(k_move_f4, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
-                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, wrap);
+                       (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, wrap,
+                       slab_count(next, wrap));
     UDX_LAUNCHED();
     return UDX_OK;
 }
 
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv, int wrap) {
-    if (!q) return move_f4(s, nmax, n_dev, b, dt, mass, accel, ff, pos, vel, wrap);
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv, int wrap,
+                 ClistBufs *next) {
+    if (!q) return move_f4(s, nmax, n_dev, b, dt, mass, accel, ff, pos, vel, wrap, next);
     if (nmax <= 0) return UDX_OK;
     hipLaunchKernelGGL(k_move_f4_wall, dim3(ceil_div(nmax, 256)), dim3(256), 0, s, nmax, n_dev, (float)b.L[0],
                        (float)b.L[1], (float)b.L[2], dt, mass, accel[0], accel[1], accel[2], ff, pos, vel, q->view,
-                       wv, wrap);
+                       wv, wrap, slab_count(next, wrap));
     UDX_LAUNCHED();
     return UDX_OK;
 }

@@ -124,6 +124,42 @@ __device__ __forceinline__ bool slab_coord(float r, int L, int sub, int *c) {
     return true;
 }
 
+// where a move kernel counts the next step's x slabs (counts null: no count)
+struct SlabCount {
+    int L[3], sub;
+    int *counts;
+    int2 *cellslot;
+};
+
+// One particle of a counting pass (every lane of the wave calls it; valid
+// selects the lanes with a particle): its x slab (cell * sub + slab) and a
+// provisional slot in it, cellslot[i] = (slab or -1 outside the box, slot).
+// Consecutive lanes with the same slab (the usual case: the particles are
+// the previous step's cell-ordered array) form a segment that takes its slots
+// with one atomic.
+// slab0: the first slab of the particle's box (boxes laid end to end).
+__device__ __forceinline__ void clist_count_one(bool valid, int i, float3 r, int Lx, int Ly, int Lz, int sub,
+                                                int *counts, int2 *cellslot, int slab0 = 0) {
+    const int lane = threadIdx.x & 63;
+    int cid = -1;
+    if (valid) {
+        int bx, cy, cz;
+        if (slab_coord(r.x, Lx, sub, &bx) && cell_coord(r.y, Ly, &cy) && cell_coord(r.z, Lz, &cz))
+            cid = slab0 + bx + Lx * sub * (cy + Ly * cz);
+    }
+    const int prev = __shfl_up(cid, 1, 64);
+    const bool head = lane == 0 || prev != cid;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int hlane = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int next = above ? __ffsll(above) - 1 : 64;
+    int base = 0;
+    if (head && cid >= 0) base = atomicAdd(counts + cid, next - lane);
+    base = __shfl(base, hlane, 64);
+    if (valid) cellslot[i] = make_int2(cid, base + (lane - hlane));
+}
+
 // ---------------------------------------------------------------- DPD pair
 // Correctly rounded square root of a positive normal float (or 0): the
 // hardware root (within 1 ulp) moved to a neighbour when an fma residual

@@ -67,6 +67,8 @@ struct Box {
 // scan_scratch_size(n) ints
 int scan_scratch_size(int n);
 int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch);
+// the same, clearing in[] as it is read
+int scan_exclusive_consume(hipStream_t s, int *in, int n, int *out, int *scratch);
 
 // ------------------------------------------------------------------ cell list
 // Particles are sorted by cell and, inside a cell, by x slab: every cell is
@@ -99,6 +101,8 @@ int clist_sort_aos(hipStream_t s, ClistBufs *c, int n, const float *pp);
 // and the cell tables
 int clist_clear(hipStream_t s, ClistBufs *c);
 int clist_finish(hipStream_t s, ClistBufs *c, int n);
+// the counting half alone (clist_finish completes the sort)
+int clist_count_only(hipStream_t s, ClistBufs *c, int n, const float4 *pos4);
 // out[k] = in[order[k]] for k < starts[ncells]; n bounds the launch
 int gather_f4(hipStream_t s, const ClistBufs *c, int n, const float4 *pin, const float4 *vin,
               float4 *pout, float4 *vout);
@@ -170,8 +174,11 @@ int dpd_forces_batch(hipStream_t s, const DpdBatch &b, int mode, int variant);
 int aos_to_f4(hipStream_t s, int n, const float *pp, float4 *pos, float4 *vel);
 int f4_to_aos(hipStream_t s, int n, const int *n_dev, const float4 *pos, const float4 *vel, float *pp);
 int f4_to_force_aos(hipStream_t s, int n, const int *n_dev, const float4 *ff, float *out);
+// next (with wrap): also count the moved particles into next's x slabs, the
+// counting pass of its next sort (clist_finish then completes it)
 int move_f4(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass,
-            const float accel[3], const float4 *ff, float4 *pos, float4 *vel, int wrap = 1);
+            const float accel[3], const float4 *ff, float4 *pos, float4 *vel, int wrap = 1,
+            ClistBufs *next = nullptr);
 int move_aos(hipStream_t s, int n, float dt, float mass, const float *ff, float *pp);
 int bforce_aos(hipStream_t s, int n, float mass, const float accel[3], float *ff);
 int wrap_aos(hipStream_t s, const Box &b, int n, float *pp);
@@ -206,7 +213,8 @@ int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp
 WallVel wall_vel_view(const udx_wall_vel &v, float t);
 int wall_repulse_aos(hipStream_t s, const SdfField *q, float lambda, int n, const float *pp, float *ff);
 int move_f4_wall(hipStream_t s, int nmax, const int *n_dev, const Box &b, float dt, float mass, const float accel[3],
-                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv, int wrap = 1);
+                 const float4 *ff, float4 *pos, float4 *vel, const SdfField *q, const WallVel &wv, int wrap = 1,
+                 ClistBufs *next = nullptr);
 // membrane bounce-back inside the solvent step: rp, rc the vertices (AoS) at
 // the start of the step and after the membrane advance (velocities updated),
 // pos0 the solvent positions at the start of the step; applies the periodic
