@@ -90,6 +90,8 @@ struct udx_fluforces {
     Trunks trunks;
     unsigned long long *pairs_dev;
     double *acc;        // FP64 force sums of the fast pass (3 per particle, kept zero between calls)
+    int *ecount, *eidx; // edge list of the gather's halo pass
+    float4 *ef;
 };
 
 struct udx_sdf {
@@ -121,6 +123,8 @@ struct udx_flu {
     int n;
     float4 *pos[2], *vel[2], *ff;
     double *acc;        // FP64 force sums of the fast pass (3 per particle, kept zero between steps)
+    int *ecount, *eidx; // edge list of the gather's halo pass
+    float4 *ef;
     int cur;
     ClistBufs cl;
     // the last step's move counted pos[cur] into the cell list's x slabs, so
@@ -317,6 +321,9 @@ int udx_fluforces_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_flufor
     if (!s) s = dev_alloc(&f->pairs_dev, 1);
     if (!s) s = frag_alloc(&f->frag, f->box);
     if (!s) s = dev_alloc(&f->acc, 3 * (size_t)(maxp > 0 ? maxp : 1));
+    if (!s) s = dev_alloc(&f->ecount, 1);
+    if (!s) s = dev_alloc(&f->eidx, (size_t)(maxp > 0 ? maxp : 1));
+    if (!s) s = dev_alloc(&f->ef, (size_t)(maxp > 0 ? maxp : 1));
     if (!s && (hipMemset(f->pairs_dev, 0, sizeof(unsigned long long)) != hipSuccess ||
                hipMemset(f->acc, 0, sizeof(double) * 3 * (size_t)(maxp > 0 ? maxp : 1)) != hipSuccess)) {
         set_error("udx_fluforces_ini: memset failed");
@@ -337,6 +344,9 @@ int udx_fluforces_fin(udx_fluforces *f) {
     dev_free(f->seeds_dev);
     dev_free(f->pairs_dev);
     dev_free(f->acc);
+    dev_free(f->ecount);
+    dev_free(f->eidx);
+    dev_free(f->ef);
     frag_free(&f->frag);
     delete f;
     return UDX_OK;
@@ -383,7 +393,7 @@ static int fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *s
     hipStream_t s = f->ctx->stream;
     if (f->n == 0) return UDX_OK;
     UDX_HIP(hipMemcpyAsync(f->seeds_dev, seeds, 27 * sizeof(float), hipMemcpyHostToDevice, s));
-    UDX_TRY(frag_build(s, &f->frag, f->box, starts));
+    UDX_TRY(frag_build(s, &f->frag, f->box, starts, f->ecount));
     DpdLaunch a;
     a.pos = f->pos;
     a.vel = f->vel;
@@ -400,6 +410,9 @@ static int fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *s
     a.pairs = f->pairs_dev;
     a.ss = ss;
     a.acc = f->acc;
+    a.ecount = f->ecount;
+    a.eidx = f->eidx;
+    a.ef = f->ef;
     UDX_TRY(dpd_forces_mode(s, a, f->mode));
     // the seeds live on the host stack: wait for the copy before returning
     UDX_HIP(hipStreamSynchronize(s));
@@ -412,6 +425,8 @@ int udx_fluforces_apply(udx_fluforces *f, const udx_pair_dpd *p, const int *star
     f->trunks.next(seeds);
     return udx_fluforces_apply_seeds(f, p, starts, periodic, seeds, ff);
 }
+
+int udx_dpd_set_split_edges(double n) { return dpd_set_split_edges(n); }
 
 int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset) {
     UDX_CHECK(f && npairs, UDX_EINVAL, "udx_fluforces_pairs: null argument");
@@ -473,6 +488,9 @@ int udx_flu_ini(udx_ctx *ctx, const udx_flu_conf *conf, udx_flu **out) {
     }
     if (!s) s = dev_alloc(&f->ff, conf->maxp);
     if (!s) s = dev_alloc(&f->acc, 3 * (size_t)conf->maxp);
+    if (!s) s = dev_alloc(&f->ecount, 1);
+    if (!s) s = dev_alloc(&f->eidx, (size_t)conf->maxp);
+    if (!s) s = dev_alloc(&f->ef, (size_t)conf->maxp);
     if (!s && hipMemset(f->acc, 0, sizeof(double) * 3 * (size_t)conf->maxp) != hipSuccess) {
         set_error("udx_flu_ini: memset failed");
         s = UDX_EHIP;
@@ -500,6 +518,9 @@ int udx_flu_fin(udx_flu *f) {
     }
     dev_free(f->ff);
     dev_free(f->acc);
+    dev_free(f->ecount);
+    dev_free(f->eidx);
+    dev_free(f->ef);
     clist_free(&f->cl);
     frag_free(&f->frag);
     dev_free(f->seeds_dev);
@@ -596,7 +617,7 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     if (f->counted) UDX_TRY(clist_finish(s, &f->cl, f->n));
     else UDX_TRY(clist_sort(s, &f->cl, f->n, f->pos[a]));
     UDX_TRY(gather_f4(s, &f->cl, f->n, f->pos[a], f->vel[a], f->pos[b], f->vel[b]));
-    UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts));
+    UDX_TRY(frag_build(s, &f->frag, f->box, f->cl.starts, f->ecount));
     if (m.r) UDX_TRY(rbc_redistribute(s, f->box, m.nc, m.nv, m.pp, m.tmp, m.cfid, m.cnew));
     const bool bounce = m.r && m.bb;
     // objects_save_mesh (src/sim/imp/run.h:30-31): the vertices at the start of the step
@@ -620,6 +641,9 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     d.accumulate = 0;
     d.pairs = f->pairs_dev;
     d.acc = f->acc;
+    d.ecount = f->ecount;
+    d.eidx = f->eidx;
+    d.ef = f->ef;
     UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
     if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
     const SdfField *sdf = nullptr;

@@ -51,6 +51,11 @@ struct DpdK {
     unsigned long long *pairs;
     float *ss;                // per-particle stress (6 floats, added) or null
     double *acc;              // FP64 force accumulators (half-shell pass) or null
+    // edge list (null: the halo pass runs inside the force kernel): the
+    // particles with pairs across the periodic boundary and their bulk sums,
+    // box entries from i0, *ecount of them
+    int *ecount, *eidx;
+    float4 *ef;
 };
 
 // What the force pass reads of a box, read once at kernel entry.  The
@@ -191,6 +196,24 @@ __device__ __forceinline__ PairIn pair_in_halo(const DpdK &A, const DpdP &P, uns
     return o;
 }
 
+// a wrapped pair from its queue entry (j | w << 27) and its noise indices u, v
+// (positions in the packed fragment lists, set at the push): the local
+// particle moved into the neighbour frame, noise mean0var1uu(seed_w, u, v)
+// (src/fluforces/halo/dev/main.h:7-12,40-48).  An idle entry (w = W_NONE, the
+// particle itself) gives a zero force that is not added.
+__device__ __forceinline__ PairIn pair_in_wrapped(const DpdP &P, unsigned ent, int u, int v, const float4 &pi,
+                                                  const float4 &vi) {
+    const unsigned w = ent >> 27;
+    const int j = (int)(ent & 0x7FFFFFFu);
+    const float4 pj = P.p(j), vj = P.v(j);
+    const int wx = (int)(w % 3u) - 1, wy = (int)((w / 3u) % 3u) - 1, wz = (int)(w / 9u) - 1;
+    const float ax = pi.x - (float)(wx * P.Lx), ay = pi.y - (float)(wy * P.Ly), az = pi.z - (float)(wz * P.Lz);
+    PairIn o;
+    pair_kin(o, ax, ay, az, vi, pj, vj);
+    o.x = P.seeds[frag_d2i(wx, wy, wz)] - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
+    return o;
+}
+
 // The hardware hands workgroups to the 8 XCDs round robin; renumbering them
 // gives each XCD one contiguous run of cells, so neighbour rows are reused
 // from that XCD's L2 instead of being fetched again by the others.
@@ -246,6 +269,12 @@ __device__ __forceinline__ void run_range(const DpdP &P, int base, int x0, int x
 template <bool B>
 using PassTag = std::integral_constant<bool, B>;
 
+// what a launch of the gather does: both passes, or the bulk pass with the
+// particles that have wrapped pairs handed to an edge list, or the halo pass
+// of the listed particles (their sums continue from the listed bulk sums, so
+// the order of the additions is that of one pass)
+enum { PH_ALL = 0, PH_BULK = 1, PH_HALO = 2 };
+
 // The force pass of one block of NT consecutive particles starting at i
 // = first + threadIdx.x; QC (a power of two) queued pairs per lane.
 //
@@ -257,7 +286,7 @@ using PassTag = std::integral_constant<bool, B>;
 // candidates per iteration.  Halo pass (periodic boxes, only waves with a
 // particle in a boundary cell): the wrapped parts of the 9 rows, read from
 // global memory.  Each lane sums its pairs in candidate order, bulk first.
-template <int WR, bool EXACT, int NT, int QC, int SC, bool STRESS = false>
+template <int WR, bool EXACT, int NT, int QC, int SC, bool STRESS = false, int PHASE = PH_ALL>
 __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     static_assert(QC >= 8 && (QC & (QC - 1)) == 0, "the queue length is a power of two");
     __shared__ unsigned q[QC * NT];
@@ -267,8 +296,24 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     const DpdP P = dpd_view(A);
     const int Lx = P.Lx, Ly = P.Ly, Lz = P.Lz;
     const int n = P.starts[Lx * Ly * Lz];
-    const int i = first + tid;
-    const bool active = i < n && i - P.i0 < P.nmax;
+    int i = first + tid;
+    bool active = i < n && i - P.i0 < P.nmax;
+    float f0x = 0.f, f0y = 0.f, f0z = 0.f;   // the listed bulk sums (halo phase)
+    if (PHASE == PH_HALO) {
+        // entries first + tid of the box's edge list; a block past its end has nothing to do
+        const int ne = *A.ecount;
+        if (first - P.i0 >= ne) return;
+        active = first + tid - P.i0 < ne;
+        i = P.i0;
+        if (active) {
+            const int k = first + tid;
+            i = A.eidx[k];
+            const float4 f = A.ef[k];
+            f0x = f.x;
+            f0y = f.y;
+            f0z = f.z;
+        }
+    }
 
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
     int cx = 0, cy = 0, cz = 0, istart = 0;
@@ -284,7 +329,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         rz = (float)((double)pi.z + 0.5 * (double)Lz - (double)cz);
     }
     const float seed_bulk = P.seeds[26];
-    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float fx = f0x, fy = f0y, fz = f0z;
     // stress 1/2 f_a dr_b, (xx, xy, xz, yy, yz, zz) (src/pair/dev.h:87-100)
     float sxx = 0.f, sxy = 0.f, sxz = 0.f, syy = 0.f, syz = 0.f, szz = 0.f;
     int npair = 0;
@@ -418,15 +463,77 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             }
         }
     };
-    for (int row = 0; row < 4; row++) bulk_row(row, PassTag<false>{});
-    bulk_row(4, PassTag<true>{});
-    for (int row = 5; row < 9; row++) bulk_row(row, PassTag<false>{});
-    drain(PassTag<true>{});
+    if (PHASE != PH_HALO) {
+        for (int row = 0; row < 4; row++) bulk_row(row, PassTag<false>{});
+        bulk_row(4, PassTag<true>{});
+        for (int row = 5; row < 9; row++) bulk_row(row, PassTag<false>{});
+        drain(PassTag<true>{});
+    }
 
-    // ---- halo pass: the cells across the periodic boundary
+    // ---- halo pass: the cells across the periodic boundary.  The noise
+    // indices of a wrapped pair are positions in packed fragment lists: the
+    // home particle's in fragment w is one value per run, and the candidate's
+    // in the opposite fragment is B + j with B one value per run (consecutive
+    // cells of a row are consecutive in a fragment's list), so both are set at
+    // the push and queued with the entry (three rings of HQC slots per lane in
+    // the memory of the bulk queue and the staging buffer).
     const bool edge =
         active && P.periodic && (cx == 0 || cx == Lx - 1 || cy == 0 || cy == Ly - 1 || cz == 0 || cz == Lz - 1);
-    if (wave_any(edge)) {
+    bool listed = false;
+    if (PHASE == PH_BULK) {
+        // hand the particles with wrapped pairs to the edge list, one slot
+        // range per wave (their order in the list does not matter)
+        const unsigned long long em = __ballot(edge);
+        int base = 0;
+        if ((tid & 63) == 0 && em) base = atomicAdd(A.ecount, __popcll(em));
+        base = __shfl(base, 0, 64);
+        if (edge) {
+            const int k = P.i0 + base +
+                          (int)__builtin_amdgcn_mbcnt_hi((unsigned)(em >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)em, 0u));
+            A.eidx[k] = i;
+            A.ef[k] = make_float4(fx, fy, fz, 0.f);
+        }
+        listed = edge;
+    }
+    if (PHASE != PH_BULK && wave_any(edge)) {
+        constexpr int HQC = QC / 2 < 8 ? QC / 2 : 8;
+        static_assert(2 * HQC <= QC && HQC * 64 * 4 <= (SC + 4) * 16, "halo rings fit the bulk queue and staging");
+        constexpr unsigned HSTEP = 4u * NT, HMASK = (unsigned)(HQC - 1) * HSTEP;
+        // rings j|w and u in the bulk queue (slot k of every lane of the
+        // block: HSTEP bytes), ring v in the wave's staging buffer (256 bytes
+        // per slot)
+        unsigned *const rj = q, *const ru = q + HQC * NT;
+        char *const rv = (char *)&sbuf[tid >> 6][0] + 4 * (tid & 63);
+        auto hs = [&](unsigned *base, unsigned k) { return (unsigned *)((char *)base + ((k & HMASK) | qlane)); };
+        auto hv = [&](unsigned k) { return (unsigned *)(rv + (k & HMASK) / (NT / 64)); };
+        unsigned hh = 0, ht = 0;
+        auto hstep = [&]() {
+            const unsigned fl = ht - hh;
+            const bool va = fl > 0u, vb = fl > HSTEP;
+            const unsigned ea = va ? *hs(rj, hh) : idle, eb = vb ? *hs(rj, hh + HSTEP) : idle;
+            const unsigned ua = va ? *hs(ru, hh) : 0u, ub = vb ? *hs(ru, hh + HSTEP) : 0u;
+            const unsigned xa_ = va ? *hv(hh) : 0u, xb_ = vb ? *hv(hh + HSTEP) : 0u;
+            const PairIn a = pair_in_wrapped(P, ea, (int)ua, (int)xa_, pi, vi);
+            const PairIn b = pair_in_wrapped(P, eb, (int)ub, (int)xb_, pi, vi);
+            float xa = a.x, xb = b.x;
+            logistic18x2(&xa, &xb);
+            const float3 f1 = dpd_pair<WR, EXACT>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
+            const float3 f2 = dpd_pair<WR, EXACT>(P.par, b.dx, b.dy, b.dz, b.r2, b.dvx, b.dvy, b.dvz, xb * R_SQRT2);
+            add(va, f1, a);
+            add(vb, f2, b);
+            const unsigned took = fl < 2u * HSTEP ? fl : 2u * HSTEP;
+            hh += took;
+            npair += (int)(took / HSTEP);
+        };
+        auto hpush = [&](bool hit, unsigned e, int u, int v) {
+            if (hit) {
+                *hs(rj, ht) = e;
+                *hs(ru, ht) = (unsigned)u;
+                *hv(ht) = (unsigned)v;
+            }
+            ht += hit ? HSTEP : 0u;
+        };
+        __builtin_amdgcn_wave_barrier();
         for (int row = 0; row < 9; row++) {
             const int dz = row / 3 - 1, dy = row % 3 - 1;
             int yy = cy + dy, zz = cz + dz;
@@ -435,39 +542,62 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             zz -= wz * Lz;
             const bool yzwrap = (wy | wz) != 0;
             int j0 = 0, n0 = 0, j1 = 0, n1 = 0, wx0 = 0, wx1 = 0, x0, x1;
+            // per run: the home particle's list position aid, the candidates' B,
+            // and which of the two is the first noise index (mask)
+            int aid0 = 0, b0 = 0, aid1 = 0, b1 = 0;
+            bool m0 = false, m1 = false;
+            auto runtab = [&](int wx, int xs, int *aid, int *bb, bool *mk) {
+                const int fid = frag_d2i(wx, wy, wz), anti = frag_d2i(-wx, -wy, -wz);
+                const int fo = A.foff[fid], ao = A.foff[anti];
+                *aid = P.fss[fo + frag_cell(wx, wy, wz, cx, cy, cz, Lx, Ly)] - P.fss[fo] + (i - istart);
+                *bb = P.fss[ao + frag_cell(-wx, -wy, -wz, xs, yy, zz, Lx, Ly)] - P.fss[ao] -
+                      P.starts[xs + Lx * (yy + Ly * zz)];
+                *mk = fid < anti;
+            };
             if (edge) {
                 const float hw = row_halfwidth(dy, dz, ry, rz);
                 const int base = Lx * (yy + Ly * zz);
-                if (xrun(0, cx, Lx, &x0, &x1, &wx0) && (yzwrap || wx0 != 0))
+                if (xrun(0, cx, Lx, &x0, &x1, &wx0) && (yzwrap || wx0 != 0)) {
                     run_range(P, base, x0, x1, pi.x - (float)(wx0 * Lx), hw, &j0, &n0);
-                if (xrun(1, cx, Lx, &x0, &x1, &wx1) && (yzwrap || wx1 != 0))
+                    if (n0 > 0) runtab(wx0, x0, &aid0, &b0, &m0);
+                }
+                if (xrun(1, cx, Lx, &x0, &x1, &wx1) && (yzwrap || wx1 != 0)) {
                     run_range(P, base, x0, x1, pi.x - (float)(wx1 * Lx), hw, &j1, &n1);
+                    if (n1 > 0) runtab(wx1, x0, &aid1, &b1, &m1);
+                }
             }
             const float ay = pi.y - (float)(wy * Ly), az = pi.z - (float)(wz * Lz);
             const float ax0 = pi.x - (float)(wx0 * Lx), ax1 = pi.x - (float)(wx1 * Lx);
             const unsigned w0 = wcode(wx0, wy, wz) << 27, w1 = wcode(wx1, wy, wz) << 27;
             const int len = n0 + n1;
             for (int t = 0; wave_any(t < len); t += 2) {
-                make_room(PassTag<false>{}, 2u);
+                if (wave_any(ht - hh + 2u * HSTEP > (unsigned)HQC * HSTEP))
+#pragma unroll 1
+                    for (int k = 0; k < DRAIN_STEPS; k++) hstep();
                 const int ta = t, tb = t + 1;
                 const bool oka = ta < len, okb = tb < len;
-                const int ja = ta >= n0 ? j1 + (ta - n0) : j0 + ta;
-                const int jb = tb >= n0 ? j1 + (tb - n0) : j0 + tb;
+                const bool sa = ta >= n0, sb = tb >= n0;
+                const int ja = sa ? j1 + (ta - n0) : j0 + ta;
+                const int jb = sb ? j1 + (tb - n0) : j0 + tb;
                 const float4 pa = P.p(oka ? ja : P.i0);
                 const float4 pb = P.p(okb ? jb : P.i0);
-                const float axa = ta >= n0 ? ax1 : ax0, axb = tb >= n0 ? ax1 : ax0;
+                const float axa = sa ? ax1 : ax0, axb = sb ? ax1 : ax0;
                 const float dxa = axa - pa.x, dya = ay - pa.y, dza = az - pa.z;
                 const float dxb = axb - pb.x, dyb = ay - pb.y, dzb = az - pb.z;
                 const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
                 const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
-                push(oka && r2a < 1.0f, (unsigned)ja | (ta >= n0 ? w1 : w0));
-                push(okb && r2b < 1.0f, (unsigned)jb | (tb >= n0 ? w1 : w0));
+                const int aa = sa ? aid1 : aid0, ba = (sa ? b1 : b0) + ja;
+                const int ab = sb ? aid1 : aid0, bbb = (sb ? b1 : b0) + jb;
+                const bool ma = sa ? m1 : m0, mb = sb ? m1 : m0;
+                hpush(oka && r2a < 1.0f, (unsigned)ja | (sa ? w1 : w0), ma ? aa : ba, ma ? ba : aa);
+                hpush(okb && r2b < 1.0f, (unsigned)jb | (sb ? w1 : w0), mb ? ab : bbb, mb ? bbb : ab);
             }
         }
-        drain(PassTag<false>{});
+        while (wave_any(ht != hh)) hstep();
+        __builtin_amdgcn_wave_barrier();
     }
 
-    if (active) {
+    if (active && !listed) {
         if (A.ff_stride == 4) {
             UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
             f4v o = {0.f, 0.f, 0.f, 0.f};
@@ -501,10 +631,10 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     }
 }
 
-template <int WR, bool EXACT>
+template <int WR, bool EXACT, int PHASE>
 // at most 128 VGPRs: four waves per SIMD
 __global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_dpd(DpdK A) {
-    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP, false, PHASE>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
 }
 
 // with the per-particle stress (the reference's FoSArray path,
@@ -518,7 +648,7 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
 // bblock[k] <= b < bblock[k + 1]; each box's blocks start at its first
 // particle, so no block spans two boxes.
 // (four waves per SIMD unless the queues' LDS allows only three blocks per CU)
-template <int WR, bool EXACT, int NT, int QC>
+template <int WR, bool EXACT, int NT, int QC, int PHASE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 4096 ? 3 : 4))) void k_dpd_batch(
     const DpdK *boxes, const int *bblock, int nbox) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -529,7 +659,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 40
         else hi = mid;
     }
     const DpdK &A = boxes[lo];
-    dpd_block<WR, EXACT, NT, QC, SCAP_BATCH>(A, A.i0 + (blk - bblock[lo]) * NT);
+    dpd_block<WR, EXACT, NT, QC, SCAP_BATCH, false, PHASE>(A, A.i0 + (blk - bblock[lo]) * NT);
 }
 
 // ---------------------------------------------------------------- half shell
@@ -886,9 +1016,11 @@ __global__ __launch_bounds__(256) void k_acc_to_ff(int nmax, const int *n_dev, i
 // fastest, src/exch/flu/dev/map.h:22-42), one extra 0 after each fragment.
 // An exclusive scan over this array gives every fragment's packed-list
 // offsets (src/exch/flu/dev/map.h:44-98) as differences G[off + h] - G[off].
+// The box's edge list counter (if any) starts the step at zero here.
 __global__ __launch_bounds__(256) void k_frag_counts(const int *starts, int Lx, int Ly, int Lz, FragOff fo,
-                                                     int *cnt) {
+                                                     int *cnt, int *ecount) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0 && ecount) *ecount = 0;
     if (g >= fo.off[26]) return;
     int f = 0;
     while (f < 25 && g >= fo.off[f + 1]) f++;
@@ -929,12 +1061,41 @@ DpdK make_k(const DpdLaunch &a) {
     k.pairs = a.pairs;
     k.ss = a.ss;
     k.acc = a.acc;
+    k.ecount = a.ecount;
+    k.eidx = a.eidx;
+    k.ef = a.ef;
     return k;
 }
 
+// Expected particles in the boundary cells of a periodic box (the ones with
+// pairs across the boundary), for nmax particles spread uniformly.
+double edge_estimate(const Box &b, int nmax) {
+    double inner = 1.0;
+    for (int c = 0; c < 3; c++) inner *= b.L[c] > 2 ? (double)(b.L[c] - 2) / b.L[c] : 0.0;
+    return (double)nmax * (1.0 - inner);
+}
+
+// The halo pass as its own launch over the listed particles keeps all its
+// lanes busy, but its chains of dependent loads are exposed unless there are
+// many of them: measured on MI355X, a 96^3 box (about 5e5 boundary particles)
+// runs faster with the halo pass inside the force kernel, where other waves'
+// bulk work hides them, and 64 boxes of 8..48 cells (1.4e6) faster split.
+double split_edges = 1048576.0;   // udx_dpd_set_split_edges
+
+// with an edge list (periodic boxes with many boundary particles) the bulk
+// pass and the halo pass of the listed particles are two launches of the same
+// grid; results are identical either way
 template <int WR, bool EXACT>
 int launch_dpd(hipStream_t s, const DpdK &k, int nmax) {
-    hipLaunchKernelGGL((k_dpd<WR, EXACT>), dim3(ceil_div(nmax, DPD_THREADS)), dim3(DPD_THREADS), 0, s, k);
+    const dim3 g(ceil_div(nmax, DPD_THREADS)), b(DPD_THREADS);
+    const Box box{{k.Lx, k.Ly, k.Lz}};
+    if (k.eidx && k.periodic && edge_estimate(box, nmax) >= split_edges) {
+        hipLaunchKernelGGL((k_dpd<WR, EXACT, PH_BULK>), g, b, 0, s, k);
+        UDX_LAUNCHED();
+        hipLaunchKernelGGL((k_dpd<WR, EXACT, PH_HALO>), g, b, 0, s, k);
+    } else {
+        hipLaunchKernelGGL((k_dpd<WR, EXACT, PH_ALL>), g, b, 0, s, k);
+    }
     UDX_LAUNCHED();
     return UDX_OK;
 }
@@ -964,11 +1125,11 @@ void frag_free(FragTables *t) {
     dev_free(t->scan_tmp);
 }
 
-int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *starts) {
+int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *starts, int *ecount) {
     FragOff fo;
     for (int f = 0; f < 27; f++) fo.off[f] = t->off[f];
     hipLaunchKernelGGL(k_frag_counts, dim3(ceil_div(t->size, 256)), dim3(256), 0, s, starts, b.L[0], b.L[1],
-                       b.L[2], fo, t->cnt);
+                       b.L[2], fo, t->cnt, ecount);
     UDX_LAUNCHED();
     return scan_exclusive(s, t->cnt, t->size, t->fss, t->scan_tmp);
 }
@@ -1027,6 +1188,12 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
 
 int dpd_forces(hipStream_t s, const DpdLaunch &a) { return dpd_forces_mode(s, a, 0); }
 
+int dpd_set_split_edges(double n) {
+    UDX_CHECK(n >= 0.0, UDX_EINVAL, "udx_dpd_set_split_edges: %g", n);
+    split_edges = n;
+    return UDX_OK;
+}
+
 // ---------------------------------------------------------------- batches
 namespace {
 constexpr int VAR_SHAPE[DPD_NVARIANT][2] = {{128, 16}, {128, 8}, {128, 32}, {256, 16}, {256, 8}, {256, 32}};
@@ -1036,8 +1203,17 @@ template <int WR, bool EXACT, int NT, int QC>
 int launch_batch(hipStream_t s, const DpdBatch &b) {
     const int w = NT == 128 ? 0 : 1;
     if (b.nblocks[w] == 0) return UDX_OK;
-    hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC>), dim3(b.nblocks[w]), dim3(NT), 0, s,
-                       (const DpdK *)b.table, b.bblock[w], b.nbox);
+    const DpdK *tab = (const DpdK *)b.table;
+    if (b.split) {
+        hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC, PH_BULK>), dim3(b.nblocks[w]), dim3(NT), 0, s, tab,
+                           b.bblock[w], b.nbox);
+        UDX_LAUNCHED();
+        hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC, PH_HALO>), dim3(b.nblocks[w]), dim3(NT), 0, s, tab,
+                           b.bblock[w], b.nbox);
+    } else {
+        hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC, PH_ALL>), dim3(b.nblocks[w]), dim3(NT), 0, s, tab,
+                           b.bblock[w], b.nbox);
+    }
     UDX_LAUNCHED();
     return UDX_OK;
 }
@@ -1086,6 +1262,13 @@ int dpd_batch_build(DpdBatch *b, const DpdLaunch *boxes, int nbox) {
                   UDX_EINVAL, "dpd: the boxes of a batch share one float4 force array and one FP64 sum array");
     b->ntot = boxes[nbox - 1].i0 + boxes[nbox - 1].nmax;
     b->n_end = boxes[nbox - 1].starts + boxes[nbox - 1].box.ncells();
+    double edges = 0.0;
+    b->split = 1;
+    for (int k = 0; k < nbox; k++) {
+        if (!boxes[k].eidx || !boxes[k].periodic) b->split = 0;
+        edges += edge_estimate(boxes[k].box, boxes[k].nmax);
+    }
+    if (edges < split_edges) b->split = 0;
     return UDX_OK;
 }
 

@@ -65,12 +65,14 @@ __global__ __launch_bounds__(256) void k_ens_count(int n, int nbox, const int *p
 
 // cell counts of the 26 fragments of every box (as k_frag_counts); the
 // first threads also copy the step's seeds of every box from the run's table
+// and clear the boxes' edge list counters
 __global__ __launch_bounds__(256) void k_ens_frag_counts(int total, int nbox, const int *fbase, const int *foff,
                                                          const int *coff, const int4 *boxL, const int *starts,
                                                          int *cnt, const float *table, const int *step,
-                                                         float *seeds) {
+                                                         float *seeds, int *ecount) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < 27 * nbox) seeds[g] = table[(long)*step * 27 * nbox + g];
+    if (g < nbox) ecount[g] = 0;
     if (g >= total) return;
     const int b = find_box(fbase, nbox, g);
     const int4 L = boxL[b];
@@ -167,6 +169,8 @@ struct udx_ens {
     MoveBox *mb;
     float4 *pos[2], *vel[2], *ff;
     double *acc;                   // FP64 force sums of the fast pass (kept zero between steps)
+    int *ecount, *eidx;            // edge lists of the gather's halo pass (a counter per box)
+    float4 *ef;
     ClistBufs cl;
     int *fcnt, *fss, *fscan;
     float *seeds_table, *seeds_cur, *seeds_host;
@@ -201,6 +205,9 @@ static int ens_build_batches(udx_ens *e) {
             a.pairs = e->pairs;
             a.i0 = e->poff_h[b];
             a.acc = e->acc;
+            a.ecount = e->ecount + b;
+            a.eidx = e->eidx;
+            a.ef = e->ef;
         }
         UDX_TRY(dpd_batch_build(&e->batch[par], d.data(), e->nbox));
     }
@@ -227,7 +234,8 @@ static int ens_step(udx_ens *e, int cur) {
     UDX_TRY(gather_f4(s, &e->cl, n, e->pos[a], e->vel[a], e->pos[b], e->vel[b]));
     const int nf = e->ftot > 27 * e->nbox ? e->ftot : 27 * e->nbox;
     hipLaunchKernelGGL(k_ens_frag_counts, dim3(ceil_div(nf, 256)), dim3(256), 0, s, e->ftot, e->nbox, e->fbase,
-                       e->foff, e->coff, e->boxL, e->cl.starts, e->fcnt, e->seeds_table, e->step, e->seeds_cur);
+                       e->foff, e->coff, e->boxL, e->cl.starts, e->fcnt, e->seeds_table, e->step, e->seeds_cur,
+                       e->ecount);
     UDX_LAUNCHED();
     UDX_TRY(scan_exclusive(s, e->fcnt, e->ftot, e->fss, e->fscan));
     UDX_TRY(dpd_forces_batch(s, e->batch[b], e->mode, e->variant));
@@ -309,6 +317,9 @@ int udx_ens_fin(udx_ens *e) {
     }
     dev_free(e->ff);
     dev_free(e->acc);
+    dev_free(e->ecount);
+    dev_free(e->eidx);
+    dev_free(e->ef);
     dev_free(e->poff);
     dev_free(e->coff);
     dev_free(e->fbase);
@@ -396,6 +407,9 @@ int udx_ens_ini(udx_ctx *ctx, int nbox, const udx_ens_box *boxes, udx_ens **out)
     if (!st) st = dev_alloc(&e->acc, 3 * (size_t)(e->ntot > 0 ? e->ntot : 1));
     if (!st && hipMemset(e->acc, 0, sizeof(double) * 3 * (size_t)(e->ntot > 0 ? e->ntot : 1)) != hipSuccess)
         st = UDX_EHIP;
+    if (!st) st = dev_alloc(&e->ecount, (size_t)nbox);
+    if (!st) st = dev_alloc(&e->eidx, (size_t)(e->ntot > 0 ? e->ntot : 1));
+    if (!st) st = dev_alloc(&e->ef, (size_t)(e->ntot > 0 ? e->ntot : 1));
     if (!st) st = clist_alloc(&e->cl, Box{{e->ctot, 1, 1}}, e->ntot > 0 ? e->ntot : 1, SOLVENT_SUB);
     if (!st) st = dev_alloc(&e->fcnt, (size_t)e->ftot);
     if (!st) st = dev_alloc(&e->fss, (size_t)e->ftot + 1);

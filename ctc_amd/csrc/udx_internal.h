@@ -121,7 +121,8 @@ struct FragOff {
 };
 int frag_alloc(FragTables *t, const Box &b);
 void frag_free(FragTables *t);
-int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *starts);
+// ecount (optional): a box's edge list counter, cleared here for the step
+int frag_build(hipStream_t s, const FragTables *t, const Box &b, const int *starts, int *ecount = nullptr);
 
 struct DpdLaunch {
     const float4 *pos, *vel;
@@ -141,12 +142,20 @@ struct DpdLaunch {
     int i0 = 0;             // first particle of the box in pos/vel (noise indices are relative)
     float *ss = nullptr;    // per-particle stress, 6 floats added (null: forces only)
     // FP64 accumulators, 3 per particle, zero on entry and left zero: with one,
-    // UDX_MODE_FAST evaluates each pair once (half shell) and sums exactly
+    // UDX_MODE_HALF evaluates each pair once (half shell) and sums exactly
     double *acc = nullptr;
+    // edge list of a periodic box (entries from i0, nmax capacity; *ecount
+    // cleared by frag_build for the step): with one the gather's halo pass
+    // runs as its own launch over the listed particles only
+    int *ecount = nullptr, *eidx = nullptr;
+    float4 *ef = nullptr;
 };
 int dpd_forces(hipStream_t s, const DpdLaunch &a);
 // mode: UDX_MODE_FAST or UDX_MODE_EXACT
 int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode);
+// the gather's halo pass runs as its own launch when the expected boundary
+// particles of a launch (all boxes of a batch) reach n
+int dpd_set_split_edges(double n);
 
 // many boxes in one launch (one descriptor per box, all sharing spow)
 struct DpdBatch {
@@ -161,6 +170,7 @@ struct DpdBatch {
     float *ff = nullptr;
     int ntot = 0;
     const int *n_end = nullptr;   // device: end of the last box's cell list
+    int split = 0;                // every box has an edge list: bulk and halo passes launched apart
 };
 // tile variants of the batched force kernel: threads per block and queued
 // pairs per lane; variant 0 is the default

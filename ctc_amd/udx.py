@@ -203,6 +203,7 @@ _SIGS = {
     "udx_ens_run": (_I, [_VP, _I, C.POINTER(_F)]),
     "udx_ens_set_graph": (_I, [_VP, _I]),
     "udx_ens_autotune": (_I, [_VP, _I, C.POINTER(_I), C.POINTER(_F)]),
+    "udx_dpd_set_split_edges": (_I, [C.c_double]),
     "udx_ens_set_variant": (_I, [_VP, _I]),
     "udx_ens_pairs": (_I, [_VP, C.POINTER(C.c_ulonglong), _I]),
 }
@@ -1072,3 +1073,9 @@ def ens_trunk_seeds(nbox, nsteps):
             for f in range(27):
                 out[k, b, f] = tr[b][f].get()
     return out
+
+
+def set_split_edges(n):
+    """udx_dpd_set_split_edges: the expected boundary particles from which the
+    gather's halo pass is a launch of its own (same forces either way)."""
+    _ok(lib().udx_dpd_set_split_edges(float(n)))

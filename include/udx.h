@@ -134,6 +134,13 @@ int udx_fluforces_apply_stress(udx_fluforces *f, const udx_pair_dpd *p, const in
 #define UDX_MODE_EXACT 1
 #define UDX_MODE_HALF 2
 int udx_fluforces_set_mode(udx_fluforces *f, int mode);
+/* The gather's pairs across the periodic boundary (the halo) are evaluated in
+ * the force kernel, or, when a launch's boxes hold at least n particles in
+ * boundary cells (estimated from the box sizes; default 2^20), by a second
+ * launch over those particles only, which keeps its lanes busy when there are
+ * many.  The forces are the same bits either way.  Process-wide; no
+ * counterpart in the reference (its halo is a separate set of kernels). */
+int udx_dpd_set_split_edges(double n);
 /* unordered pair interactions (r^2 < 1) counted by apply calls since the last reset */
 int udx_fluforces_pairs(udx_fluforces *f, unsigned long long *npairs, int reset);
 

@@ -183,6 +183,30 @@ def test_forces_fast_mode_tolerance(ctx, case):
 
 
 @pytest.mark.parametrize("case", CASES)
+def test_forces_split_halo_pass_same_bits(ctx, case):
+    """The halo pass as a launch of its own over the listed boundary
+    particles (forced here for small boxes) gives the bits of the pass inside
+    the force kernel, in both modes, and the oracle's."""
+    L, rho, a, g, kBT, dt, spow = case
+    pp = random_box(L, rho, 9)
+    starts, counts, srt, _ = O.clist_build(L, pp)
+    seeds = O.seeds_table(2)[1]
+    par = (a, g, O.dpd_sigma(kBT, g, dt), spow)
+    f_ref, np_ref = O.dpd_forces(L, srt, starts, counts, par, seeds)
+    got = {}
+    try:
+        for split in (0.0, 1e30):
+            U.set_split_edges(split)
+            for mode in (U.MODE_FAST, U.MODE_EXACT):
+                got[split, mode] = gpu_forces(ctx, L, srt, starts, U.PairDpd(*par), seeds, mode=mode)
+    finally:
+        U.set_split_edges(1048576.0)
+    for (split, mode), (f, npair) in got.items():
+        assert npair == np_ref // 2, (split, mode)
+        assert np.array_equal(f.view(np.uint32), f_ref.view(np.uint32)), (split, mode)
+
+
+@pytest.mark.parametrize("case", CASES)
 def test_forces_half_mode_exact_sums(ctx, case):
     """Half shell with exact sums: the oracle's exact-sum forces bit for bit,
     identical from run to run, the contract metric against the double sum."""

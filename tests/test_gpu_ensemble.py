@@ -100,13 +100,21 @@ def run_ensemble(ctx, boxes, pps, nsteps, seeds, exact, graph=False, variant=Non
     return host(out), pairs
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("exact", [True, False, U.MODE_HALF])
-def test_ensemble_equals_single_boxes(ctx, exact):
+def test_ensemble_equals_single_boxes(ctx, exact, split):
+    """split: the batched halo pass as a launch of its own (the single boxes
+    keep it inside their force kernel)."""
     nbox, nsteps = 7, 12
     boxes, pps = make_boxes(nbox, 3)
     seeds = U.ens_trunk_seeds(nbox, nsteps)
-    say("ensemble vs single boxes, exact", exact, [b["L"] for b in boxes])
-    got, pairs = run_ensemble(ctx, boxes, pps, nsteps, None, exact)    # the default trunks = seeds
+    say("ensemble vs single boxes, exact", exact, "split", split, [b["L"] for b in boxes])
+    try:
+        if split:
+            U.set_split_edges(0.0)
+        got, pairs = run_ensemble(ctx, boxes, pps, nsteps, None, exact)    # the default trunks = seeds
+    finally:
+        U.set_split_edges(1048576.0)
     off = np.cumsum([0] + [b["n"] for b in boxes])
     total = 0
     for b in range(nbox):
