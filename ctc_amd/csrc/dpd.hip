@@ -631,9 +631,10 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     }
 }
 
+// at most 102 VGPRs: five waves per SIMD (measured 10% faster on the 96^3 box
+// than four waves at 104 VGPRs; six spill)
 template <int WR, bool EXACT, int PHASE>
-// at most 128 VGPRs: four waves per SIMD
-__global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_dpd(DpdK A) {
+__global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_dpd(DpdK A) {
     dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP, false, PHASE>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
 }
 
@@ -647,9 +648,9 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
 // Many boxes in one launch: block b belongs to the box k with
 // bblock[k] <= b < bblock[k + 1]; each box's blocks start at its first
 // particle, so no block spans two boxes.
-// (four waves per SIMD unless the queues' LDS allows only three blocks per CU)
+// (five waves per SIMD unless the queues' LDS allows only three blocks per CU)
 template <int WR, bool EXACT, int NT, int QC, int PHASE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 4096 ? 3 : 4))) void k_dpd_batch(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 4096 ? 3 : 5))) void k_dpd_batch(
     const DpdK *boxes, const int *bblock, int nbox) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     int lo = 0, hi = nbox;
