@@ -139,6 +139,7 @@ struct udx_flu {
     struct udx_wall *wall;
     float t;            // step clock, float like the reference's (src/scheme/time_line/imp/type.h:1-5)
     int timing;
+    int fused;          // the last step moved the particles in the gather (walls before it)
     // timing: five events per step (clist, DPD, walls and objects, move,
     // end), recorded on the library's stream and read after the run, so the
     // timed steps run without host synchronisation
@@ -644,8 +645,6 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
     d.ecount = f->ecount;
     d.eidx = f->eidx;
     d.ef = f->ef;
-    UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
-    if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
     const SdfField *sdf = nullptr;
     WallVel wv{};
     if (f->wall) {
@@ -653,10 +652,45 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
         f->wall->t = f->t;
         wv = wall_vel_view(f->wall->wvel, f->t);
         sdf = wall_sdf(f->wall);
+    }
+    // Without objects the gather also moves the particles (into the other
+    // pair of streams, which the next step's reorder reads): the wall forces
+    // go first, into ff of the particles near a wall, and the gather adds
+    // them to its sums, so the forces and the move are those of the passes
+    // one after the other.
+    f->fused = !m.r && dpd_move_fused(d, f->conf.mode);
+    if (f->fused) {
+        if (f->wall)
+            UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
+                                f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1,
+                                f->conf.mode == UDX_MODE_EXACT, wv, 1, f->cl.starts));
+        if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
+        FusedMove mv;
+        mv.opos = f->pos[a];
+        mv.ovel = f->vel[a];
+        mv.dt = f->conf.dt;
+        mv.mass = f->conf.mass;
+        for (int c = 0; c < 3; c++) mv.accel[c] = f->conf.accel[c];
+        mv.walls = sdf != nullptr;
+        mv.sdf = sdf ? sdf->view : SdfView{};
+        mv.wv = wv;
+        mv.next = &f->cl;
+        d.move = &mv;
+        UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
+        f->counted = 1;
+        if (f->evk) {
+            UDX_HIP(hipEventRecord(f->evk[3], s));
+            UDX_HIP(hipEventRecord(f->evk[4], s));
+        }
+        f->t += f->conf.dt;   // time_line_advance (src/scheme/time_line/imp/main.h:10-21)
+        return UDX_OK;
+    }
+    UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));
+    if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
+    if (f->wall)
         UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
                             f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1,
-                            f->conf.mode == UDX_MODE_EXACT, wv));
-    }
+                            f->conf.mode == UDX_MODE_EXACT, wv, 0, f->cl.starts));
     if (m.r && nvtx > 0) {
         UDX_HIP(hipMemsetAsync(m.fo, 0, 3 * sizeof(float) * (size_t)nvtx, s));
         if (f->wall) {
@@ -818,11 +852,13 @@ int udx_flu_run(udx_flu *f, int nsteps, const float *seeds_host) {
     if (f->timing) {
         float acc[4] = {0, 0, 0, 0};
         UDX_HIP(hipEventSynchronize(f->ev[5 * (size_t)(nsteps - 1) + 4]));
+        // a fused step records walls before the gather (with the move)
+        const int sw = f->fused ? 1 : 0;
         for (int k = 0; k < nsteps; k++)
             for (int c = 0; c < 4; c++) {
                 float t;
                 UDX_HIP(hipEventElapsedTime(&t, f->ev[5 * (size_t)k + c], f->ev[5 * (size_t)k + c + 1]));
-                acc[c] += t;
+                acc[sw && (c == 1 || c == 2) ? 3 - c : c] += t;
             }
         for (int c = 0; c < 4; c++) f->tms[c] = acc[c] / nsteps;
         f->tms[4] = f->tms[0] + f->tms[1] + f->tms[2] + f->tms[3];
@@ -851,6 +887,7 @@ int udx_flu_wall_pairs(udx_flu *f, unsigned long long *npairs) {
 int udx_sdf_fin(udx_sdf *q) {
     if (!q) return UDX_OK;
     dev_free(q->f.tex);
+    dev_free(q->f.segs);
     delete q;
     return UDX_OK;
 }

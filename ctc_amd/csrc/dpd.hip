@@ -30,11 +30,15 @@ namespace {
 
 // 128 threads: measured 5% faster per step than 256 on the 96^3, density-10 box
 constexpr int DPD_THREADS = 128;
-constexpr int QCAP = 16;   // queued pairs per lane (a power of two)
+constexpr int QCAP = 32;   // queued pairs per lane (a power of two)
 // staged candidates per wave and row (320 for batches of small boxes measured
 // 4% slower than 160: more staging loads, fewer blocks per CU)
 constexpr int SCAP = 160, SCAP_BATCH = 160;
 constexpr int DRAIN_STEPS = 2;  // pair steps per lane when a queue is full
+// bulk queue entries: generation (GBITS) and offset (OBITS) in 16 bits
+constexpr int OBITS = 11, GBITS = 5, GTAB = 1 << GBITS;
+constexpr unsigned OMASK = (1u << OBITS) - 1u;
+static_assert(OBITS + GBITS == 16, "16-bit entries");
 
 struct DpdK {
     const float4 *pos, *vel;
@@ -56,6 +60,12 @@ struct DpdK {
     // box entries from i0, *ecount of them
     int *ecount, *eidx;
     float4 *ef;
+    // the move fused into the pass (k_dpd with MOVE set; FusedMove)
+    float4 *opos, *ovel;
+    float mdt, mmass, max_, may, maz;
+    SdfView sdf;
+    WallVel wv;
+    SlabCount sc;
 };
 
 // What the force pass reads of a box, read once at kernel entry.  The
@@ -153,15 +163,19 @@ __device__ __forceinline__ void pair_kin(PairIn &o, float ax, float ay, float az
 
 // a pair inside the box: noise mean0var1ii(seed, max, min)
 // (src/fluforces/bulk/dev/main.h:32-38)
-__device__ __forceinline__ PairIn pair_in_bulk(const DpdP &P, unsigned ent, int i, const float4 &pi,
-                                               const float4 &vi, float seed_bulk) {
-    const int j = (int)(ent & 0x7FFFFFFu);
+__device__ __forceinline__ PairIn pair_in_bulk_j(const DpdP &P, int j, int i, const float4 &pi, const float4 &vi,
+                                                 float seed_bulk) {
     const float4 pj = P.p(j), vj = P.v(j);
     const int u = (i > j ? i : j) - P.i0, v = (i > j ? j : i) - P.i0;
     PairIn o;
     pair_kin(o, pi.x, pi.y, pi.z, vi, pj, vj);
     o.x = seed_bulk - low_discrepancy((float)(u & 0x3FF), (float)u, (float)(v & 0x3FF), (float)v);
     return o;
+}
+
+__device__ __forceinline__ PairIn pair_in_bulk(const DpdP &P, unsigned ent, int i, const float4 &pi,
+                                               const float4 &vi, float seed_bulk) {
+    return pair_in_bulk_j(P, (int)(ent & 0x7FFFFFFu), i, pi, vi, seed_bulk);
 }
 
 // a pair across the periodic boundary: the local particle is moved into the
@@ -286,11 +300,18 @@ enum { PH_ALL = 0, PH_BULK = 1, PH_HALO = 2 };
 // candidates per iteration.  Halo pass (periodic boxes, only waves with a
 // particle in a boundary cell): the wrapped parts of the 9 rows, read from
 // global memory.  Each lane sums its pairs in candidate order, bulk first.
-template <int WR, bool EXACT, int NT, int QC, int SC, bool STRESS = false, int PHASE = PH_ALL>
+//
+// MOVE (one launch of a periodic box, float4 forces): 1 the particles are
+// also moved (FusedMove), 2 the same with walls.
+template <int WR, bool EXACT, int NT, int QC, int SC, bool STRESS = false, int PHASE = PH_ALL, int MOVE = 0>
 __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     static_assert(QC >= 8 && (QC & (QC - 1)) == 0, "the queue length is a power of two");
-    __shared__ unsigned q[QC * NT];
+    // 16-bit bulk entries, one region of QC * 128 bytes per wave, which the
+    // wave's halo rings reuse: the block's two waves run their passes
+    // independently, so neither may touch the other's region
+    __shared__ unsigned q[QC * NT / 2];
     __shared__ float4 sbuf[NT / 64][SC + 4];
+    __shared__ int gtab[NT / 64][GTAB];    // candidate bases of the bulk entries' generations
     __shared__ unsigned long long lpairs[NT / 64];
     const int tid = threadIdx.x;
     const DpdP P = dpd_view(A);
@@ -341,16 +362,25 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     // wave takes DRAIN_STEPS steps only: lanes fill their queues at very
     // different rates (a lane near the -z face of its cell hits most in the
     // dz = -1 rows), and emptying every queue at that point would run the
-    // expensive pair body with most lanes idle.  An idle slot evaluates a
-    // discarded pair with an in-bounds particle (i itself, or the box's first
-    // particle for a lane past the end).
-    // head and tail count in units of STEP bytes (one slot of every lane), so
-    // the LDS address of an entry is (count & QMASK) | (lane address)
-    constexpr unsigned STEP = 4u * NT, QMASK = (unsigned)(QC - 1) * STEP;
+    // expensive pair body with most lanes idle; deeper queues leave fewer
+    // idle slots, so an entry is 16 bits: a generation g (GBITS) and an
+    // offset (OBITS), the candidate being gtab[g mod GTAB] + offset.  Each
+    // row stages one generation per 2^OBITS candidates of the wave's union,
+    // and a generation's table slot is reused only after a full drain.  An
+    // idle slot evaluates a discarded pair with an in-bounds particle (i
+    // itself, or the box's first particle for a lane past the end).
+    // head and tail count in units of STEP bytes (one slot of every lane of
+    // the wave), so the LDS address of an entry in the wave's region is
+    // (count & QMASK) | (lane address)
+    constexpr unsigned STEP = 2u * 64u, QMASK = (unsigned)(QC - 1) * STEP;
     unsigned head = 0, tail = 0;
-    const unsigned qlane = (unsigned)(tid * 4);
-    auto slot = [&](unsigned k) { return (unsigned *)((char *)q + ((k & QMASK) | qlane)); };
-    const unsigned idle = (unsigned)(active ? i : P.i0) | (W_NONE << 27);
+    char *const qw = (char *)q + (tid >> 6) * (QC * 128);
+    const unsigned qlane = (unsigned)((tid & 63) * 2);
+    auto slot = [&](unsigned k) { return (unsigned short *)(qw + ((k & QMASK) | qlane)); };
+    int *const gt = gtab[tid >> 6];
+    int gen = 0, gfull = 0;   // next generation; generations below gfull are drained
+    const int idle_j = active ? i : P.i0;
+    const unsigned idle = (unsigned)idle_j | (W_NONE << 27);
     auto add = [&](bool v, const float3 &f, const PairIn &pq) {
         if (v) {
             fx += f.x;
@@ -366,20 +396,14 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             }
         }
     };
-    auto pair_step = [&](auto pass) {
-        constexpr bool BULK = decltype(pass)::value;
+    auto decode = [&](unsigned e) { return gt[e >> OBITS] + (int)(e & OMASK); };
+    auto pair_step = [&](auto) {
         const unsigned fl = tail - head;
         const bool va = fl > 0u, vb = fl > STEP;
-        const unsigned ea = va ? *slot(head) : idle;
-        const unsigned eb = vb ? *slot(head + STEP) : idle;
-        PairIn a, b;
-        if (BULK) {
-            a = pair_in_bulk(P, ea, i, pi, vi, seed_bulk);
-            b = pair_in_bulk(P, eb, i, pi, vi, seed_bulk);
-        } else {
-            a = pair_in_halo(A, P, ea, i, pi, vi, cx, cy, cz, istart, seed_bulk);
-            b = pair_in_halo(A, P, eb, i, pi, vi, cx, cy, cz, istart, seed_bulk);
-        }
+        const int ja = va ? decode(*slot(head)) : idle_j;
+        const int jb = vb ? decode(*slot(head + STEP)) : idle_j;
+        const PairIn a = pair_in_bulk_j(P, ja, i, pi, vi, seed_bulk);
+        const PairIn b = pair_in_bulk_j(P, jb, i, pi, vi, seed_bulk);
         float xa = a.x, xb = b.x;
         logistic18x2(&xa, &xb);
         const float3 f1 = dpd_pair<WR, EXACT>(P.par, a.dx, a.dy, a.dz, a.r2, a.dvx, a.dvy, a.dvz, xa * R_SQRT2);
@@ -398,12 +422,26 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     };
     auto drain = [&](auto pass) {
         while (wave_any(tail != head)) pair_step(pass);
+        gfull = gen;
     };
     // the slot at tail is free whenever there is room (make_room first)
     auto push = [&](bool hit, unsigned e) {
-        unsigned *const w = slot(tail);
-        if (hit) *w = e;
+        unsigned short *const w = slot(tail);
+        if (hit) *w = (unsigned short)e;
         tail += hit ? STEP : 0u;
+    };
+    // generations for the candidates [lo, lo + nch 2^OBITS) of a row: the
+    // first one returned; table slots are free once every entry older than
+    // GTAB generations is drained
+    auto new_gens = [&](int lo, int nch) {
+        if (gen + nch - gfull > GTAB) drain(PassTag<true>{});
+        __builtin_amdgcn_wave_barrier();
+        const int k = tid & 63;
+        if (k < nch) gt[(gen + k) & (GTAB - 1)] = lo + (k << OBITS);
+        __builtin_amdgcn_wave_barrier();
+        const int g0 = gen;
+        gen += nch;
+        return g0;
     };
 
     // ---- bulk pass: rows 0-8; only the particle's own row (4) can hold i
@@ -428,6 +466,8 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         // candidate k of the lane's range is i itself (own row only)
         const int kself = SELF ? i - j0 : -1;
         if (hi - lo <= SC) {
+            // one generation: entry = g << OBITS | (j - lo)
+            const unsigned e0 = ((unsigned)(new_gens(lo, 1) & (GTAB - 1)) << OBITS) + (unsigned)(n0 > 0 ? j0 - lo : 0);
             __builtin_amdgcn_wave_barrier();
             for (int k = (tid & 63); k < hi - lo; k += 64) sb[k] = P.p(lo + k);
             __builtin_amdgcn_wave_barrier();
@@ -439,7 +479,7 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
     {                                                                                              \
         const float ddx = pi.x - C.x, ddy = pi.y - C.y, ddz = pi.z - C.z;                          \
         const float r2 = __builtin_fmaf(ddz, ddz, __builtin_fmaf(ddx, ddx, ddy * ddy));            \
-        push(t + U < n0 && (!SELF || t + U != kself) && r2 < 1.0f, (unsigned)(j0 + t + U));       \
+        push(t + U < n0 && (!SELF || t + U != kself) && r2 < 1.0f, e0 + (unsigned)(t + U));        \
     }
                 UDX_TEST(c0, 0)
                 UDX_TEST(c1, 1)
@@ -449,17 +489,30 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             }
             __builtin_amdgcn_wave_barrier();
         } else {
-            for (int t = 0; wave_any(t < n0); t += 2) {
-                make_room(PassTag<true>{}, 2u);
-                const bool oka = t < n0, okb = t + 1 < n0;
-                const int ja = j0 + t, jb = j0 + t + 1;
-                const float4 pa = P.p(oka ? ja : P.i0), pb = P.p(okb ? jb : P.i0);
-                const float dxa = pi.x - pa.x, dya = pi.y - pa.y, dza = pi.z - pa.z;
-                const float dxb = pi.x - pb.x, dyb = pi.y - pb.y, dzb = pi.z - pb.z;
-                const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
-                const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
-                push(oka && (!SELF || t != kself) && r2a < 1.0f, (unsigned)ja);
-                push(okb && (!SELF || t + 1 != kself) && r2b < 1.0f, (unsigned)jb);
+            // the union read from global memory in pieces of GTAB generations
+            // (one per 2^OBITS candidates; a piece takes only its candidates)
+            const int nch = ((hi - 1 - lo) >> OBITS) + 1;
+            for (int c0 = 0; c0 < nch; c0 += GTAB) {
+                const int pl = lo + (c0 << OBITS);
+                const int np = nch - c0 < GTAB ? nch - c0 : GTAB;
+                const int ph = pl + (np << OBITS);
+                const unsigned g0 = (unsigned)new_gens(pl, np);
+                auto ent = [&](int j) {
+                    const int o = j - pl;
+                    return (((g0 + (unsigned)(o >> OBITS)) & (GTAB - 1)) << OBITS) | (unsigned)(o & OMASK);
+                };
+                for (int t = 0; wave_any(t < n0); t += 2) {
+                    make_room(PassTag<true>{}, 2u);
+                    const int ja = j0 + t, jb = j0 + t + 1;
+                    const bool oka = t < n0 && ja >= pl && ja < ph, okb = t + 1 < n0 && jb >= pl && jb < ph;
+                    const float4 pa = P.p(oka ? ja : P.i0), pb = P.p(okb ? jb : P.i0);
+                    const float dxa = pi.x - pa.x, dya = pi.y - pa.y, dza = pi.z - pa.z;
+                    const float dxb = pi.x - pb.x, dyb = pi.y - pb.y, dzb = pi.z - pb.z;
+                    const float r2a = __builtin_fmaf(dza, dza, __builtin_fmaf(dxa, dxa, dya * dya));
+                    const float r2b = __builtin_fmaf(dzb, dzb, __builtin_fmaf(dxb, dxb, dyb * dyb));
+                    push(oka && (!SELF || t != kself) && r2a < 1.0f, ent(ja));
+                    push(okb && (!SELF || t + 1 != kself) && r2b < 1.0f, ent(jb));
+                }
             }
         }
     };
@@ -496,16 +549,17 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         listed = edge;
     }
     if (PHASE != PH_BULK && wave_any(edge)) {
-        constexpr int HQC = QC / 2 < 8 ? QC / 2 : 8;
-        static_assert(2 * HQC <= QC && HQC * 64 * 4 <= (SC + 4) * 16, "halo rings fit the bulk queue and staging");
-        constexpr unsigned HSTEP = 4u * NT, HMASK = (unsigned)(HQC - 1) * HSTEP;
-        // rings j|w and u in the bulk queue (slot k of every lane of the
-        // block: HSTEP bytes), ring v in the wave's staging buffer (256 bytes
-        // per slot)
-        unsigned *const rj = q, *const ru = q + HQC * NT;
-        char *const rv = (char *)&sbuf[tid >> 6][0] + 4 * (tid & 63);
-        auto hs = [&](unsigned *base, unsigned k) { return (unsigned *)((char *)base + ((k & HMASK) | qlane)); };
-        auto hv = [&](unsigned k) { return (unsigned *)(rv + (k & HMASK) / (NT / 64)); };
+        constexpr int HQC = QC / 4 < 8 ? QC / 4 : 8;
+        static_assert(2 * HQC * 256 <= QC * 128 && HQC * 256 <= (SC + 4) * 16,
+                      "halo rings fit the wave's queue region and staging buffer");
+        constexpr unsigned HSTEP = 4u * 64u, HMASK = (unsigned)(HQC - 1) * HSTEP;
+        // rings j|w and u in the wave's queue region, ring v in the wave's
+        // staging buffer (slot k of every lane of the wave: HSTEP bytes)
+        unsigned *const rj = (unsigned *)qw, *const ru = (unsigned *)(qw + HQC * HSTEP);
+        char *const rv = (char *)&sbuf[tid >> 6][0];
+        const unsigned hlane = (unsigned)((tid & 63) * 4);
+        auto hs = [&](unsigned *base, unsigned k) { return (unsigned *)((char *)base + ((k & HMASK) | hlane)); };
+        auto hv = [&](unsigned k) { return (unsigned *)(rv + ((k & HMASK) | hlane)); };
         unsigned hh = 0, ht = 0;
         auto hstep = [&]() {
             const unsigned fl = ht - hh;
@@ -597,7 +651,43 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         __builtin_amdgcn_wave_barrier();
     }
 
-    if (active && !listed) {
+    if (MOVE) {
+        // the step's total force (wall force added for the particles near a
+        // wall: 0 + sum, then + wall, as ff = sum and ff += wall), the
+        // kick-drift, the bounce-back and the wrap of move_f4_wall, then the
+        // count into the next step's x slabs
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (active) {
+            UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
+            float gx = 0.f + fx, gy = 0.f + fy, gz = 0.f + fz;
+            if (MOVE == 2 && !sdf_far(A.sdf, pi.x, pi.y, pi.z)) {
+                const f4v w = *f4;
+                gx = gx + w.x;
+                gy = gy + w.y;
+                gz = gz + w.z;
+            }
+            *f4 = f4v{gx, gy, gz, 0.f};
+            p = pi;
+            float4 v = vi;
+            kick_drift(A.mdt, A.mmass, A.max_, gx, &p.x, &v.x);
+            kick_drift(A.mdt, A.mmass, A.may, gy, &p.y, &v.y);
+            kick_drift(A.mdt, A.mmass, A.maz, gz, &p.z, &v.z);
+            if (MOVE == 2) {
+                float r[3] = {p.x, p.y, p.z}, u[3] = {v.x, v.y, v.z};
+                if (sdf_bounce(A.sdf, A.wv, A.mdt, r, u)) {
+                    p.x = r[0]; p.y = r[1]; p.z = r[2];
+                    v.x = u[0]; v.y = u[1]; v.z = u[2];
+                }
+            }
+            p.x = wrap_coord(p.x, (float)Lx);
+            p.y = wrap_coord(p.y, (float)Ly);
+            p.z = wrap_coord(p.z, (float)Lz);
+            *((UDX_GLOBAL f4v *)A.opos + i) = f4v{p.x, p.y, p.z, 0.f};
+            *((UDX_GLOBAL f4v *)A.ovel + i) = f4v{v.x, v.y, v.z, 0.f};
+        }
+        clist_count_one(active, i, make_float3(p.x, p.y, p.z), A.sc.L[0], A.sc.L[1], A.sc.L[2], A.sc.sub,
+                        A.sc.counts, A.sc.cellslot);
+    } else if (active && !listed) {
         if (A.ff_stride == 4) {
             UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
             f4v o = {0.f, 0.f, 0.f, 0.f};
@@ -633,9 +723,10 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
 
 // at most 102 VGPRs: five waves per SIMD (measured 10% faster on the 96^3 box
 // than four waves at 104 VGPRs; six spill)
-template <int WR, bool EXACT, int PHASE>
+template <int WR, bool EXACT, int PHASE, int MOVE = 0>
 __global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_dpd(DpdK A) {
-    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP, false, PHASE>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP, false, PHASE, MOVE>(A,
+                                                                      xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
 }
 
 // with the per-particle stress (the reference's FoSArray path,
@@ -648,9 +739,9 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
 // Many boxes in one launch: block b belongs to the box k with
 // bblock[k] <= b < bblock[k + 1]; each box's blocks start at its first
 // particle, so no block spans two boxes.
-// (five waves per SIMD unless the queues' LDS allows only three blocks per CU)
+// (five waves per SIMD: the 16-bit queues of every tile variant leave room)
 template <int WR, bool EXACT, int NT, int QC, int PHASE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(QC * NT > 4096 ? 3 : 5))) void k_dpd_batch(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_dpd_batch(
     const DpdK *boxes, const int *bblock, int nbox) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     int lo = 0, hi = nbox;
@@ -1065,6 +1156,28 @@ DpdK make_k(const DpdLaunch &a) {
     k.ecount = a.ecount;
     k.eidx = a.eidx;
     k.ef = a.ef;
+    k.opos = k.ovel = nullptr;
+    k.mdt = k.mmass = k.max_ = k.may = k.maz = 0.f;
+    k.sdf = SdfView{};
+    k.wv = WallVel{};
+    k.sc = SlabCount{};
+    if (const FusedMove *m = a.move) {
+        k.opos = m->opos;
+        k.ovel = m->ovel;
+        k.mdt = m->dt;
+        k.mmass = m->mass;
+        k.max_ = m->accel[0];
+        k.may = m->accel[1];
+        k.maz = m->accel[2];
+        if (m->walls) {
+            k.sdf = m->sdf;
+            k.wv = m->wv;
+        }
+        for (int c = 0; c < 3; c++) k.sc.L[c] = m->next->box.L[c];
+        k.sc.sub = m->next->sub;
+        k.sc.counts = m->next->bcounts;
+        k.sc.cellslot = m->next->cellslot;
+    }
     return k;
 }
 
@@ -1086,11 +1199,17 @@ double split_edges = 1048576.0;   // udx_dpd_set_split_edges
 // with an edge list (periodic boxes with many boundary particles) the bulk
 // pass and the halo pass of the listed particles are two launches of the same
 // grid; results are identical either way
+bool split_halo(const DpdLaunch &a) { return a.eidx && a.periodic && edge_estimate(a.box, a.nmax) >= split_edges; }
+
 template <int WR, bool EXACT>
 int launch_dpd(hipStream_t s, const DpdK &k, int nmax) {
     const dim3 g(ceil_div(nmax, DPD_THREADS)), b(DPD_THREADS);
     const Box box{{k.Lx, k.Ly, k.Lz}};
-    if (k.eidx && k.periodic && edge_estimate(box, nmax) >= split_edges) {
+    if (k.opos) {
+        // the fused move (dpd_move_fused): both force modes give the same bits
+        if (k.sdf.tex) hipLaunchKernelGGL((k_dpd<WR, false, PH_ALL, 2>), g, b, 0, s, k);
+        else hipLaunchKernelGGL((k_dpd<WR, false, PH_ALL, 1>), g, b, 0, s, k);
+    } else if (k.eidx && k.periodic && edge_estimate(box, nmax) >= split_edges) {
         hipLaunchKernelGGL((k_dpd<WR, EXACT, PH_BULK>), g, b, 0, s, k);
         UDX_LAUNCHED();
         hipLaunchKernelGGL((k_dpd<WR, EXACT, PH_HALO>), g, b, 0, s, k);
@@ -1144,6 +1263,8 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
               "dpd: a periodic box needs at least 3 cells per side (got %d %d %d)", a.box.L[0], a.box.L[1],
               a.box.L[2]);
     UDX_CHECK(a.nmax < (1 << 27), UDX_ECAP, "dpd: %d particles exceed the 2^27 queue index range", a.nmax);
+    UDX_CHECK(!a.move || (dpd_move_fused(a, mode) && a.move->next), UDX_EINVAL,
+              "dpd: the fused move needs the gather in one launch with float4 forces");
     if (a.nmax <= 0) return UDX_OK;
     const DpdK k = make_k(a);
     const float sp = a.par.spow;
@@ -1188,6 +1309,11 @@ int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode) {
 }
 
 int dpd_forces(hipStream_t s, const DpdLaunch &a) { return dpd_forces_mode(s, a, 0); }
+
+bool dpd_move_fused(const DpdLaunch &a, int mode) {
+    return (mode == UDX_MODE_FAST || mode == UDX_MODE_EXACT) && a.periodic && !split_halo(a) && a.ff_stride == 4 &&
+           !a.accumulate && !a.ss;
+}
 
 int dpd_set_split_edges(double n) {
     UDX_CHECK(n >= 0.0, UDX_EINVAL, "udx_dpd_set_split_edges: %g", n);

@@ -149,7 +149,27 @@ struct DpdLaunch {
     // runs as its own launch over the listed particles only
     int *ecount = nullptr, *eidx = nullptr;
     float4 *ef = nullptr;
+    // with one, the gather also moves the particles (dpd_move_fused)
+    const struct FusedMove *move = nullptr;
 };
+// The move fused into the gather, for the solvent step without objects: after
+// its sum each particle adds its wall force (written to ff before the pass by
+// wall_forces with assign set, for the particles that are not sdf_far), stores
+// the total in ff, kick-drifts with the body force, bounces off the walls,
+// wraps, writes itself to opos/ovel at its own index and counts itself into
+// next's x slabs: what move_f4_wall does after the pass, with the same
+// operations in the same order.
+struct FusedMove {
+    float4 *opos, *ovel;
+    float dt, mass, accel[3];
+    int walls;         // sdf and wv are set
+    SdfView sdf;
+    WallVel wv;
+    ClistBufs *next;
+};
+// whether dpd_forces_mode can run a with a.move set: the gather (FAST or
+// EXACT) in one launch (no split halo) writing float4 forces
+bool dpd_move_fused(const DpdLaunch &a, int mode);
 int dpd_forces(hipStream_t s, const DpdLaunch &a);
 // mode: UDX_MODE_FAST or UDX_MODE_EXACT
 int dpd_forces_mode(hipStream_t s, const DpdLaunch &a, int mode);
@@ -200,6 +220,10 @@ struct SdfField {
     float *tex;      // T[0] T[1] T[2] texels, x fastest
     SdfView view;    // device view (tex pointer, grid, transform, threshold)
     int ready;
+    // runs [x, y) of consecutive box cells that can hold a particle that is
+    // not sdf_far (every other cell's particles are far from the walls)
+    int2 *segs = nullptr;
+    int nseg = 0;
 };
 struct WallSet {
     ClistBufs cl;    // cell list of the wall particles on the margin box
@@ -214,9 +238,14 @@ int sdf_probe(hipStream_t s, const SdfField *q, int n, const float *r3, int what
 int wall_build(hipStream_t s, WallSet *w, const SdfField *q, int *n, float *pp);
 int wall_build_frozen(hipStream_t s, WallSet *w, const SdfField *q, int nw, float *frozen);
 void wall_free(WallSet *w);
+// assign: the particles that are not sdf_far get ff = their wall force (the
+// others are left alone) instead of ff += it.  cell_starts: the particles are
+// in the cell order of the SDF's box with these starts, so only the runs of
+// cells near a wall are visited.
 int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, const int *n_dev, const float4 *pos,
                 const float4 *vel, const udx_pair_dpd &p, const float *seed_dev, float *ff, int ff_stride,
-                unsigned long long *pairs, int exact, const WallVel &wv);
+                unsigned long long *pairs, int exact, const WallVel &wv, int assign = 0,
+                const int *cell_starts = nullptr);
 int wall_bounce_aos(hipStream_t s, const SdfField *q, float dt, int n, float *pp, const WallVel &wv);
 // the wall velocity of v at time t: shear sin resolved to its rate gdot sin(w t)
 // (src/wall/wvel/imp/main.h:71-84)

@@ -188,24 +188,38 @@ __global__ __launch_bounds__(256) void k_image_scatter(int nw, const float *wall
 // inline, about one candidate in six passes and the whole wave would wait
 // for the pair body at nearly every candidate.
 constexpr int WALL_THREADS = 128, WQ = 16;
+struct WallArgs {
+    const float4 *wpos;
+    const int *wstarts;
+    int Wx, Wy, Wz;
+    DpdPar par;
+    const float *seedp;
+    float *ff;
+    int ff_stride;
+    unsigned long long *pairs;
+    int assign;
+};
+
+// particle a (valid or not: every lane of the wave calls it)
 template <int WR, bool EXACT>
-__global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, WallVel wv, int nmax, const int *n_dev,
-                                                             const float4 *pos, const float4 *vel,
-                                                             const float4 *wpos, const int *wstarts, int Wx, int Wy,
-                                                             int Wz, DpdPar par, const float *seedp, float *ff,
-                                                             int ff_stride, unsigned long long *pairs) {
+__device__ __forceinline__ void wall_one(const SdfView &v, const WallVel &wv, int a, bool valid, const float4 *pos,
+                                         const float4 *vel, const WallArgs &W) {
     __shared__ unsigned q[WQ * WALL_THREADS];
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const float4 *const wpos = W.wpos;
+    const int *const wstarts = W.wstarts;
+    const int Wx = W.Wx, Wy = W.Wy, Wz = W.Wz;
+    const DpdPar par = W.par;
+    float *const ff = W.ff;
     int npair = 0;
     bool near = false;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), va = pa;
-    if (a < nmax && a < *n_dev) {
+    if (valid) {
         pa = pos[a];
         near = !sdf_far(v, pa.x, pa.y, pa.z);
     }
     if (!wave_any(near)) return;
     if (near) va = vel[a];
-    const float seed = *seedp;
+    const float seed = *W.seedp;
     int b0[3] = {1, 1, 1};
     if (near) {
         const float pc[3] = {pa.x, pa.y, pa.z};
@@ -288,10 +302,14 @@ __global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, WallVel 
         }
     while (wave_any(fill > 0)) drain_step();
     if (near) {
-        if (ff_stride == 4) {
+        if (W.ff_stride == 4) {
             float4 *f4 = (float4 *)ff + a;
-            const float4 o = *f4;
-            *f4 = make_float4(o.x + fx, o.y + fy, o.z + fz, o.w);
+            if (W.assign) {
+                *f4 = make_float4(fx, fy, fz, 0.f);
+            } else {
+                const float4 o = *f4;
+                *f4 = make_float4(o.x + fx, o.y + fy, o.z + fz, o.w);
+            }
         } else {
             float *f3 = ff + 3 * (long)a;
             f3[0] += fx;
@@ -299,11 +317,59 @@ __global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, WallVel 
             f3[2] += fz;
         }
     }
-    if (pairs) {
+    if (W.pairs) {
         unsigned long long c = (unsigned long long)npair;
         for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-        if ((threadIdx.x & 63) == 0 && c) atomicAdd(pairs, c);
+        if ((threadIdx.x & 63) == 0 && c) atomicAdd(W.pairs, c);
     }
+}
+
+template <int WR, bool EXACT>
+__global__ __launch_bounds__(WALL_THREADS) void k_wall_force(SdfView v, WallVel wv, int nmax, const int *n_dev,
+                                                             const float4 *pos, const float4 *vel, WallArgs W) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    wall_one<WR, EXACT>(v, wv, a, a < nmax && a < *n_dev, pos, vel, W);
+}
+
+// the same over the cell-ordered particles of the runs of cells that can
+// hold a particle near a wall (SdfField::segs): one block per run
+template <int WR, bool EXACT>
+__global__ __launch_bounds__(WALL_THREADS) void k_wall_force_cells(SdfView v, WallVel wv, const int2 *segs,
+                                                                   const int *starts, const float4 *pos,
+                                                                   const float4 *vel, WallArgs W) {
+    const int2 sg = segs[blockIdx.x];
+    const int a0 = starts[sg.x], a1 = starts[sg.y];
+    for (int base = a0; base < a1; base += WALL_THREADS) {
+        const int a = base + (int)threadIdx.x;
+        wall_one<WR, EXACT>(v, wv, a, a < a1, pos, vel, W);
+    }
+}
+
+// Cells of the box that can hold a particle that is not sdf_far: the texels
+// sdf_cheap can pick for a position in cell c (the rounded texture coordinates
+// of the cell's float bounds, monotone in the position) hold a value above
+// cheap_threshold - 1.
+__global__ __launch_bounds__(256) void k_sdf_near_cells(SdfView v, int Lx, int Ly, int Lz, int *flag) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Lx * Ly * Lz) return;
+    const int cc[3] = {c % Lx, (c / Lx) % Ly, c / (Lx * Ly)};
+    const int L[3] = {Lx, Ly, Lz};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        const float xlo = (float)((double)cc[d] - 0.5 * (double)L[d]);
+        const float xup = (float)((double)(cc[d] + 1) - 0.5 * (double)L[d]);   // exact (a half-integer)
+        const unsigned ub = __float_as_uint(xup);
+        // the float below xup
+        const float xhi = xup > 0.f ? __uint_as_float(ub - 1u) : (xup < 0.f ? __uint_as_float(ub + 1u) : -1e-45f);
+        lo[d] = sdf_iround(__builtin_fmaf(v.s[d], xlo, v.o[d]));
+        hi[d] = sdf_iround(__builtin_fmaf(v.s[d], xhi, v.o[d]));
+    }
+    const float far = v.cheap_threshold - 1;
+    int near = 0;
+    for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+            for (int i = lo[0]; i <= hi[0]; i++) near |= !(sdf_fetch(v, i, j, k) <= far);
+    flag[c] = near;
 }
 
 __global__ __launch_bounds__(256) void k_wall_bounce_aos(SdfView v, WallVel wv, float dt, int n, float *pp) {
@@ -359,6 +425,47 @@ __global__ void k_aos_pos4(int n, const float *pp, float4 *pos) {
 }  // namespace
 
 // ------------------------------------------------------------ host side
+// the runs of consecutive cells (at most SEG_CELLS each) that can hold a
+// particle near a wall, for the wall force of cell-ordered particles
+constexpr int SEG_CELLS = 16;
+static int sdf_near_segments(hipStream_t s, SdfField *q) {
+    const int nc = q->L[0] * q->L[1] * q->L[2];
+    dev_free(q->segs);
+    q->segs = nullptr;
+    q->nseg = 0;
+    int *flag = nullptr;
+    UDX_TRY(dev_alloc(&flag, nc));
+    std::vector<int> h(nc);
+    hipLaunchKernelGGL(k_sdf_near_cells, dim3(ceil_div(nc, 256)), dim3(256), 0, s, q->view, q->L[0], q->L[1], q->L[2],
+                       flag);
+    int st = hipGetLastError() == hipSuccess ? UDX_OK : UDX_EHIP;
+    if (!st && (hipMemcpyAsync(h.data(), flag, sizeof(int) * (size_t)nc, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        st = UDX_EHIP;
+    dev_free(flag);
+    if (st) {
+        set_error("sdf: near-wall cells failed");
+        return st;
+    }
+    std::vector<int2> seg;
+    for (int c = 0; c < nc;) {
+        if (!h[c]) {
+            c++;
+            continue;
+        }
+        int e = c;
+        while (e < nc && h[e] && e - c < SEG_CELLS) e++;
+        seg.push_back(make_int2(c, e));
+        c = e;
+    }
+    if (seg.empty()) return UDX_OK;
+    UDX_TRY(dev_alloc(&q->segs, (long)seg.size()));
+    UDX_HIP(hipMemcpyAsync(q->segs, seg.data(), sizeof(int2) * seg.size(), hipMemcpyHostToDevice, s));
+    UDX_HIP(hipStreamSynchronize(s));
+    q->nseg = (int)seg.size();
+    return UDX_OK;
+}
+
 int sdf_build(hipStream_t s, SdfField *q, const float ext[3], const int N[3], const float *D_host) {
     const long n0 = (long)N[0] * N[1] * N[2];
     UDX_CHECK(n0 > 0, UDX_EINVAL, "sdf: empty field");
@@ -398,6 +505,7 @@ int sdf_build(hipStream_t s, SdfField *q, const float ext[3], const int N[3], co
     }
     to_grid(lo, hi, q->T, q->view.o, q->view.s);
     q->view.cheap_threshold = -sqrtf(3.f) * ((float)(q->L[0] + 2 * M) / (float)q->T[0]);
+    UDX_TRY(sdf_near_segments(s, q));
     q->ready = 1;
     return UDX_OK;
 }
@@ -523,14 +631,24 @@ void wall_free(WallSet *w) {
 
 int wall_forces(hipStream_t s, const SdfField *q, const WallSet *w, int nmax, const int *n_dev, const float4 *pos,
                 const float4 *vel, const udx_pair_dpd &p, const float *seed_dev, float *ff, int ff_stride,
-                unsigned long long *pairs, int exact, const WallVel &wv) {
+                unsigned long long *pairs, int exact, const WallVel &wv, int assign, const int *cell_starts) {
     if (nmax <= 0 || w->n <= 0) return UDX_OK;
     const Box &b = w->cl.box;
-    DpdPar par{p.a, p.g, p.s, p.spow};
-    const dim3 grid(ceil_div(nmax, WALL_THREADS)), blk(WALL_THREADS);
-#define UDX_WALL_LAUNCH(WR, EX)                                                                              \
-    hipLaunchKernelGGL((k_wall_force<WR, EX>), grid, blk, 0, s, q->view, wv, nmax, n_dev, pos, vel, w->pos,   \
-                       w->cl.starts, b.L[0], b.L[1], b.L[2], par, seed_dev, ff, ff_stride, pairs)
+    const WallArgs W{w->pos, w->cl.starts, b.L[0], b.L[1], b.L[2], DpdPar{p.a, p.g, p.s, p.spow}, seed_dev, ff,
+                     ff_stride, pairs, assign};
+    // cell-ordered particles: only the runs of cells that can hold a particle
+    // near a wall
+    const bool cells = cell_starts && q->segs;
+    if (cells && q->nseg == 0) return UDX_OK;
+    const dim3 grid(cells ? q->nseg : ceil_div(nmax, WALL_THREADS)), blk(WALL_THREADS);
+#define UDX_WALL_LAUNCH(WR, EX)                                                                                \
+    do {                                                                                                       \
+        if (cells)                                                                                             \
+            hipLaunchKernelGGL((k_wall_force_cells<WR, EX>), grid, blk, 0, s, q->view, wv, q->segs, cell_starts, \
+                               pos, vel, W);                                                                   \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_wall_force<WR, EX>), grid, blk, 0, s, q->view, wv, nmax, n_dev, pos, vel, W);   \
+    } while (0)
     const float sp = p.spow;
     if (exact) {
         if (sp == 0.25f) UDX_WALL_LAUNCH(WR_QUARTER, true);
