@@ -653,19 +653,14 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
         wv = wall_vel_view(f->wall->wvel, f->t);
         sdf = wall_sdf(f->wall);
     }
-    // Without objects the gather also moves the particles (into the other
-    // pair of streams, which the next step's reorder reads): the wall forces
-    // go first, into ff of the particles near a wall, and the gather adds
-    // them to its sums, so the forces and the move are those of the passes
-    // one after the other.
+    // Without objects the gather also evaluates the wall forces and moves the
+    // particles (into the other pair of streams, which the next step's reorder
+    // reads): the forces and the move are those of the passes one after the
+    // other.
     f->fused = !m.r && dpd_move_fused(d, f->conf.mode);
     if (f->fused) {
-        if (f->wall)
-            UDX_TRY(wall_forces(s, sdf, wall_set(f->wall), f->n, f->cl.starts + nc, f->pos[b], f->vel[b],
-                                f->conf.dpd, seeds_dev + 27, (float *)f->ff, 4, f->pairs_dev + 1,
-                                f->conf.mode == UDX_MODE_EXACT, wv, 1, f->cl.starts));
         if (f->evk) UDX_HIP(hipEventRecord(f->evk[2], s));
-        FusedMove mv;
+        FusedMove mv{};
         mv.opos = f->pos[a];
         mv.ovel = f->vel[a];
         mv.dt = f->conf.dt;
@@ -674,6 +669,15 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
         mv.walls = sdf != nullptr;
         mv.sdf = sdf ? sdf->view : SdfView{};
         mv.wv = wv;
+        if (sdf) {
+            const WallSet *w = wall_set(f->wall);
+            mv.wpos = w->pos;
+            mv.wstarts = w->cl.starts;
+            for (int c = 0; c < 3; c++) mv.W[c] = w->cl.box.L[c];
+            mv.nwall = w->n;
+            mv.wseed = seeds_dev + 27;
+            mv.wpairs = f->pairs_dev + 1;
+        }
         mv.next = &f->cl;
         d.move = &mv;
         UDX_TRY(dpd_forces_mode(s, d, f->conf.mode));

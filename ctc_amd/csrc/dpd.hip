@@ -66,6 +66,13 @@ struct DpdK {
     SdfView sdf;
     WallVel wv;
     SlabCount sc;
+    // the wall pass (MOVE == 2): frozen wall particles, their cell list on
+    // the margin grid W, the wall noise seed and the wall pair counter
+    const float4 *wpos;
+    const int *wstarts;
+    int Wx, Wy, Wz, nwall;
+    const float *wseed;
+    unsigned long long *wpairs;
 };
 
 // What the force pass reads of a box, read once at kernel entry.  The
@@ -651,8 +658,111 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         __builtin_amdgcn_wave_barrier();
     }
 
+    // ---- wall pass (MOVE == 2): the particles that are not sdf_far and the
+    // frozen wall particles in the 27 cells of the wall cell list around
+    // them, as k_wall_force evaluates them (candidate order, noise
+    // mean0var1ii(seed_wall, i, wall index), wall velocity at the wall
+    // particle); summed apart and added after the solvent sum.  The ring is
+    // the wave's queue region (its bulk and halo queues are empty here).
+    float wsx = 0.f, wsy = 0.f, wsz = 0.f;
+    bool wnear = false;
+    if (MOVE == 2) {
+        wnear = active && !sdf_far(A.sdf, pi.x, pi.y, pi.z);
+        if (A.nwall > 0 && wave_any(wnear)) {
+            constexpr unsigned WQ = (unsigned)QC / 2u;   // 32-bit slots per lane in QC * 128 bytes
+            const int Wx = A.Wx, Wy = A.Wy, Wz = A.Wz;
+            const UDX_GLOBAL f4v *const wp = (const UDX_GLOBAL f4v *)A.wpos;
+            const UDX_GLOBAL int *const ws = gptr(A.wstarts);
+            int b0x = 1, b0y = 1, b0z = 1;
+            if (wnear) {
+                auto clampc = [](float x, int W) {
+                    const int qq = (int)__builtin_floor((double)x + 0.5 * (double)W);
+                    return qq < 1 ? 1 : (qq > W - 2 ? W - 2 : qq);
+                };
+                b0x = clampc(pi.x, Wx);
+                b0y = clampc(pi.y, Wy);
+                b0z = clampc(pi.z, Wz);
+            }
+            const float wseed = *A.wseed;
+            const float fa_lo = (float)(i & 0x3FF), fa_hi = (float)i;
+            const unsigned wlane = (unsigned)((tid & 63) * 4);
+            auto wslot = [&](unsigned k) { return (unsigned *)(qw + (((k & (WQ - 1u)) << 8) | wlane)); };
+            unsigned wh = 0, wt = 0;
+            int nw = 0;
+            auto wstep = [&]() {
+                const unsigned fl = wt - wh;
+                const bool ok0 = fl > 0u, ok1 = fl > 1u;
+                const unsigned e0 = ok0 ? *wslot(wh) : 0u;   // wall particle 0 exists (nwall > 0)
+                const unsigned e1 = ok1 ? *wslot(wh + 1u) : e0;
+                const f4v w0 = wp[e0], w1 = wp[e1];
+                const float dx0 = pi.x - w0.x, dy0 = pi.y - w0.y, dz0 = pi.z - w0.z;
+                const float dx1 = pi.x - w1.x, dy1 = pi.y - w1.y, dz1 = pi.z - w1.z;
+                const float r20 = __builtin_fmaf(dz0, dz0, __builtin_fmaf(dx0, dx0, dy0 * dy0));
+                const float r21 = __builtin_fmaf(dz1, dz1, __builtin_fmaf(dx1, dx1, dy1 * dy1));
+                // mean0var1ii(seed, i, wall) (src/math/rnd/dev.h:57-63)
+                float x0 = wseed - low_discrepancy(fa_lo, fa_hi, (float)((int)e0 & 0x3FF), (float)(int)e0);
+                float x1 = wseed - low_discrepancy(fa_lo, fa_hi, (float)((int)e1 & 0x3FF), (float)(int)e1);
+                logistic18x2(&x0, &x1);
+                float u0[3], u1[3];
+                wvel_at(A.wv, w0.x, w0.y, w0.z, u0);
+                wvel_at(A.wv, w1.x, w1.y, w1.z, u1);
+                const float3 f0 = dpd_pair<WR, EXACT>(P.par, dx0, dy0, dz0, r20, vi.x - u0[0], vi.y - u0[1],
+                                                      vi.z - u0[2], x0 * R_SQRT2);
+                const float3 f1 = dpd_pair<WR, EXACT>(P.par, dx1, dy1, dz1, r21, vi.x - u1[0], vi.y - u1[1],
+                                                      vi.z - u1[2], x1 * R_SQRT2);
+                if (ok0) {
+                    wsx += f0.x;
+                    wsy += f0.y;
+                    wsz += f0.z;
+                }
+                if (ok1) {
+                    wsx += f1.x;
+                    wsy += f1.y;
+                    wsz += f1.z;
+                }
+                const unsigned took = fl < 2u ? fl : 2u;
+                wh += took;
+                nw += (int)took;
+            };
+            __builtin_amdgcn_wave_barrier();
+            for (int dz = -1; dz <= 1; dz++)
+                for (int dy = -1; dy <= 1; dy++) {
+                    int c0 = 0, len = 0;
+                    if (wnear) {
+                        const int c = (b0x - 1) + Wx * ((b0y + dy) + Wy * (b0z + dz));
+                        c0 = ws[c];
+                        len = ws[c + 3] - c0;
+                    }
+                    for (int t = 0; wave_any(t < len); t += 4) {
+                        if (wave_any(wt - wh + 4u > WQ))
+#pragma unroll 1
+                            for (int k = 0; k < DRAIN_STEPS; k++) wstep();
+                        f4v pw[4];
+#pragma unroll
+                        for (int u = 0; u < 4; u++) pw[u] = t + u < len ? wp[c0 + t + u] : f4v{1e30f, 1e30f, 1e30f, 0.f};
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float dx = pi.x - pw[u].x, dyy = pi.y - pw[u].y, dzz = pi.z - pw[u].z;
+                            const float r2 = __builtin_fmaf(dzz, dzz, __builtin_fmaf(dx, dx, dyy * dyy));
+                            const bool hit = t + u < len && r2 < 1.0f;
+                            if (hit) *wslot(wt) = (unsigned)(c0 + t + u);
+                            wt += hit ? 1u : 0u;
+                        }
+                    }
+                }
+            while (wave_any(wt != wh)) wstep();
+            __builtin_amdgcn_wave_barrier();
+            if (A.wpairs) {
+                unsigned long long c = (unsigned long long)nw;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+                if ((tid & 63) == 0 && c) atomicAdd(A.wpairs, c);
+            }
+        }
+    }
+
     if (MOVE) {
-        // the step's total force (wall force added for the particles near a
+        // the step's total force (the wall sum added for the particles near a
         // wall: 0 + sum, then + wall, as ff = sum and ff += wall), the
         // kick-drift, the bounce-back and the wrap of move_f4_wall, then the
         // count into the next step's x slabs
@@ -660,11 +770,10 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         if (active) {
             UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
             float gx = 0.f + fx, gy = 0.f + fy, gz = 0.f + fz;
-            if (MOVE == 2 && !sdf_far(A.sdf, pi.x, pi.y, pi.z)) {
-                const f4v w = *f4;
-                gx = gx + w.x;
-                gy = gy + w.y;
-                gz = gz + w.z;
+            if (MOVE == 2 && wnear) {
+                gx = gx + wsx;
+                gy = gy + wsy;
+                gz = gz + wsz;
             }
             *f4 = f4v{gx, gy, gz, 0.f};
             p = pi;
@@ -1161,6 +1270,11 @@ DpdK make_k(const DpdLaunch &a) {
     k.sdf = SdfView{};
     k.wv = WallVel{};
     k.sc = SlabCount{};
+    k.wpos = nullptr;
+    k.wstarts = nullptr;
+    k.Wx = k.Wy = k.Wz = k.nwall = 0;
+    k.wseed = nullptr;
+    k.wpairs = nullptr;
     if (const FusedMove *m = a.move) {
         k.opos = m->opos;
         k.ovel = m->ovel;
@@ -1172,6 +1286,14 @@ DpdK make_k(const DpdLaunch &a) {
         if (m->walls) {
             k.sdf = m->sdf;
             k.wv = m->wv;
+            k.wpos = m->wpos;
+            k.wstarts = m->wstarts;
+            k.Wx = m->W[0];
+            k.Wy = m->W[1];
+            k.Wz = m->W[2];
+            k.nwall = m->nwall;
+            k.wseed = m->wseed;
+            k.wpairs = m->wpairs;
         }
         for (int c = 0; c < 3; c++) k.sc.L[c] = m->next->box.L[c];
         k.sc.sub = m->next->sub;

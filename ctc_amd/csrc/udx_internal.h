@@ -153,18 +153,23 @@ struct DpdLaunch {
     const struct FusedMove *move = nullptr;
 };
 // The move fused into the gather, for the solvent step without objects: after
-// its sum each particle adds its wall force (written to ff before the pass by
-// wall_forces with assign set, for the particles that are not sdf_far), stores
-// the total in ff, kick-drifts with the body force, bounces off the walls,
-// wraps, writes itself to opos/ovel at its own index and counts itself into
-// next's x slabs: what move_f4_wall does after the pass, with the same
-// operations in the same order.
+// its sum each particle that is not sdf_far adds its wall force (the gather's
+// wall pass: what wall_forces computes), stores the total in ff, kick-drifts
+// with the body force, bounces off the walls, wraps, writes itself to
+// opos/ovel at its own index and counts itself into next's x slabs: what
+// wall_forces and move_f4_wall do after the pass, with the same operations in
+// the same order.
 struct FusedMove {
     float4 *opos, *ovel;
     float dt, mass, accel[3];
-    int walls;         // sdf and wv are set
+    int walls;         // the fields below are set
     SdfView sdf;
     WallVel wv;
+    const float4 *wpos;        // frozen wall particles (nwall) in the cell order of the margin grid W
+    const int *wstarts;
+    int W[3], nwall;
+    const float *wseed;        // device, the step's wall noise seed
+    unsigned long long *wpairs;   // solvent-wall pair counter or null
     ClistBufs *next;
 };
 // whether dpd_forces_mode can run a with a.move set: the gather (FAST or

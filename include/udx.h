@@ -257,7 +257,9 @@ int udx_flu_set_time(udx_flu *f, float t);
 int udx_flu_set_graph(udx_flu *f, int enable);
 /* per-step kernel timing of the last udx_flu_run (ms, HIP events on the
  * bound stream): [0] cell list + reorder + fragment tables, [1] solvent DPD,
- * [2] wall DPD, [3] integrate + bounce-back, [4] their sum */
+ * [2] wall DPD, [3] integrate + bounce-back, [4] their sum.  Without objects
+ * the solvent DPD kernel also evaluates the wall forces and moves the
+ * particles: [1] holds all three and [2], [3] are the gaps between them. */
 int udx_flu_timing(udx_flu *f, int enable, float out[5]);
 
 /* ---- RBC membrane: replaces rbc_force_ini / rbc_force_set_conf /
