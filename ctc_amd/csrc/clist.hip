@@ -51,30 +51,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *in, int
     if (threadIdx.x == SCAN_THREADS - 1) part[blockIdx.x] = s;
 }
 
-// exclusive scan of the per-tile sums by one block; part_ex[nb] = total
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(const int *part, int nb, int *part_ex) {
-    __shared__ int lw[4];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int b0 = 0; b0 < nb; b0 += SCAN_THREADS) {
-        int idx = b0 + threadIdx.x;
-        int v = idx < nb ? part[idx] : 0;
-        int inc = block_inclusive_scan(v, lw);
-        int c = carry;
-        if (idx < nb) part_ex[idx] = c + inc - v;
-        __syncthreads();
-        if (threadIdx.x == SCAN_THREADS - 1) carry = c + inc;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part_ex[nb] = carry;
-}
-
-// zero: the input is cleared after it is read (counts consumed by the scan
-// are ready for the next counting pass)
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, const int *part_ex, int nb, int *out,
+// Each tile's offset is the sum of the tile totals before it, added up by
+// the tile's own block (integers, so the order of the additions does not
+// matter): no launch between the two passes.  zero: the input is cleared
+// after it is read (counts consumed by the scan are ready for the next
+// counting pass).  out[n] = the total.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, const int *part, int nb, int *out,
                                                              int zero) {
     __shared__ int lw[4];
+    __shared__ int tile0;
+    int pre = 0;
+    for (int k = threadIdx.x; k < (int)blockIdx.x; k += SCAN_THREADS) pre += part[k];
+    pre = block_inclusive_scan(pre, lw);
+    if (threadIdx.x == SCAN_THREADS - 1) tile0 = pre;
+    __syncthreads();
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
     int v[SCAN_ITEMS], s = 0;
 #pragma unroll
@@ -86,14 +76,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, con
 #pragma unroll
         for (int k = 0; k < SCAN_ITEMS; k++)
             if (base + k < n) in[base + k] = 0;
-    int inc = block_inclusive_scan(s, lw);
-    int run = part_ex[blockIdx.x] + inc - s;
+    const int inc = block_inclusive_scan(s, lw);
+    int run = tile0 + inc - s;
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part_ex[nb];
+    if (blockIdx.x == nb - 1 && threadIdx.x == SCAN_THREADS - 1) out[n] = tile0 + inc;
 }
 
 __global__ void k_fill_int(int *p, int n, int v) {
@@ -141,44 +131,72 @@ __global__ __launch_bounds__(256) void k_clist_scatter(int n, const int2 *cellsl
     if (cs.x >= 0) order[starts[cs.x] + cs.y] = i;
 }
 
-// sort each cell's input indices ascending: the stable order
-// A cell of at most 16 particles is sorted in registers by a bitonic network
-// (one load and one store per entry); larger cells by insertion in memory.
+// ascending bitonic network over the entries [s, e) of order (e - s <= N),
+// in registers: one load and one store per entry
+template <int N>
+__device__ __forceinline__ void sort_net(int s, int e, int *order) {
+    int v[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = s + k < e ? order[s + k] : 0x7fffffff;
+#pragma unroll
+    for (int size = 2; size <= N; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                const int j = k ^ stride;
+                if (j > k) {
+                    const int lo = min(v[k], v[j]), hi = max(v[k], v[j]);
+                    const bool up = (k & size) == 0;
+                    v[k] = up ? lo : hi;
+                    v[j] = up ? hi : lo;
+                }
+            }
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if (s + k < e) order[s + k] = v[k];
+}
+
+// sort each cell's (slab's) input indices ascending: the stable order.  A
+// wave sorts its cells with the network sized for its largest cell (2, 4, 8
+// or 16 entries: the same for every lane; slabs hold a few particles, so the
+// 16-entry network would mostly sort padding); cells above 16 by insertion
+// in memory.
 __global__ __launch_bounds__(256) void k_clist_cellsort(int ncells, const int *starts, int *order) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    int s = starts[c], e = starts[c + 1];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = 0, e = 0;
+    if (c < ncells) {
+        s = starts[c];
+        e = starts[c + 1];
+    }
+    int m = e - s;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if (m <= 1) return;
+    if (m <= 2) {
+        sort_net<2>(s, e, order);
+        return;
+    }
+    if (m <= 4) {
+        sort_net<4>(s, e, order);
+        return;
+    }
+    if (m <= 8) {
+        sort_net<8>(s, e, order);
+        return;
+    }
     if (e - s <= 1) return;
     if (e - s <= 16) {
-        int v[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) v[k] = s + k < e ? order[s + k] : 0x7fffffff;
-#pragma unroll
-        for (int size = 2; size <= 16; size <<= 1)
-#pragma unroll
-            for (int stride = size >> 1; stride > 0; stride >>= 1)
-#pragma unroll
-                for (int k = 0; k < 16; k++) {
-                    const int j = k ^ stride;
-                    if (j > k) {
-                        const int lo = min(v[k], v[j]), hi = max(v[k], v[j]);
-                        const bool up = (k & size) == 0;
-                        v[k] = up ? lo : hi;
-                        v[j] = up ? hi : lo;
-                    }
-                }
-#pragma unroll
-        for (int k = 0; k < 16; k++)
-            if (s + k < e) order[s + k] = v[k];
+        sort_net<16>(s, e, order);
         return;
     }
     for (int k = s + 1; k < e; k++) {
-        int v = order[k], m = k - 1;
-        while (m >= s && order[m] > v) {
-            order[m + 1] = order[m];
-            m--;
+        int v = order[k], w = k - 1;
+        while (w >= s && order[w] > v) {
+            order[w + 1] = order[w];
+            w--;
         }
-        order[m + 1] = v;
+        order[w + 1] = v;
     }
 }
 
@@ -212,12 +230,10 @@ int scan_scratch_size(int n) {
 static int scan_impl(hipStream_t s, int *in, int n, int *out, int *scratch, int zero) {
     int nb = ceil_div(n, SCAN_TILE);
     if (nb == 0) nb = 1;
-    int *part = scratch, *part_ex = scratch + nb;
+    int *part = scratch;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part);
     UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_THREADS), 0, s, part, nb, part_ex);
-    UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part_ex, nb, out, zero);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part, nb, out, zero);
     UDX_LAUNCHED();
     return UDX_OK;
 }
