@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--phase-steps", type=int, default=50, help="steps of the per-phase timing run after the timed one")
     ap.add_argument("--L", type=int, default=96)
     ap.add_argument("--rho", type=float, default=10.0)
     ap.add_argument("--half-width", type=float, default=40.0, help="channel half-width (0: periodic, no walls)")
@@ -158,33 +159,39 @@ def main():
     flu.run(args.warmup)
     torch.cuda.synchronize()
 
-    # timed region; per-phase kernel times from HIP events recorded on the
-    # library's stream around each phase of every timed step (read after the
-    # run: no host synchronisation inside it)
+    # timed region: the steps alone (no timing events)
     flu.pairs(reset=True)
     w0 = flu.wall_pairs() if walls else 0
-    flu.timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     flu.run(args.steps)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    t_phase = flu.timing(False)          # ms: clist, solvent DPD, wall DPD, integrate, sum
     pairs = flu.pairs()
     wpairs = (flu.wall_pairs() - w0) if walls else 0
+    # per-phase kernel times from HIP events recorded on the library's stream
+    # around each phase of the next steps (read after the run)
+    flu.timing(True)
+    flu.run(args.phase_steps)
+    torch.cuda.synchronize()
+    t_phase = flu.timing(False)          # ms: clist, solvent DPD, wall DPD, integrate, sum
     sec = t1 - t0
     ms_step = 1e3 * sec / args.steps
     steps_s = args.steps / sec
     pairs_step = (pairs + wpairs) / args.steps
     pairs_s = (pairs + wpairs) / sec
 
-    # dominant kernel: the solvent force gather k_dpd, one launch per step.
-    # Algorithmic bytes per launch = n * 36 B (24 B particle in, 12 B force out)
+    # dominant kernel: the solvent force gather k_dpd, one launch per step;
+    # in this step (no objects) it also evaluates the wall forces, moves the
+    # particles and counts them into the next step's slabs.  Algorithmic bytes
+    # per launch = n * (36 + 60 + 16) B: forces (24 B particle in, 12 B force
+    # out), integrate (36 B in, 24 B out), slab count (16 B)
     t_force = t_phase[1] * 1e-3
-    force_bytes = n * 36.0
+    force_bytes = n * (36.0 + 60.0 + 16.0)
     force_gbs = force_bytes / t_force / 1e9
-    # algorithmic FLOPs: 100 per pair (RNG included) + 9 per half-shell distance test
-    flops = (pairs / args.steps) * 100.0 + n * 13.5 * args.rho * 9.0
+    # algorithmic FLOPs: 100 per solvent or wall pair (RNG included) + 9 per
+    # half-shell distance test
+    flops = ((pairs + wpairs) / args.steps) * 100.0 + n * 13.5 * args.rho * 9.0
     step_bytes = n * 164.0 + ncells * 16.0
     traffic = read_traffic()
 
@@ -214,10 +221,14 @@ def main():
             "walls": walls, "sdf_grid": args.sdf_grid if walls else None,
             "parallelism": "single GPU",
         },
+        # HIP-event phases of args.phase_steps further steps; the gather also
+        # does the walls and the move, so wall_dpd and integrate are gaps
         "phase_ms": {"clist": t_phase[0], "dpd": t_phase[1], "wall_dpd": t_phase[2], "integrate": t_phase[3]},
+        "phase_steps": args.phase_steps,
         "setup_s": t_setup,
         "roofline": {
-            "bound": "hbm", "kernel": "k_dpd (solvent force gather)",
+            "bound": "hbm", "kernel": "k_dpd (solvent force gather with the wall forces, the move and the slab "
+                                      "count)",
             "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": force_gbs / HBM_PEAK_GBS,
             "traffic": traffic,
         },
