@@ -831,11 +831,14 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
 }
 
 // at most 102 VGPRs: five waves per SIMD (measured 10% faster on the 96^3 box
-// than four waves at 104 VGPRs; six spill)
+// than four waves at 104 VGPRs; six spill).  Blocks in launch order: the
+// hardware deals them to the XCDs round robin, which spreads the waves near
+// the walls (the first and last planes of a channel) over every XCD; giving
+// each XCD a contiguous eighth of the cells fetched fewer bytes but left the
+// wall work on two XCDs (1% slower with the wall pass fused).
 template <int WR, bool EXACT, int PHASE, int MOVE = 0>
 __global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void k_dpd(DpdK A) {
-    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP, false, PHASE, MOVE>(A,
-                                                                      xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+    dpd_block<WR, EXACT, DPD_THREADS, QCAP, SCAP, false, PHASE, MOVE>(A, blockIdx.x * DPD_THREADS);
 }
 
 // with the per-particle stress (the reference's FoSArray path,
