@@ -103,9 +103,10 @@ struct FsiK {
 // one candidate in six passes and every lane of the wave would wait for the
 // body at nearly every candidate.  The queues keep candidate order, so the
 // sums are the same as a sequential walk.
-constexpr int FQ_V = 24, FQ_S = 16;   // ring entries per lane: instance side, solvent side
+constexpr int FQ_V = 16, FQ_S = 16;   // ring entries per lane: instance side, solvent side
 constexpr int FSI_DRAIN_STEPS = 2;
 constexpr int FSI_SCAP = 160;         // staged instances per wave and row (solvent side)
+constexpr int FSI_VCAP = 160;         // staged solvent particles per wave and row (instance side)
 struct FsiIn {
     float dx, dy, dz, r2, dvx, dvy, dvz, x;
     bool pair;
@@ -122,8 +123,10 @@ __global__ __launch_bounds__(FSI_THREADS) void k_fsi_instance(FsiK K, const int 
                                                               const int2 *meta, const int *order, float *sinst,
                                                               unsigned long long *pairs) {
     __shared__ unsigned q[FQ_V * FSI_THREADS];
+    __shared__ float4 vstage[FSI_THREADS / 64][FSI_VCAP];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = t < *n_dev;
+    float4 *const sv = vstage[threadIdx.x >> 6];
     const float *p = inst + 6 * (long)(active ? t : 0);
     const float rx = p[0], ry = p[1], rz = p[2], ux = p[3], uy = p[4], uz = p[5];
     const int2 m = active ? meta[t] : make_int2(0, 0);
@@ -187,6 +190,41 @@ __global__ __launch_bounds__(FSI_THREADS) void k_fsi_instance(FsiK K, const int 
             const int cid = xs + K.Lx * (y + K.Ly * z);
             b0 = K.starts[cid];
             len = K.starts[cid + (xe - xs)] - b0;
+        }
+        // the lanes are consecutive instances in cell order: the union of
+        // their solvent runs of this row is usually one small range, staged
+        // in LDS with coalesced loads (one round trip instead of one per four
+        // candidates); the candidate order is the same either way
+        int lo = len > 0 ? b0 : 0x7fffffff, hi = len > 0 ? b0 + len : 0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+        }
+        if (hi <= lo) continue;
+        if (hi - lo <= FSI_VCAP) {
+            __builtin_amdgcn_wave_barrier();
+            for (int k = threadIdx.x & 63; k < hi - lo; k += 64) sv[k] = K.spos[lo + k];
+            __builtin_amdgcn_wave_barrier();
+            const int o = len > 0 ? b0 - lo : 0;
+            for (int c = 0; wave_any(c < len); c += 4) {
+                if (wave_any(fill + 4 > FQ_V))
+#pragma unroll 1
+                    for (int k = 0; k < FSI_DRAIN_STEPS; k++) drain_step();
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = o + c + u < FSI_VCAP ? o + c + u : 0;
+                    const float4 pb = c + u < len ? sv[k] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+                    const float dx = rx - pb.x, dy = ry - pb.y, dz = rz - pb.z;
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
+                    if (c + u < len && r2 < 1.0f) {
+                        ql[ring(fill) * FSI_THREADS] = (unsigned)(b0 + c + u);
+                        fill++;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            continue;
         }
         // four candidates per iteration: four loads in flight per lane
         for (int c = 0; wave_any(c < len); c += 4) {
