@@ -761,13 +761,16 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
         }
     }
 
-    if (MOVE) {
+    if (MOVE || A.opos) {
         // the step's total force (the wall sum added for the particles near a
         // wall: 0 + sum, then + wall, as ff = sum and ff += wall), the
         // kick-drift, the bounce-back and the wrap of move_f4_wall, then the
-        // count into the next step's x slabs
+        // count into the next step's x slabs (MOVE: the single-box kernel;
+        // A.opos without MOVE: a batch without walls, where a particle
+        // listed for the halo launch is moved there)
+        const bool mine = active && !listed;
         float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (active) {
+        if (mine) {
             UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
             float gx = 0.f + fx, gy = 0.f + fy, gz = 0.f + fz;
             if (MOVE == 2 && wnear) {
@@ -794,8 +797,8 @@ __device__ __forceinline__ void dpd_block(const DpdK &A, int first) {
             *((UDX_GLOBAL f4v *)A.opos + i) = f4v{p.x, p.y, p.z, 0.f};
             *((UDX_GLOBAL f4v *)A.ovel + i) = f4v{v.x, v.y, v.z, 0.f};
         }
-        clist_count_one(active, i, make_float3(p.x, p.y, p.z), A.sc.L[0], A.sc.L[1], A.sc.L[2], A.sc.sub,
-                        A.sc.counts, A.sc.cellslot);
+        clist_count_one(mine, i, make_float3(p.x, p.y, p.z), A.sc.L[0], A.sc.L[1], A.sc.L[2], A.sc.sub,
+                        A.sc.counts, A.sc.cellslot, A.sc.slab0);
     } else if (active && !listed) {
         if (A.ff_stride == 4) {
             UDX_GLOBAL f4v *f4 = (UDX_GLOBAL f4v *)A.ff + i;
@@ -854,7 +857,9 @@ __global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
 // (five waves per SIMD: the 16-bit queues of every tile variant leave room)
 template <int WR, bool EXACT, int NT, int QC, int PHASE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5))) void k_dpd_batch(
-    const DpdK *boxes, const int *bblock, int nbox) {
+    const DpdK *boxes, const int *bblock, int nbox, int *step) {
+    // the fused moves' step counter (the step's seeds were read before it)
+    if (PHASE != PH_HALO && step && blockIdx.x == 0 && threadIdx.x == 0) *step += 1;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     int lo = 0, hi = nbox;
     while (hi - lo > 1) {
@@ -1298,10 +1303,11 @@ DpdK make_k(const DpdLaunch &a) {
             k.wseed = m->wseed;
             k.wpairs = m->wpairs;
         }
-        for (int c = 0; c < 3; c++) k.sc.L[c] = m->next->box.L[c];
+        for (int c = 0; c < 3; c++) k.sc.L[c] = a.box.L[c];
         k.sc.sub = m->next->sub;
         k.sc.counts = m->next->bcounts;
         k.sc.cellslot = m->next->cellslot;
+        k.sc.slab0 = m->slab0;
     }
     return k;
 }
@@ -1458,13 +1464,13 @@ int launch_batch(hipStream_t s, const DpdBatch &b) {
     const DpdK *tab = (const DpdK *)b.table;
     if (b.split) {
         hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC, PH_BULK>), dim3(b.nblocks[w]), dim3(NT), 0, s, tab,
-                           b.bblock[w], b.nbox);
+                           b.bblock[w], b.nbox, b.step);
         UDX_LAUNCHED();
         hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC, PH_HALO>), dim3(b.nblocks[w]), dim3(NT), 0, s, tab,
-                           b.bblock[w], b.nbox);
+                           b.bblock[w], b.nbox, b.step);
     } else {
         hipLaunchKernelGGL((k_dpd_batch<WR, EXACT, NT, QC, PH_ALL>), dim3(b.nblocks[w]), dim3(NT), 0, s, tab,
-                           b.bblock[w], b.nbox);
+                           b.bblock[w], b.nbox, b.step);
     }
     UDX_LAUNCHED();
     return UDX_OK;
@@ -1514,6 +1520,10 @@ int dpd_batch_build(DpdBatch *b, const DpdLaunch *boxes, int nbox) {
                   UDX_EINVAL, "dpd: the boxes of a batch share one float4 force array and one FP64 sum array");
     b->ntot = boxes[nbox - 1].i0 + boxes[nbox - 1].nmax;
     b->n_end = boxes[nbox - 1].starts + boxes[nbox - 1].box.ncells();
+    b->step = boxes[0].move ? boxes[0].move->step : nullptr;
+    for (int k = 0; k < nbox; k++)
+        UDX_CHECK(!boxes[k].move == !boxes[0].move && (!boxes[k].move || !boxes[k].move->walls), UDX_EINVAL,
+                  "dpd: the boxes of a batch are all moved in the gather (without walls) or none");
     double edges = 0.0;
     b->split = 1;
     for (int k = 0; k < nbox; k++) {

@@ -183,11 +183,27 @@ struct udx_ens {
     int graph_variant;
 };
 
+// Both parities' tables carry the boxes' moves: in FAST and EXACT mode the
+// gather moves the particles into the other pair of streams and counts them
+// into the next step's slabs (the half-shell mode moves them in k_ens_move).
 static int ens_build_batches(udx_ens *e) {
     for (int par = 0; par < 2; par++) {
         std::vector<DpdLaunch> d(e->nbox);
+        std::vector<FusedMove> mv(e->nbox);
         for (int b = 0; b < e->nbox; b++) {
+            FusedMove &m = mv[b];
+            m = FusedMove{};
+            m.opos = e->pos[1 - par];
+            m.ovel = e->vel[1 - par];
+            m.dt = e->box[b].dt;
+            m.mass = e->box[b].mass;
+            for (int c = 0; c < 3; c++) m.accel[c] = e->box[b].accel[c];
+            m.walls = 0;
+            m.next = &e->cl;
+            m.slab0 = e->coff_h[b] * SOLVENT_SUB;
+            m.step = e->step;
             DpdLaunch &a = d[b];
+            a.move = &m;
             a.pos = e->pos[par];
             a.vel = e->vel[par];
             a.starts = e->cl.starts + e->coff_h[b];
@@ -225,8 +241,10 @@ static int ens_count(udx_ens *e, int cur) {
 }
 
 // The move counts the next step's slabs, so a step starts from the scan
-// (ens_count first when the positions were set since).
-static int ens_step(udx_ens *e, int cur) {
+// (ens_count first when the positions were set since).  *next: the parity
+// that holds the state after the step (the gather's move writes it back to
+// cur; k_ens_move moves it in 1 - cur).
+static int ens_step(udx_ens *e, int cur, int *next) {
     hipStream_t s = e->s;
     const int a = cur, b = 1 - cur, n = e->ntot;
     if (!e->counted) UDX_TRY(ens_count(e, a));
@@ -239,10 +257,15 @@ static int ens_step(udx_ens *e, int cur) {
     UDX_LAUNCHED();
     UDX_TRY(scan_exclusive(s, e->fcnt, e->ftot, e->fss, e->fscan));
     UDX_TRY(dpd_forces_batch(s, e->batch[b], e->mode, e->variant));
-    hipLaunchKernelGGL(k_ens_move, dim3(ceil_div(n > 0 ? n : 1, 256)), dim3(256), 0, s, n, e->nbox, e->poff, e->mb,
-                       e->ff, e->pos[b], e->vel[b], e->coff, e->boxL, e->cl.bcounts, e->cl.cellslot, e->step);
-    UDX_LAUNCHED();
+    const bool fused = e->mode != UDX_MODE_HALF;
+    if (!fused) {
+        hipLaunchKernelGGL(k_ens_move, dim3(ceil_div(n > 0 ? n : 1, 256)), dim3(256), 0, s, n, e->nbox, e->poff,
+                           e->mb, e->ff, e->pos[b], e->vel[b], e->coff, e->boxL, e->cl.bcounts, e->cl.cellslot,
+                           e->step);
+        UDX_LAUNCHED();
+    }
     e->counted = 1;
+    *next = fused ? a : b;
     return UDX_OK;
 }
 
@@ -261,8 +284,10 @@ static int ens_capture(udx_ens *e) {
     const int counted0 = e->counted;
     e->counted = 1;
     UDX_HIP(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
-    int st = ens_step(e, e->cur);
-    if (!st) st = ens_step(e, 1 - e->cur);
+    int mid = e->cur, end = e->cur;
+    int st = ens_step(e, e->cur, &mid);
+    if (!st) st = ens_step(e, mid, &end);
+    if (!st && end != e->cur) st = UDX_EINVAL;   // the pair ends where it started
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e->s, &g);
     e->counted = counted0;
@@ -508,8 +533,9 @@ int udx_ens_run(udx_ens *e, int nsteps, const float *seeds) {
         for (; k + 2 <= nsteps; k += 2) UDX_HIP(hipGraphLaunch(e->exec, e->s));
     }
     for (; k < nsteps; k++) {
-        UDX_TRY(ens_step(e, e->cur));
-        e->cur = 1 - e->cur;
+        int next = e->cur;
+        UDX_TRY(ens_step(e, e->cur, &next));
+        e->cur = next;
     }
     UDX_HIP(hipEventRecord(e->done, e->s));
     UDX_HIP(hipStreamWaitEvent(e->ctx->stream, e->done, 0));

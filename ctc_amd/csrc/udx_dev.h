@@ -129,6 +129,7 @@ struct SlabCount {
     int L[3], sub;
     int *counts;
     int2 *cellslot;
+    int slab0;   // the box's first slab (boxes end to end in one list)
 };
 
 // One particle of a counting pass (every lane of the wave calls it; valid

@@ -171,6 +171,8 @@ struct FusedMove {
     const float *wseed;        // device, the step's wall noise seed
     unsigned long long *wpairs;   // solvent-wall pair counter or null
     ClistBufs *next;
+    int slab0 = 0;             // the box's first slab in next (boxes end to end)
+    int *step = nullptr;       // device step counter advanced once per step (batches)
 };
 // whether dpd_forces_mode can run a with a.move set: the gather (FAST or
 // EXACT) in one launch (no split halo) writing float4 forces
@@ -196,6 +198,9 @@ struct DpdBatch {
     int ntot = 0;
     const int *n_end = nullptr;   // device: end of the last box's cell list
     int split = 0;                // every box has an edge list: bulk and halo passes launched apart
+    // the boxes' moves fused into the gather (FusedMove of every box): the
+    // device step counter the first launch of a step advances, or null
+    int *step = nullptr;
 };
 // tile variants of the batched force kernel: threads per block and queued
 // pairs per lane; variant 0 is the default
