@@ -614,7 +614,7 @@ def host(t):
 
 def cpu_threads():
     """host threads of the CPU baseline: the cores this process may run on,
-    at most 16 (a GPU box's CPU share per GPU)"""
+    at most 16 (the host's CPU share per GPU on MI355X nodes)"""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:

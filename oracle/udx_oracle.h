@@ -1,7 +1,8 @@
 /*
  * CPU oracle of the uDeviceX DPD per-timestep hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library is the reference the HIP path is
+ * compared with:
  * it is loaded by tests/, by the smoke check of __graft_entry__.py and by the
  * cpu_baseline leg of bench.py, never by the product library (ctc_amd/).
  *
@@ -120,8 +121,8 @@ long orc_dpd_forces_f64(const int L[3], int n, const float *pp, const int *start
 /* The same forces with the reference's traversal: a half shell inside the
  * box (each pair once, reaction to the smaller index), one-sided pairs across
  * the periodic boundary; per-thread force buffers added in thread order.
- * The CPU baseline's force pass (not a checker: its FP32 sums are in another
- * order).  Returns ordered pairs as orc_dpd_forces. */
+ * The CPU baseline's force pass (not a parity reference: its FP32 sums are
+ * in another order).  Returns ordered pairs as orc_dpd_forces. */
 long orc_dpd_forces_half(const int L[3], int n, const float *pp, const int *starts, const int *counts,
                          orc_dpd p, const float seeds[27], int nthreads, float *ff);
 

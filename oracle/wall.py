@@ -214,7 +214,7 @@ def flu_wall_step(L, pp, params, seeds28, dt, sdf, wpp, wstarts, mass=1.0, accel
     """One wall-bounded solvent step: clist, DPD forces, wall forces, kick-drift,
     bounce-back, wrap.  Returns (cell-ordered new pp, solvent pairs, wall pairs).
     half=True: the solvent forces by the reference's half-shell traversal
-    (the CPU baseline; not a checker)."""
+    (the CPU baseline; not a parity reference)."""
     from . import oracle as O
     starts, counts, srt, _ = O.clist_build(L, pp)
     forces = O.dpd_forces_half if half else O.dpd_forces
