@@ -235,14 +235,6 @@ __device__ __forceinline__ PairIn pair_in_wrapped(const DpdP &P, unsigned ent, i
     return o;
 }
 
-// The hardware hands workgroups to the 8 XCDs round robin; renumbering them
-// gives each XCD one contiguous run of cells, so neighbour rows are reused
-// from that XCD's L2 instead of being fetched again by the others.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int per = nb >> 3, rem = nb & 7, x = b & 7, k = b >> 3;
-    return x * per + (x < rem ? x : rem) + k;
-}
-
 // Candidate window along x.  With x-slab starts a row run [x0, x1] of cells
 // (wrap wx) is cut to the slabs that can hold a particle within the cutoff:
 // a candidate at y-z distance >= d from the particle has |dx| < sqrt(1 - d^2).
@@ -848,7 +840,7 @@ __global__ __launch_bounds__(DPD_THREADS) __attribute__((amdgpu_waves_per_eu(5))
 // src/fluforces/bulk/imp/main.h:42-53)
 template <int WR>
 __global__ __launch_bounds__(DPD_THREADS) void k_dpd_stress(DpdK A) {
-    dpd_block<WR, true, DPD_THREADS, QCAP, SCAP, true>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS);
+    dpd_block<WR, true, DPD_THREADS, QCAP, SCAP, true>(A, blockIdx.x * DPD_THREADS);
 }
 
 // Many boxes in one launch: block b belongs to the box k with
@@ -1183,13 +1175,15 @@ __device__ __forceinline__ void dpd_half_block(const DpdK &A, int first, HsWave 
 template <int WR>
 __global__ __launch_bounds__(DPD_THREADS) void k_dpd_half(DpdK A) {
     __shared__ HsWave lds[DPD_THREADS / 64];
-    dpd_half_block<WR, DPD_THREADS>(A, xcd_block(blockIdx.x, gridDim.x) * DPD_THREADS, lds);
+    // blocks in launch order (dealt to the XCDs round robin): 4% faster on the
+    // 96^3 wall box than contiguous eighths of the cells per XCD
+    dpd_half_block<WR, DPD_THREADS>(A, blockIdx.x * DPD_THREADS, lds);
 }
 
 template <int WR>
 __global__ __launch_bounds__(DPD_THREADS) void k_dpd_half_batch(const DpdK *boxes, const int *bblock, int nbox) {
     __shared__ HsWave lds[DPD_THREADS / 64];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int blk = blockIdx.x;   // launch order, as k_dpd_batch
     int lo = 0, hi = nbox;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
