@@ -75,6 +75,24 @@ def test_adjacency_rings():
                 assert (b, a, m.adj1[v * m.md + k]) in faces
             mm = m.map(v * m.md)
             assert mm[1] == ring[0] and mm[2] == ring[1] and mm[3] == ring[2 % deg[v]]
+            # every slot: i1, i2, i3 are the slot's neighbour and its two
+            # cyclic successors (also across the end of a padded ring), i4
+            # the vertex across (i0, i1); padded slots are invalid; the
+            # second cell's slots carry its vertex offset
+            # (src/rbc/adj/imp/map.h:2-27 hst0)
+            for c in (0, 1):
+                for j in range(m.md):
+                    mm = m.map((c * m.nv + v) * m.md + j)
+                    if j >= deg[v]:
+                        assert mm is None
+                        continue
+                    want = [c * m.nv + v] + [c * m.nv + x for x in
+                                             (ring[j], ring[(j + 1) % deg[v]], ring[(j + 2) % deg[v]],
+                                              m.adj1[v * m.md + j])] + [c]
+                    assert list(mm) == want, (nv, v, c, j)
+        # both ring ends occur: full rings (j + 1 wraps by the modulus) and
+        # padded ones (wraps at the first -1)
+        assert (deg == m.md).any() and (deg < m.md).any()
 
 
 def minsin_per_vertex(m, pp):
