@@ -753,13 +753,15 @@ static int flu_step(udx_flu *f, const float *seeds_dev, const float *fsi_seeds, 
 }
 
 namespace {
-// the seeds of step *step from the run's table, and the step counter
-__global__ void k_flu_seeds(const float *table, const int *step, float *cur) {
+// the seeds of step *step from the run's table, then the counter advanced to
+// the next step (one wave: every lane reads the counter before lane 0 writes
+// it; nothing else in the step reads it)
+__global__ void k_flu_seeds(const float *table, int *step, float *cur) {
     const int k = threadIdx.x;
-    if (k < NSEED) cur[k] = table[(long)*step * NSEED + k];
-}
-__global__ void k_flu_next(int *step) {
-    if (threadIdx.x == 0) *step += 1;
+    const int st = *step;
+    if (k < NSEED) cur[k] = table[(long)st * NSEED + k];
+    __syncthreads();
+    if (k == 0) *step = st + 1;
 }
 }  // namespace
 
@@ -791,10 +793,6 @@ static int flu_capture(udx_flu *f) {
         hipLaunchKernelGGL(k_flu_seeds, dim3(1), dim3(64), 0, f->gs, f->seeds_dev, f->step_dev, f->seeds_cur);
         st = hipGetLastError() == hipSuccess ? UDX_OK : UDX_EHIP;
         if (!st) st = flu_step(f, f->seeds_cur, nullptr, nullptr);
-        if (!st) {
-            hipLaunchKernelGGL(k_flu_next, dim3(1), dim3(64), 0, f->gs, f->step_dev);
-            st = hipGetLastError() == hipSuccess ? UDX_OK : UDX_EHIP;
-        }
     }
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(f->gs, &g);

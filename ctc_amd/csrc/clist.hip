@@ -29,7 +29,8 @@ __device__ __forceinline__ int wave_inclusive_scan(int v) {
     return v;
 }
 
-// block-wide inclusive scan of one value per thread (blockDim = 256)
+// block-wide inclusive scan of one value per thread (lds_waves: one entry
+// per wave of the block)
 __device__ __forceinline__ int block_inclusive_scan(int v, int *lds_waves) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     v = wave_inclusive_scan(v);
@@ -86,6 +87,58 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, con
     if (blockIdx.x == nb - 1 && threadIdx.x == SCAN_THREADS - 1) out[n] = tile0 + inc;
 }
 
+// The whole exclusive scan in one block when n <= SCAN1_MAX (the two-kernel
+// path costs a second launch, which dominates for the small arrays of a small
+// box: slab counts of 16^3 cells, fragment counts, membrane and contact
+// tables).  Wave w owns the entries [1024 w, 1024 (w + 1)) in 16 rounds of 64
+// consecutive entries (lane l takes entry 64 k + l of round k), so every load
+// and store is coalesced; the 16 rounds are scanned side by side across the
+// wave, chained by their totals, and the waves by one pass through LDS.
+constexpr int SCAN1_THREADS = 1024;
+constexpr int SCAN1_ITEMS = 16;
+constexpr int SCAN1_MAX = SCAN1_THREADS * SCAN1_ITEMS;
+
+__global__ __launch_bounds__(SCAN1_THREADS) void k_scan_one(int *in, int n, int *out, int zero) {
+    __shared__ int lw[SCAN1_THREADS / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int base = wave * 64 * SCAN1_ITEMS + lane;
+    int x[SCAN1_ITEMS], v[SCAN1_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; k++) {
+        const int i = base + 64 * k;
+        x[k] = i < n ? in[i] : 0;
+        v[k] = x[k];
+    }
+    if (zero)
+#pragma unroll
+        for (int k = 0; k < SCAN1_ITEMS; k++)
+            if (base + 64 * k < n) in[base + 64 * k] = 0;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+        for (int k = 0; k < SCAN1_ITEMS; k++) {
+            const int t = __shfl_up(v[k], d, 64);
+            if (lane >= d) v[k] += t;
+        }
+    int carry = 0;   // the wave's entries before round k
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; k++) {
+        const int tot = __builtin_amdgcn_readlane(v[k], 63);
+        v[k] += carry;
+        carry += tot;
+    }
+    if (lane == 0) lw[wave] = carry;
+    __syncthreads();
+    int add = 0;
+    for (int w = 0; w < wave; w++) add += lw[w];
+#pragma unroll
+    for (int k = 0; k < SCAN1_ITEMS; k++) {
+        const int i = base + 64 * k;
+        if (i < n) out[i] = add + v[k] - x[k];
+    }
+    if (threadIdx.x == SCAN1_THREADS - 1) out[n] = add + carry;
+}
+
 __global__ void k_fill_int(int *p, int n, int v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -113,19 +166,18 @@ __global__ __launch_bounds__(256) void k_clist_count(Pos pos, int n, int Lx, int
     clist_count_one(i < n, i, r, Lx, Ly, Lz, sub, counts, cellslot);
 }
 
-// cell tables from the slab starts: starts[c] = bstarts[c sub]
-__global__ __launch_bounds__(256) void k_cell_tables(int ncells, int sub, const int *bstarts, int *starts,
-                                                     int *counts) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c > ncells) return;
-    const int s0 = bstarts[c * sub];
-    starts[c] = s0;
-    if (c < ncells) counts[c] = bstarts[(c + 1) * sub] - s0;
-}
-
+// slot -> order; with x slabs (sub > 1) threads i <= ncells also write the
+// cell tables from the slab starts, cstarts[c] = starts[c sub] and the counts
+// (they need only the scan, so they take no launch of their own)
 __global__ __launch_bounds__(256) void k_clist_scatter(int n, const int2 *cellslot, const int *starts,
-                                                       int *order) {
+                                                       int *order, int ncells, int sub, int *cstarts,
+                                                       int *ccounts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sub > 1 && i <= ncells) {
+        const int s0 = starts[i * sub];
+        cstarts[i] = s0;
+        if (i < ncells) ccounts[i] = starts[(i + 1) * sub] - s0;
+    }
     if (i >= n) return;
     int2 cs = cellslot[i];
     if (cs.x >= 0) order[starts[cs.x] + cs.y] = i;
@@ -228,6 +280,11 @@ int scan_scratch_size(int n) {
 }
 
 static int scan_impl(hipStream_t s, int *in, int n, int *out, int *scratch, int zero) {
+    if (n <= SCAN1_MAX) {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN1_THREADS), 0, s, in, n, out, zero);
+        UDX_LAUNCHED();
+        return UDX_OK;
+    }
     int nb = ceil_div(n, SCAN_TILE);
     if (nb == 0) nb = 1;
     int *part = scratch;
@@ -309,17 +366,16 @@ int clist_finish(hipStream_t s, ClistBufs *c, int n) {
     // cells come from the slab starts); without slabs counts[] is the output
     if (c->sub > 1) UDX_TRY(scan_exclusive_consume(s, c->bcounts, nb, c->bstarts, c->scan_tmp));
     else UDX_TRY(scan_exclusive(s, c->bcounts, nb, c->bstarts, c->scan_tmp));
-    if (n > 0) {
-        hipLaunchKernelGGL(k_clist_scatter, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c->cellslot,
-                           c->bstarts, c->order);
-        UDX_LAUNCHED();
-        hipLaunchKernelGGL(k_clist_cellsort, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, c->bstarts,
-                           c->order);
+    // one launch covers the particles and, with x slabs, the nc + 1 cell entries
+    const int nt = c->sub > 1 ? (n > nc + 1 ? n : nc + 1) : n;
+    if (nt > 0) {
+        hipLaunchKernelGGL(k_clist_scatter, dim3(ceil_div(nt, 256)), dim3(256), 0, s, n, c->cellslot,
+                           c->bstarts, c->order, nc, c->sub, c->starts, c->counts);
         UDX_LAUNCHED();
     }
-    if (c->sub > 1) {
-        hipLaunchKernelGGL(k_cell_tables, dim3(ceil_div(nc + 1, 256)), dim3(256), 0, s, nc, c->sub, c->bstarts,
-                           c->starts, c->counts);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_clist_cellsort, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, c->bstarts,
+                           c->order);
         UDX_LAUNCHED();
     }
     return UDX_OK;
