@@ -306,6 +306,42 @@ const int *udx_clist_starts(const udx_clist *c) { return c->c.starts; }
 const int *udx_clist_counts(const udx_clist *c) { return c->c.counts; }
 const int *udx_clist_order(const udx_clist *c) { return c->c.order; }
 
+// ------------------------------------------------------------------ scan
+// scan_ini / scan_apply / scan_fin (src/algo/scan/imp.h:4-6; the CUDA
+// version src/algo/scan/cuda/imp.h:9-31 checks size against the workspace)
+struct udx_scan {
+    udx_ctx *ctx;
+    int size;
+    int *tmp;
+};
+
+int udx_scan_ini(udx_ctx *ctx, int size, udx_scan **w) {
+    UDX_CHECK(ctx && w && size >= 0, UDX_EINVAL, "udx_scan_ini: bad arguments");
+    udx_scan *q = new udx_scan{ctx, size, nullptr};
+    const int st = dev_alloc(&q->tmp, (size_t)scan_scratch_size(size));
+    if (st) {
+        delete q;
+        return st;
+    }
+    *w = q;
+    return UDX_OK;
+}
+
+int udx_scan_apply(udx_scan *w, const int *input, int size, int *output) {
+    UDX_CHECK(w && size >= 0, UDX_EINVAL, "udx_scan_apply: bad arguments");
+    UDX_CHECK(size <= w->size, UDX_ECAP, "udx_scan_apply: size = %d > w->size = %d", size, w->size);
+    if (size == 0) return UDX_OK;
+    UDX_CHECK(input && output, UDX_EINVAL, "udx_scan_apply: null array");
+    return scan_exclusive_n(w->ctx->stream, input, size, output, w->tmp);
+}
+
+int udx_scan_fin(udx_scan *w) {
+    if (!w) return UDX_OK;
+    dev_free(w->tmp);
+    delete w;
+    return UDX_OK;
+}
+
 // ------------------------------------------------------------------ fluforces
 int udx_fluforces_ini(udx_ctx *ctx, int Lx, int Ly, int Lz, int maxp, udx_fluforces **out) {
     UDX_CHECK(ctx && out, UDX_EINVAL, "udx_fluforces_ini: null argument");

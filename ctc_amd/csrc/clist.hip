@@ -56,9 +56,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int *in, int
 // the tile's own block (integers, so the order of the additions does not
 // matter): no launch between the two passes.  zero: the input is cleared
 // after it is read (counts consumed by the scan are ready for the next
-// counting pass).  out[n] = the total.
+// counting pass).  total: out[n] = the total.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, const int *part, int nb, int *out,
-                                                             int zero) {
+                                                             int zero, int total) {
     __shared__ int lw[4];
     __shared__ int tile0;
     int pre = 0;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(int *in, int n, con
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
-    if (blockIdx.x == nb - 1 && threadIdx.x == SCAN_THREADS - 1) out[n] = tile0 + inc;
+    if (total && blockIdx.x == nb - 1 && threadIdx.x == SCAN_THREADS - 1) out[n] = tile0 + inc;
 }
 
 // The whole exclusive scan in one block when n <= SCAN1_MAX (the two-kernel
@@ -98,7 +98,7 @@ constexpr int SCAN1_THREADS = 1024;
 constexpr int SCAN1_ITEMS = 16;
 constexpr int SCAN1_MAX = SCAN1_THREADS * SCAN1_ITEMS;
 
-__global__ __launch_bounds__(SCAN1_THREADS) void k_scan_one(int *in, int n, int *out, int zero) {
+__global__ __launch_bounds__(SCAN1_THREADS) void k_scan_one(int *in, int n, int *out, int zero, int total) {
     __shared__ int lw[SCAN1_THREADS / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int base = wave * 64 * SCAN1_ITEMS + lane;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(SCAN1_THREADS) void k_scan_one(int *in, int n, int 
         const int i = base + 64 * k;
         if (i < n) out[i] = add + v[k] - x[k];
     }
-    if (threadIdx.x == SCAN1_THREADS - 1) out[n] = add + carry;
+    if (total && threadIdx.x == SCAN1_THREADS - 1) out[n] = add + carry;
 }
 
 __global__ void k_fill_int(int *p, int n, int v) {
@@ -279,9 +279,9 @@ int scan_scratch_size(int n) {
     return 2 * nb + 2;
 }
 
-static int scan_impl(hipStream_t s, int *in, int n, int *out, int *scratch, int zero) {
+static int scan_impl(hipStream_t s, int *in, int n, int *out, int *scratch, int zero, int total = 1) {
     if (n <= SCAN1_MAX) {
-        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN1_THREADS), 0, s, in, n, out, zero);
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(SCAN1_THREADS), 0, s, in, n, out, zero, total);
         UDX_LAUNCHED();
         return UDX_OK;
     }
@@ -290,7 +290,7 @@ static int scan_impl(hipStream_t s, int *in, int n, int *out, int *scratch, int 
     int *part = scratch;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part);
     UDX_LAUNCHED();
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part, nb, out, zero);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, part, nb, out, zero, total);
     UDX_LAUNCHED();
     return UDX_OK;
 }
@@ -301,6 +301,10 @@ int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch) 
 
 int scan_exclusive_consume(hipStream_t s, int *in, int n, int *out, int *scratch) {
     return scan_impl(s, in, n, out, scratch, 1);
+}
+
+int scan_exclusive_n(hipStream_t s, const int *in, int n, int *out, int *scratch) {
+    return scan_impl(s, (int *)in, n, out, scratch, 0, 0);
 }
 
 int clist_alloc(ClistBufs *c, const Box &b, int maxp, int sub) {

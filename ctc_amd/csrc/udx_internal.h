@@ -69,6 +69,8 @@ int scan_scratch_size(int n);
 int scan_exclusive(hipStream_t s, const int *in, int n, int *out, int *scratch);
 // the same, clearing in[] as it is read
 int scan_exclusive_consume(hipStream_t s, int *in, int n, int *out, int *scratch);
+// out[0..n) only (no total at out[n])
+int scan_exclusive_n(hipStream_t s, const int *in, int n, int *out, int *scratch);
 
 // ------------------------------------------------------------------ cell list
 // Particles are sorted by cell and, inside a cell, by x slab: every cell is

@@ -84,6 +84,9 @@ _SIGS = {
     "udx_clist_counts": (_VP, [_VP]),
     "udx_clist_order": (_VP, [_VP]),
     "udx_clist_build_noalloc": (_I, [_VP, _I, _I, _I, _I, _VP, _VP, _VP, _VP, _VP]),
+    "udx_scan_ini": (_I, [_VP, _I, C.POINTER(_VP)]),
+    "udx_scan_apply": (_I, [_VP, _VP, _I, _VP]),
+    "udx_scan_fin": (_I, [_VP]),
     "udx_fluforces_ini": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_VP)]),
     "udx_fluforces_fin": (_I, [_VP]),
     "udx_fluforces_prepare": (_I, [_VP, _I, _VP]),
@@ -348,6 +351,32 @@ class Clist:
         if self.h:
             lib().udx_clist_fin(self.h)
             self.h = C.c_void_p()
+
+
+# ---------------------------------------------------------------- scan
+class Scan:
+    """scan_ini / scan_apply / scan_fin (src/algo/scan/imp.h:4-6): exclusive
+    scan of device int32 tensors of at most `size` entries."""
+
+    def __init__(self, ctx, size):
+        self.ctx, self.size = ctx, size
+        self.h = C.c_void_p()
+        _ok(lib().udx_scan_ini(ctx.h, size, C.byref(self.h)))
+
+    def apply(self, inp, out, size=None):
+        n = inp.numel() if size is None else size
+        _ok(lib().udx_scan_apply(self.h, _ptr(inp), n, _ptr(out)))
+
+    def close(self):
+        if self.h:
+            lib().udx_scan_fin(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # static working space of udx_clist_build_noalloc (include/udx.h)

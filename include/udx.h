@@ -92,6 +92,18 @@ const int *udx_clist_order(const udx_clist *c);    /* device, input index per ou
 int udx_clist_build_noalloc(udx_ctx *ctx, int Lx, int Ly, int Lz, int n, const float *pp, float *ppout,
                             int *starts, int *counts, int *order);
 
+/* ---- exclusive scan: replaces scan_ini / scan_apply / scan_fin
+ *      (src/algo/scan/imp.h:4-6) ----
+ * output[i] = input[0] + ... + input[i - 1] for 0 <= i < size (device int
+ * arrays, on the context's stream; output[size] is not written).  size above
+ * the capacity given to udx_scan_ini returns UDX_ECAP, as the reference's
+ * scan_apply aborts.  The reference compresses each count to 8 bits (counts
+ * above 255 overflow); here every count is a full int. */
+typedef struct udx_scan udx_scan;
+int udx_scan_ini(udx_ctx *ctx, int size, udx_scan **w);
+int udx_scan_apply(udx_scan *w, const int *input, int size, int *output);
+int udx_scan_fin(udx_scan *w);
+
 /* ---- fluid-fluid DPD forces: replaces fluforces_bulk_ini / prepare / apply
  *      and fluforces_halo_ini / prepare / apply (src/fluforces/imp.h:8-25) ----
  * One periodic domain: pairs across the box boundary are computed in the
